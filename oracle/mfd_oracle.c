@@ -1,0 +1,790 @@
+/*
+ * mfd_oracle.c -- CPU restatement of the numeric phase of the reference
+ * supernodal multifrontal Cholesky (to-mo-ki/multifrontal-DAG), kept in the
+ * reference's own tile storage so that every array can be compared entry by
+ * entry.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle: only tests/, the smoke
+ * check in __graft_entry__.py and the cpu_baseline leg of bench.py may load it,
+ * and only as the checker.  The product path (multifrontal-dag_amd/) never
+ * links, loads or calls it.
+ *
+ * Indexing follows the reference: nodes, tiles and in-tile positions are
+ * 1-based in every formula below; element (r, q) of a column-major (ld, *)
+ * array is stored at [(q-1)*ld + (r-1)].  Sizes and offsets are int64 (the
+ * reference uses int32 intra-node offsets, integer_function_m.f90:14-30).
+ *
+ * Tile BLAS are textbook loops (no BLAS/LAPACK is installed in the image);
+ * their calling conventions are exactly the reference's calls:
+ *   mydpotrf  DPOTRF("U")              factorize_kernel_m.f90:4-9
+ *   mydtrsm   DTRSM("L","U","T","N")   factorize_kernel_m.f90:11-16
+ *   mydsyrk   DSYRK("U","T", -1, 1)    factorize_kernel_m.f90:18-23
+ *   mydgemm   DGEMM("T","N", -1, 1)    factorize_kernel_m.f90:25-33
+ *   mydtrsv_l DTRSV("U","T","N")       solve_kernel_m.f90:4-9
+ *   mydtrsv_u DTRSV("U","N","N")       solve_kernel_m.f90:11-16
+ *   mydgemv_t DGEMV("T", -1, 1)        solve_kernel_m.f90:18-23
+ *   mydgemv_n DGEMV("N", -1, 1)        solve_kernel_m.f90:25-30
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef int64_t i64;
+
+/* ---------------------------------------------------------------- integers */
+/* integer_function_m.f90:14-30 */
+static i64 psum1(i64 e) { return e < 1 ? 0 : e * (e + 1) / 2; }
+static i64 psum2(i64 s, i64 e) { return e < s ? 0 : psum1(e) - psum1(s - 1); }
+/* integer_function_m.f90:45-48: Fortran integer division truncates toward 0 */
+static int div_ceiling(int a, int p) { return (a - 1) / p + 1; }
+static int mod2(int a, int p) { return (a - 1) % p + 1; }
+/* block_size_calculator_m.f90:12-21 */
+static int block_size(int idx, int nb, int n) { return (i64)idx * nb > n ? n % nb : nb; }
+
+/* ------------------------------------------------------------ tile kernels */
+#define CM(a, ld, r, q) (a)[((i64)(q) - 1) * (ld) + ((r) - 1)]
+
+/* A = U^T U on the upper triangle of a column-major n x n array (lda = n). */
+void orc_dpotrf_u(int n, double *a, int *info)
+{
+    *info = 0;
+    for (int j = 1; j <= n; ++j) {
+        double s = CM(a, n, j, j);
+        for (int k = 1; k < j; ++k) s -= CM(a, n, k, j) * CM(a, n, k, j);
+        if (!(s > 0.0)) { *info = j; return; }
+        double d = sqrt(s);
+        CM(a, n, j, j) = d;
+        for (int i = j + 1; i <= n; ++i) {
+            double t = CM(a, n, j, i);
+            for (int k = 1; k < j; ++k) t -= CM(a, n, k, j) * CM(a, n, k, i);
+            CM(a, n, j, i) = t / d;
+        }
+    }
+}
+
+/* B(m x nrhs, ldb = m) <- U^{-T} B with U upper (m x m, lda = m). */
+void orc_dtrsm_lutn(int m, int nrhs, const double *u, double *b)
+{
+    for (int c = 1; c <= nrhs; ++c)
+        for (int i = 1; i <= m; ++i) {
+            double t = CM(b, m, i, c);
+            for (int k = 1; k < i; ++k) t -= CM(u, m, k, i) * CM(b, m, k, c);
+            CM(b, m, i, c) = t / CM(u, m, i, i);
+        }
+}
+
+/* C(n x n, upper) -= A^T A with A (k x n, lda = k). */
+void orc_dsyrk_ut(int n, int k, const double *a, double *c)
+{
+    for (int j = 1; j <= n; ++j)
+        for (int i = 1; i <= j; ++i) {
+            double t = 0.0;
+            for (int p = 1; p <= k; ++p) t += CM(a, k, p, i) * CM(a, k, p, j);
+            CM(c, n, i, j) -= t;
+        }
+}
+
+/* C(m x n, ldc = m) -= A^T B with A (k x m), B (k x n), lda = ldb = k. */
+void orc_dgemm_tn(int m, int n, int k, const double *a, const double *b, double *c)
+{
+    for (int j = 1; j <= n; ++j)
+        for (int i = 1; i <= m; ++i) {
+            double t = 0.0;
+            for (int p = 1; p <= k; ++p) t += CM(a, k, p, i) * CM(b, k, p, j);
+            CM(c, m, i, j) -= t;
+        }
+}
+
+/* x <- U^{-T} x */
+void orc_dtrsv_ut(int n, const double *u, double *x)
+{
+    for (int i = 1; i <= n; ++i) {
+        double t = x[i - 1];
+        for (int k = 1; k < i; ++k) t -= CM(u, n, k, i) * x[k - 1];
+        x[i - 1] = t / CM(u, n, i, i);
+    }
+}
+
+/* x <- U^{-1} x */
+void orc_dtrsv_un(int n, const double *u, double *x)
+{
+    for (int i = n; i >= 1; --i) {
+        double t = x[i - 1];
+        for (int k = i + 1; k <= n; ++k) t -= CM(u, n, i, k) * x[k - 1];
+        x[i - 1] = t / CM(u, n, i, i);
+    }
+}
+
+/* y(n) -= A^T x with A (m x n, lda = m) */
+void orc_dgemv_t(int m, int n, const double *a, const double *x, double *y)
+{
+    for (int j = 1; j <= n; ++j) {
+        double t = 0.0;
+        for (int i = 1; i <= m; ++i) t += CM(a, m, i, j) * x[i - 1];
+        y[j - 1] -= t;
+    }
+}
+
+/* y(m) -= A x with A (m x n, lda = m) */
+void orc_dgemv_n(int m, int n, const double *a, const double *x, double *y)
+{
+    for (int j = 1; j <= n; ++j)
+        for (int i = 1; i <= m; ++i) y[i - 1] -= CM(a, m, i, j) * x[j - 1];
+}
+
+/* extend_add_kernel_m.f90:6-19: to(col_local(j), row_local(i)) += from(coffset+j, roffset+i) */
+void orc_extend_add_rect(const double *from, double *to, const int *col_local, int ncol,
+                         const int *row_local, int nrow, int ld_from, int ld_to, int coffset, int roffset)
+{
+    for (int i = 1; i <= nrow; ++i)
+        for (int j = 1; j <= ncol; ++j)
+            CM(to, ld_to, col_local[j - 1], row_local[i - 1]) += CM(from, ld_from, coffset + j, roffset + i);
+}
+
+/* extend_add_kernel_m.f90:21-34: lower triangle of one index group */
+void orc_extend_add_tri(const double *from, double *to, const int *local, int nloc,
+                        int ld_from, int ld_to, int offset)
+{
+    for (int i = 1; i <= nloc; ++i)
+        for (int j = 1; j <= i; ++j)
+            CM(to, ld_to, local[j - 1], local[i - 1]) += CM(from, ld_from, offset + j, offset + i);
+}
+
+/* matrix_splitter_m.f90:5-29 */
+void orc_split_tri(const double *origin, double *left, double *right, int lsize, int rsize)
+{
+    int lo = lsize + rsize;
+    for (int i = 1; i <= lsize; ++i)
+        for (int j = 1; j <= i; ++j) CM(left, lsize, j, i) = CM(origin, lo, j, i);
+    for (int i = lsize + 1; i <= lo; ++i)
+        for (int j = 1; j <= lsize; ++j) CM(left, lsize, j, i) = CM(origin, lo, j, i);
+    for (int i = 1; i <= rsize; ++i)
+        for (int j = 1; j <= i; ++j) CM(right, rsize, j, i) = CM(origin, lo, j + lsize, i + lsize);
+}
+
+/* matrix_splitter_m.f90:31-46 */
+void orc_split_rect(const double *origin, double *left, double *right, int lsize, int rsize, int nrow)
+{
+    int lo = lsize + rsize;
+    for (int i = 1; i <= nrow; ++i)
+        for (int j = 1; j <= lsize; ++j) CM(left, lsize, j, i) = CM(origin, lo, j, i);
+    for (int i = 1; i <= nrow; ++i)
+        for (int j = 1; j <= rsize; ++j) CM(right, rsize, j, i) = CM(origin, lo, lsize + j, i);
+}
+
+/* ------------------------------------------------------------- node data */
+/* node_data_m.f90:8-184 */
+typedef struct {
+    int num_node, nb;
+    int *c, *w, *parent;          /* supernode_size, work_size, parent (1-based; 0 = root) */
+    int *ss, *wsz;                /* border_supernode_size, border_work_size */
+    i64 *first_col;               /* first global (postordered) column of each node, 1-based */
+    /* factors: three packed arrays (factors_m.f90:26-36) */
+    i64 *sn_off, *wk_off, *bd_off;  /* per-node start (0-based) into each array, num_node+1 */
+    double *sn, *wk, *bd;
+    /* right-hand side arrays (right_hand_m.f90:24-35) */
+    i64 *rsn_off, *rwk_off, *rbd_off;
+    double *rsn, *rwk, *rbd;
+    /* relative indices (block_local_index_info_m.f90) */
+    i64 *li_off;                  /* per-node start into lidx, num_node+1 */
+    int *lidx;                    /* local_index values (parent front positions) */
+    int *blidx;                   /* same values after rebuild_val (mod2(.,nb)) */
+    i64 *grp_off;                 /* per-node start into the group arrays, num_node+1 */
+    int *grp_start;               /* group start row (1-based) within the node's list */
+    int *grp_len, *grp_parent, *grp_child;
+} orc_t;
+
+static int n_of(const orc_t *h, int s) { return h->c[s] + h->w[s]; }
+static int num_matrix_block(const orc_t *h, int s) { return div_ceiling(n_of(h, s), h->nb); }
+static int work_start(const orc_t *h, int s)
+{
+    int ws = h->c[s] / h->nb + 1;
+    if (h->w[s] == 0) ws = div_ceiling(h->c[s], h->nb) + 1;
+    return ws;
+}
+static int matrix_block_size(const orc_t *h, int idx, int s) { return block_size(idx, h->nb, n_of(h, s)); }
+static int supernode_block_size(const orc_t *h, int idx, int s) { return block_size(idx, h->nb, h->c[s]); }
+static int work_block_size(const orc_t *h, int idx, int s)
+{
+    return idx == work_start(h, s) ? h->wsz[s] : block_size(idx, h->nb, n_of(h, s));
+}
+static int divisible(const orc_t *h, int s) { return h->ss[s] == 0; }
+static int exist_border(const orc_t *h, int s) { return !divisible(h, s) && h->w[s] != 0; }
+enum { EX_SUPERNODE = 1, EX_BORDER = 2, EX_WORK = 3 };
+static int extractor_type(const orc_t *h, int idx, int s)
+{
+    int ws = work_start(h, s);
+    if (divisible(h, s)) return idx <= ws - 1 ? EX_SUPERNODE : EX_WORK;
+    if (idx < ws) return EX_SUPERNODE;
+    if (idx > ws) return EX_WORK;
+    return EX_BORDER;
+}
+
+/* ----------------------------------------------------- factor extractors */
+/* supernode_matrix_extractor_m.f90:14-54 */
+static i64 sn_node_size(const orc_t *h, int s)
+{
+    int nb = h->nb, c = h->c[s], w = h->w[s];
+    return psum2(w + 1, c + w) + psum1(nb - 1) * (c / nb) + psum1(h->ss[s] - 1);
+}
+static i64 sn_tile_pos(const orc_t *h, int s, int i, int j)
+{
+    int n = n_of(h, s), nb = h->nb, col = supernode_block_size(h, j, s);
+    i64 left = psum2((i64)n - (i64)(j - 1) * nb + 1, n) + psum1(nb - 1) * (j - 1);
+    return left + (i64)(i - j) * nb * col;   /* 0-based */
+}
+/* work_matrix_extractor_m.f90:14-74 */
+static i64 wk_node_size(const orc_t *h, int s)
+{
+    int nb = h->nb, w = h->w[s];
+    int last = num_matrix_block(h, s);
+    int lower_rows = matrix_block_size(h, last, s);
+    int fw = h->wsz[s];
+    int wn = (num_matrix_block(h, s) - work_start(h, s) + 1) - 2;
+    return (i64)w * fw + psum1(wn) * nb * nb + (i64)(w - fw) * lower_rows;
+}
+static i64 wk_tile_pos(const orc_t *h, int s, int i, int j)
+{
+    int nb = h->nb, c = h->c[s], w = h->w[s], fw = h->wsz[s];
+    int ws = work_start(h, s);
+    int colw = work_block_size(h, j, s);
+    i64 up_rows, left;
+    if (j == ws) {
+        i64 t = (i64)nb * (i - 1) - c;
+        up_rows = t > 0 ? t : 0;
+        left = 0;
+    } else {
+        up_rows = (i64)(i - j) * nb;
+        int wn = j - ws - 1;
+        left = psum2((i64)c + w - (i64)(j - 1) * nb + 1, w) + psum1(nb - 1) * wn + psum1(fw - 1);
+    }
+    return left + colw * up_rows;
+}
+/* border_matrix_extractor_m.f90:13-54 */
+static i64 bd_node_size(const orc_t *h, int s)
+{
+    if (!exist_border(h, s)) return 0;
+    int nb = h->nb, c = h->c[s], w = h->w[s];
+    int ws = work_start(h, s);
+    return (i64)(c + w - (c / nb) * nb) * matrix_block_size(h, ws, s);
+}
+static i64 bd_tile_pos(const orc_t *h, int s, int i, int j) { return (i64)(i - j) * h->nb * h->nb; }
+
+static double *get_supernode(orc_t *h, int s, int i, int j) { return h->sn + h->sn_off[s] + sn_tile_pos(h, s, i, j); }
+static double *get_work(orc_t *h, int s, int i, int j) { return h->wk + h->wk_off[s] + wk_tile_pos(h, s, i, j); }
+static double *get_border(orc_t *h, int s, int i, int j) { return h->bd + h->bd_off[s] + bd_tile_pos(h, s, i, j); }
+/* factors_m.f90:38-55 */
+static double *get_matrix(orc_t *h, int s, int i, int j)
+{
+    switch (extractor_type(h, j, s)) {
+    case EX_SUPERNODE: return get_supernode(h, s, i, j);
+    case EX_BORDER: return get_border(h, s, i, j);
+    default: return get_work(h, s, i, j);
+    }
+}
+
+/* --------------------------------------------------------- rhs extractors */
+/* supernode_array_extractor_m.f90, work_array_extractor_m.f90, border_array_extractor_m.f90 */
+static double *rh_supernode(orc_t *h, int s, int idx) { return h->rsn + h->rsn_off[s] + (i64)(idx - 1) * h->nb; }
+static double *rh_work(orc_t *h, int s, int idx)
+{
+    int ws = work_start(h, s);
+    i64 pos = idx == ws ? 0 : h->wsz[s] + (i64)(idx - ws - 1) * h->nb;
+    return h->rwk + h->rwk_off[s] + pos;
+}
+static double *rh_border(orc_t *h, int s, int idx) { (void)idx; return h->rbd + h->rbd_off[s]; }
+/* right_hand_m.f90:47-64 */
+static double *rh_array(orc_t *h, int s, int idx)
+{
+    switch (extractor_type(h, idx, s)) {
+    case EX_SUPERNODE: return rh_supernode(h, s, idx);
+    case EX_BORDER: return rh_border(h, s, idx);
+    default: return rh_work(h, s, idx);
+    }
+}
+
+/* ------------------------------------------------------------ construction */
+static i64 *prefix(int n, i64 (*f)(const orc_t *, int), const orc_t *h)
+{
+    i64 *off = (i64 *)calloc((size_t)n + 2, sizeof(i64));
+    for (int s = 1; s <= n; ++s) off[s + 1] = off[s] + f(h, s);
+    return off;
+}
+static i64 rsn_size(const orc_t *h, int s) { return h->c[s]; }
+static i64 rwk_size(const orc_t *h, int s) { return h->w[s]; }
+static i64 rbd_size(const orc_t *h, int s) { return exist_border(h, s) ? matrix_block_size(h, work_start(h, s), s) : 0; }
+static i64 li_size(const orc_t *h, int s) { return h->w[s]; }
+
+/*
+ * Build node data, the three factor arrays, the rhs arrays and the
+ * block_local_index structures from the front structure.
+ *   c, w, parent : num_node entries (parent 1-based, 0 for the root)
+ *   lidx         : sum(w) entries, node by node; entry i of node s is the
+ *                  1-based position of update row i of s inside its parent
+ *                  front (create_local_index_m.f90:10-45).  Ignored for roots.
+ * node_data_m.f90:31-52; factors_m.f90:26-36; right_hand_m.f90:24-35;
+ * block_local_index_info_m.f90:29-56; block_local_index_creator_m.f90:13-122.
+ */
+orc_t *orc_create(int num_node, const int *c, const int *w, const int *parent, int nb, const int *lidx)
+{
+    orc_t *h = (orc_t *)calloc(1, sizeof(orc_t));
+    h->num_node = num_node;
+    h->nb = nb;
+    h->c = (int *)calloc((size_t)num_node + 1, sizeof(int));
+    h->w = (int *)calloc((size_t)num_node + 1, sizeof(int));
+    h->parent = (int *)calloc((size_t)num_node + 1, sizeof(int));
+    h->ss = (int *)calloc((size_t)num_node + 1, sizeof(int));
+    h->wsz = (int *)calloc((size_t)num_node + 1, sizeof(int));
+    h->first_col = (i64 *)calloc((size_t)num_node + 2, sizeof(i64));
+    h->first_col[1] = 1;
+    for (int s = 1; s <= num_node; ++s) {
+        h->c[s] = c[s - 1];
+        h->w[s] = w[s - 1];
+        h->parent[s] = parent[s - 1];
+        h->ss[s] = h->c[s] % nb;
+        int r = nb - h->ss[s];
+        h->wsz[s] = r < h->w[s] ? r : h->w[s];
+        h->first_col[s + 1] = h->first_col[s] + h->c[s];
+    }
+    h->sn_off = prefix(num_node, sn_node_size, h);
+    h->wk_off = prefix(num_node, wk_node_size, h);
+    h->bd_off = prefix(num_node, bd_node_size, h);
+    h->sn = (double *)calloc((size_t)h->sn_off[num_node + 1] + 1, sizeof(double));
+    h->wk = (double *)calloc((size_t)h->wk_off[num_node + 1] + 1, sizeof(double));
+    h->bd = (double *)calloc((size_t)h->bd_off[num_node + 1] + 1, sizeof(double));
+    h->rsn_off = prefix(num_node, rsn_size, h);
+    h->rwk_off = prefix(num_node, rwk_size, h);
+    h->rbd_off = prefix(num_node, rbd_size, h);
+    h->rsn = (double *)calloc((size_t)h->rsn_off[num_node + 1] + 1, sizeof(double));
+    h->rwk = (double *)calloc((size_t)h->rwk_off[num_node + 1] + 1, sizeof(double));
+    h->rbd = (double *)calloc((size_t)h->rbd_off[num_node + 1] + 1, sizeof(double));
+
+    /* local index and its tile-aligned groups */
+    h->li_off = prefix(num_node, li_size, h);
+    i64 nli = h->li_off[num_node + 1];
+    h->lidx = (int *)calloc((size_t)nli + 1, sizeof(int));
+    h->blidx = (int *)calloc((size_t)nli + 1, sizeof(int));
+    memcpy(h->lidx + 1, lidx, (size_t)nli * sizeof(int));
+    h->grp_off = (i64 *)calloc((size_t)num_node + 2, sizeof(i64));
+    /* count groups: create_num_blocks, block_local_index_creator_m.f90:13-44
+       (the last node is the root and is skipped, as in the reference) */
+    for (int s = 1; s <= num_node; ++s) {
+        i64 ng = 0;
+        if (s < num_node) {
+            int prev_p = 0, prev_c = 0;
+            for (int i = 1; i <= h->w[s]; ++i) {
+                int pn = (h->lidx[h->li_off[s] + i] - 1) / nb + 1;
+                int cn = (h->c[s] + i - 1) / nb + 1;
+                if (pn != prev_p || cn != prev_c) { prev_p = pn; prev_c = cn; ++ng; }
+            }
+        }
+        h->grp_off[s + 1] = h->grp_off[s] + ng;
+    }
+    i64 ngt = h->grp_off[num_node + 1];
+    h->grp_start = (int *)calloc((size_t)ngt + 1, sizeof(int));
+    h->grp_len = (int *)calloc((size_t)ngt + 1, sizeof(int));
+    h->grp_parent = (int *)calloc((size_t)ngt + 1, sizeof(int));
+    h->grp_child = (int *)calloc((size_t)ngt + 1, sizeof(int));
+    /* create_num_indices / create_block_nums, block_local_index_creator_m.f90:46-122 */
+    for (int s = 1; s < num_node; ++s) {
+        int prev_p = 0, prev_c = 0;
+        i64 g = h->grp_off[s];
+        for (int i = 1; i <= h->w[s]; ++i) {
+            int pn = (h->lidx[h->li_off[s] + i] - 1) / nb + 1;
+            int cn = (h->c[s] + i - 1) / nb + 1;
+            if (pn != prev_p || cn != prev_c) {
+                prev_p = pn; prev_c = cn; ++g;
+                h->grp_start[g] = i;
+                h->grp_parent[g] = pn;
+                h->grp_child[g] = cn;
+            }
+            h->grp_len[g]++;
+        }
+    }
+    /* rebuild_val: in-place mod2 (block_local_index_creator_m.f90:80-89) */
+    for (i64 k = 1; k <= nli; ++k) h->blidx[k] = mod2(h->lidx[k], nb);
+    return h;
+}
+
+void orc_destroy(orc_t *h)
+{
+    if (!h) return;
+    free(h->c); free(h->w); free(h->parent); free(h->ss); free(h->wsz); free(h->first_col);
+    free(h->sn_off); free(h->wk_off); free(h->bd_off); free(h->sn); free(h->wk); free(h->bd);
+    free(h->rsn_off); free(h->rwk_off); free(h->rbd_off); free(h->rsn); free(h->rwk); free(h->rbd);
+    free(h->li_off); free(h->lidx); free(h->blidx); free(h->grp_off);
+    free(h->grp_start); free(h->grp_len); free(h->grp_parent); free(h->grp_child);
+    free(h);
+}
+
+/* block_local_index_info_m.f90:71-85 */
+static int block_offset(const orc_t *h, int s, int g)
+{
+    int r = h->grp_start[h->grp_off[s] + g];
+    if (r <= h->wsz[s]) return r - 1;
+    return (h->ss[s] + r - 1) % h->nb;
+}
+static const int *group_local(const orc_t *h, int s, int g)
+{
+    return h->blidx + h->li_off[s] + h->grp_start[h->grp_off[s] + g];
+}
+
+/* ------------------------------------------------------------- assembly */
+/* zero_setter_m.f90:12-31 with set_zero_kernel_m.f90:6-25: zero every tile's
+   lower (row-major) part; coefficient_setter_m.f90:15-50: scatter A.
+   colptr (N+1, 0-based offsets), rows (front-relative 1-based rows =
+   supernodal_index), vals: the postordered lower CCS of A, column-major over
+   the global postordered columns. */
+void orc_assemble(orc_t *h, const i64 *colptr, const int *rows, const double *vals)
+{
+    int nb = h->nb;
+    for (int s = 1; s <= h->num_node; ++s) {
+        int B = num_matrix_block(h, s);
+        for (int j = 1; j <= B; ++j) {
+            int ncol = matrix_block_size(h, j, s);
+            double *a = get_matrix(h, s, j, j);
+            for (int q = 1; q <= ncol; ++q)
+                for (int r = 1; r <= q; ++r) CM(a, ncol, r, q) = 0.0;
+            for (int i = j + 1; i <= B; ++i) {
+                int nrow = matrix_block_size(h, i, s);
+                a = get_matrix(h, s, i, j);
+                memset(a, 0, sizeof(double) * (size_t)ncol * nrow);
+            }
+        }
+    }
+    for (int s = 1; s <= h->num_node; ++s) {
+        for (i64 jg = h->first_col[s]; jg < h->first_col[s + 1]; ++jg) {
+            int col = (int)(jg - h->first_col[s] + 1);
+            int bcol = div_ceiling(col, nb), bj = mod2(col, nb);
+            for (i64 p = colptr[jg - 1]; p < colptr[jg]; ++p) {
+                int row = rows[p];
+                int brow = div_ceiling(row, nb), bi = mod2(row, nb);
+                int ld = matrix_block_size(h, bcol, s);
+                double *m = get_matrix(h, s, brow, bcol);
+                CM(m, ld, bj, bi) = vals[p];
+            }
+        }
+    }
+}
+
+/* ----------------------------------------------------------- factorize */
+/* factorize_subroutines_m.f90:9-66 */
+static int st_factorize(orc_t *h, int s, int j)
+{
+    int info;
+    orc_dpotrf_u(matrix_block_size(h, j, s), get_matrix(h, s, j, j), &info);
+    return info;
+}
+static void st_solve(orc_t *h, int s, int i, int j)
+{
+    orc_dtrsm_lutn(matrix_block_size(h, j, s), matrix_block_size(h, i, s), get_matrix(h, s, j, j), get_matrix(h, s, i, j));
+}
+static void st_sym_update(orc_t *h, int s, int i, int j)
+{
+    orc_dsyrk_ut(matrix_block_size(h, i, s), matrix_block_size(h, j, s), get_matrix(h, s, i, j), get_matrix(h, s, i, i));
+}
+/* factorize_subroutines_m.f90:51-65; the panel width is passed explicitly
+   (the reference passes upper_n, which equals it whenever k > i) */
+static void st_update(orc_t *h, int s, int up, int lo, int col)
+{
+    int upper_n = matrix_block_size(h, up, s), lower_n = matrix_block_size(h, lo, s);
+    int kw = matrix_block_size(h, col, s);
+    orc_dgemm_tn(upper_n, lower_n, kw, get_matrix(h, s, up, col), get_matrix(h, s, lo, col), get_matrix(h, s, lo, up));
+}
+
+/* seq_factorize_m.f90:31-52 */
+static int supernode_factorize(orc_t *h, int s)
+{
+    int B = num_matrix_block(h, s), ws = work_start(h, s);
+    for (int j = 1; j <= ws - 1; ++j) {
+        int info = st_factorize(h, s, j);
+        if (info) return (int)(h->first_col[s] - 1) + (j - 1) * h->nb + info;
+        for (int i = j + 1; i <= B; ++i) st_solve(h, s, i, j);
+        for (int i = j + 1; i <= B; ++i) {
+            st_sym_update(h, s, i, j);
+            for (int k = i + 1; k <= B; ++k) st_update(h, s, i, k, j);
+        }
+    }
+    return 0;
+}
+
+/* seq_factorize_m.f90:54-79 with rearrange_subroutines_m.f90:12-43 and
+   border_subroutines_m.f90:9-73 / border_kernel_m.f90:9-30 */
+static int border_factorize(orc_t *h, int s)
+{
+    int j = work_start(h, s), B = num_matrix_block(h, s);
+    int ss = h->ss[s], wsz = h->wsz[s];
+    orc_split_tri(get_matrix(h, s, j, j), get_supernode(h, s, j, j), get_work(h, s, j, j), ss, wsz);
+    for (int i = j + 1; i <= B; ++i)
+        orc_split_rect(get_matrix(h, s, i, j), get_supernode(h, s, i, j), get_work(h, s, i, j), ss, wsz,
+                       matrix_block_size(h, i, s));
+    /* border_potrf */
+    double *sup = get_supernode(h, s, j, j);
+    int info;
+    orc_dpotrf_u(ss, sup, &info);
+    if (info) return (int)(h->first_col[s] - 1) + (j - 1) * h->nb + info;
+    orc_dtrsm_lutn(ss, wsz, sup, sup + (i64)ss * ss);
+    orc_dsyrk_ut(wsz, ss, sup + (i64)ss * ss, get_work(h, s, j, j));
+    /* border_trsm */
+    for (int i = j + 1; i <= B; ++i) {
+        int nrow = matrix_block_size(h, i, s);
+        double *sol = get_supernode(h, s, i, j);
+        orc_dtrsm_lutn(ss, nrow, sup, sol);
+        orc_dgemm_tn(wsz, nrow, ss, sup + (i64)ss * ss, sol, get_work(h, s, i, j));
+    }
+    for (int i = j + 1; i <= B; ++i) {
+        int nrow = matrix_block_size(h, i, s);
+        orc_dsyrk_ut(nrow, ss, get_supernode(h, s, i, j), get_matrix(h, s, i, i));
+        for (int k = i + 1; k <= B; ++k)
+            orc_dgemm_tn(matrix_block_size(h, i, s), matrix_block_size(h, k, s), ss, get_supernode(h, s, i, j),
+                         get_supernode(h, s, k, j), get_matrix(h, s, k, i));
+    }
+    return 0;
+}
+
+/* seq_factorize_m.f90:81-98 with extend_add_subroutines_m.f90:13-59 */
+static void extend_add(orc_t *h, int s, int p)
+{
+    int G = (int)(h->grp_off[s + 1] - h->grp_off[s]);
+    for (int j = 1; j <= G; ++j) {
+        i64 gj = h->grp_off[s] + j;
+        int cj = h->grp_child[gj], pj = h->grp_parent[gj];
+        int ldp = matrix_block_size(h, pj, p), ldc = work_block_size(h, cj, s);
+        orc_extend_add_tri(get_work(h, s, cj, cj), get_matrix(h, p, pj, pj), group_local(h, s, j), h->grp_len[gj], ldc,
+                           ldp, block_offset(h, s, j));
+        for (int i = j + 1; i <= G; ++i) {
+            i64 gi = h->grp_off[s] + i;
+            int ci = h->grp_child[gi], pi = h->grp_parent[gi];
+            orc_extend_add_rect(get_work(h, s, ci, cj), get_matrix(h, p, pi, pj), group_local(h, s, j), h->grp_len[gj],
+                                group_local(h, s, i), h->grp_len[gi], ldc, ldp, block_offset(h, s, j),
+                                block_offset(h, s, i));
+        }
+    }
+}
+
+/* seq_factorize_m.f90:13-29.  Returns 0, or the 1-based global (postordered)
+   column at which a pivot was not positive (the reference ignores DPOTRF's
+   info, factorize_subroutines_m.f90:18). */
+int orc_factorize(orc_t *h)
+{
+    for (int s = 1; s <= h->num_node; ++s) {
+        int info = supernode_factorize(h, s);
+        if (info) return info;
+        if (exist_border(h, s)) {
+            info = border_factorize(h, s);
+            if (info) return info;
+        }
+        if (h->parent[s] > 0) extend_add(h, s, h->parent[s]);
+    }
+    return 0;
+}
+
+/* ---------------------------------------------------------------- solve */
+/* seq_forward_m.f90:14-81 */
+static void forward_sweep(orc_t *h)
+{
+    for (int s = 1; s <= h->num_node; ++s) {
+        int B = num_matrix_block(h, s), ws = work_start(h, s);
+        for (int j = 1; j <= ws - 1; ++j) {
+            orc_dtrsv_ut(matrix_block_size(h, j, s), get_supernode(h, s, j, j), rh_array(h, s, j));
+            for (int i = j + 1; i <= B; ++i)
+                orc_dgemv_t(supernode_block_size(h, j, s), matrix_block_size(h, i, s), get_supernode(h, s, i, j),
+                            rh_array(h, s, j), rh_array(h, s, i));
+        }
+        if (exist_border(h, s)) {
+            int j = ws, ncol = supernode_block_size(h, j, s), nrow = matrix_block_size(h, j, s);
+            /* split_array, array_rearrange_kernel_m.f90:8-15 */
+            double *bd = rh_border(h, s, j), *sp = rh_supernode(h, s, j), *wk = rh_work(h, s, j);
+            for (int k = 0; k < ncol; ++k) sp[k] += bd[k];
+            for (int k = 0; k < nrow - ncol; ++k) wk[k] = bd[ncol + k];
+            /* border_dtrsv_l, border_solve_kernel_m.f90:9-16 */
+            double *a = get_supernode(h, s, j, j);
+            orc_dtrsv_ut(ncol, a, sp);
+            orc_dgemv_t(ncol, nrow - ncol, a + (i64)ncol * ncol, sp, wk);
+            for (int i = j + 1; i <= B; ++i)
+                orc_dgemv_t(ncol, matrix_block_size(h, i, s), get_supernode(h, s, i, j), sp, rh_array(h, s, i));
+        }
+        int p = h->parent[s];
+        if (p > 0) {
+            /* scatter_add, sparse_add_subroutines_m.f90:13-30 */
+            int G = (int)(h->grp_off[s + 1] - h->grp_off[s]);
+            for (int g = 1; g <= G; ++g) {
+                i64 gg = h->grp_off[s] + g;
+                const double *x = rh_work(h, s, h->grp_child[gg]) + block_offset(h, s, g);
+                double *y = rh_array(h, p, h->grp_parent[gg]);
+                const int *loc = group_local(h, s, g);
+                for (int k = 0; k < h->grp_len[gg]; ++k) y[loc[k] - 1] += x[k];
+            }
+        }
+    }
+}
+
+/* seq_backward_m.f90:14-131 */
+static void backward_sweep(orc_t *h)
+{
+    for (int s = h->num_node; s >= 1; --s) {
+        int p = h->parent[s];
+        if (p > 0) {
+            /* gather, sparse_add_subroutines_m.f90:32-49 */
+            int G = (int)(h->grp_off[s + 1] - h->grp_off[s]);
+            for (int g = 1; g <= G; ++g) {
+                i64 gg = h->grp_off[s] + g;
+                double *y = rh_work(h, s, h->grp_child[gg]) + block_offset(h, s, g);
+                const double *x = rh_array(h, p, h->grp_parent[gg]);
+                const int *loc = group_local(h, s, g);
+                for (int k = 0; k < h->grp_len[gg]; ++k) y[k] = x[loc[k] - 1];
+            }
+        }
+        int B = num_matrix_block(h, s), ws = work_start(h, s);
+        if (exist_border(h, s)) {
+            int j = ws, ncol = supernode_block_size(h, j, s), nrow = matrix_block_size(h, j, s);
+            double *sp = rh_supernode(h, s, j), *wk = rh_work(h, s, j), *bd = rh_border(h, s, j);
+            for (int i = j + 1; i <= B; ++i)
+                orc_dgemv_n(ncol, matrix_block_size(h, i, s), get_supernode(h, s, i, j), rh_array(h, s, i), sp);
+            double *a = get_supernode(h, s, j, j);
+            orc_dgemv_n(ncol, nrow - ncol, a + (i64)ncol * ncol, wk, sp);
+            orc_dtrsv_un(ncol, a, sp);
+            /* join_array, array_rearrange_kernel_m.f90:17-24 */
+            for (int k = 0; k < ncol; ++k) bd[k] = sp[k];
+            for (int k = 0; k < nrow - ncol; ++k) bd[ncol + k] = wk[k];
+        }
+        for (int j = ws - 1; j >= 1; --j) {
+            for (int i = j + 1; i <= B; ++i)
+                orc_dgemv_n(supernode_block_size(h, j, s), matrix_block_size(h, i, s), get_supernode(h, s, i, j),
+                            rh_array(h, s, i), rh_array(h, s, j));
+            orc_dtrsv_un(matrix_block_size(h, j, s), get_supernode(h, s, j, j), rh_array(h, s, j));
+        }
+    }
+}
+
+/* hppllt.f90:152-173 between the two permutations: bperm (length N, in the
+   postordered supernodal numbering) is overwritten with the solution.
+   right_hand_m.f90:37-45: work and border arrays are zeroed, the supernode
+   array is the permuted vector itself. */
+void orc_solve(orc_t *h, double *bperm)
+{
+    i64 N = h->rsn_off[h->num_node + 1];
+    memcpy(h->rsn, bperm, sizeof(double) * (size_t)N);
+    memset(h->rwk, 0, sizeof(double) * (size_t)h->rwk_off[h->num_node + 1]);
+    memset(h->rbd, 0, sizeof(double) * (size_t)h->rbd_off[h->num_node + 1]);
+    forward_sweep(h);
+    backward_sweep(h);
+    memcpy(bperm, h->rsn, sizeof(double) * (size_t)N);
+}
+/* forward sweep only (seq_forward_test.f90 checks the result after it) */
+void orc_forward(orc_t *h, double *bperm)
+{
+    i64 N = h->rsn_off[h->num_node + 1];
+    memcpy(h->rsn, bperm, sizeof(double) * (size_t)N);
+    memset(h->rwk, 0, sizeof(double) * (size_t)h->rwk_off[h->num_node + 1]);
+    memset(h->rbd, 0, sizeof(double) * (size_t)h->rbd_off[h->num_node + 1]);
+    forward_sweep(h);
+    memcpy(bperm, h->rsn, sizeof(double) * (size_t)N);
+}
+/* backward sweep only (seq_backward_test.f90) */
+void orc_backward(orc_t *h, double *bperm)
+{
+    i64 N = h->rsn_off[h->num_node + 1];
+    memcpy(h->rsn, bperm, sizeof(double) * (size_t)N);
+    memset(h->rwk, 0, sizeof(double) * (size_t)h->rwk_off[h->num_node + 1]);
+    memset(h->rbd, 0, sizeof(double) * (size_t)h->rbd_off[h->num_node + 1]);
+    backward_sweep(h);
+    memcpy(bperm, h->rsn, sizeof(double) * (size_t)N);
+}
+
+/* -------------------------------------------------------------- access */
+i64 orc_array_size(orc_t *h, int which)
+{
+    switch (which) {
+    case 1: return h->sn_off[h->num_node + 1];
+    case 2: return h->wk_off[h->num_node + 1];
+    case 3: return h->bd_off[h->num_node + 1];
+    default: return -1;
+    }
+}
+double *orc_array(orc_t *h, int which)
+{
+    switch (which) {
+    case 1: return h->sn;
+    case 2: return h->wk;
+    case 3: return h->bd;
+    default: return 0;
+    }
+}
+/* 0-based start of tile (i,j) of node s inside array `which`, and its
+   (rows, cols) in the row-major tile sense */
+i64 orc_tile(orc_t *h, int which, int s, int i, int j, int *rows, int *cols)
+{
+    switch (which) {
+    case 1:
+        *rows = matrix_block_size(h, i, s);
+        *cols = supernode_block_size(h, j, s);
+        return h->sn_off[s] + sn_tile_pos(h, s, i, j);
+    case 2:
+        *rows = work_block_size(h, i, s);
+        *cols = work_block_size(h, j, s);
+        return h->wk_off[s] + wk_tile_pos(h, s, i, j);
+    default:
+        *rows = matrix_block_size(h, i, s);
+        *cols = matrix_block_size(h, j, s);
+        return h->bd_off[s] + bd_tile_pos(h, s, i, j);
+    }
+}
+int orc_num_matrix_block(orc_t *h, int s) { return num_matrix_block(h, s); }
+int orc_work_start(orc_t *h, int s) { return work_start(h, s); }
+int orc_num_groups(orc_t *h, int s) { return (int)(h->grp_off[s + 1] - h->grp_off[s]); }
+/* group g of node s: fills start_row, parent tile, child tile, block offset,
+   and copies the tile-local indices into out (must hold len entries) */
+int orc_group(orc_t *h, int s, int g, int *start_row, int *parent_num, int *child_num, int *offset, int *out)
+{
+    i64 gg = h->grp_off[s] + g;
+    *start_row = h->grp_start[gg];
+    *parent_num = h->grp_parent[gg];
+    *child_num = h->grp_child[gg];
+    *offset = block_offset(h, s, g);
+    const int *loc = group_local(h, s, g);
+    for (int k = 0; k < h->grp_len[gg]; ++k) out[k] = loc[k];
+    return h->grp_len[gg];
+}
+
+/* Dense front view of node s after factorization, column-major n x n
+   (ld = n, 0-based front positions): the first c columns hold L, the
+   trailing w x w block holds the update matrix; only r >= q is written. */
+void orc_front(orc_t *h, int s, double *out)
+{
+    int n = n_of(h, s), nb = h->nb, c = h->c[s];
+    int B = num_matrix_block(h, s), ws = work_start(h, s);
+    for (int j = 1; j <= B; ++j) {
+        int et = extractor_type(h, j, s);
+        for (int i = j; i <= B; ++i) {
+            int rows = matrix_block_size(h, i, s);
+            for (int ri = 1; ri <= rows; ++ri) {
+                int r = (i - 1) * nb + ri - 1;
+                if (et == EX_SUPERNODE || et == EX_BORDER) {
+                    int cols = supernode_block_size(h, j, s);
+                    double *t = get_supernode(h, s, i, j);
+                    for (int qj = 1; qj <= cols; ++qj) {
+                        int q = (j - 1) * nb + qj - 1;
+                        if (q > r) break;
+                        out[(i64)q * n + r] = t[(i64)(ri - 1) * cols + (qj - 1)];
+                    }
+                }
+                if (et == EX_WORK || et == EX_BORDER) {
+                    /* work tile (i, j): rows of work block i, cols of work block j */
+                    int wcols = work_block_size(h, j, s);
+                    int wrow0 = (i == ws) ? c : (i - 1) * nb;   /* first front row of work block i */
+                    int wcol0 = (j == ws) ? c : (j - 1) * nb;
+                    if (r < wrow0) continue;
+                    double *t = get_work(h, s, i, j);
+                    int wr = r - wrow0;
+                    for (int qj = 0; qj < wcols; ++qj) {
+                        int q = wcol0 + qj;
+                        if (q > r) break;
+                        out[(i64)q * n + r] = t[(i64)wr * wcols + qj];
+                    }
+                }
+            }
+        }
+    }
+}
