@@ -1,0 +1,163 @@
+/*
+ * mfdag.h -- C-ABI of the MI355X numeric phase of a supernodal multifrontal
+ * Cholesky solver for sparse SPD systems.
+ *
+ * This library replaces the numeric phase behind the reference's Fortran
+ * module `hppllt` (to-mo-ki/multifrontal-DAG, src/main/hppllt.f90:8-9).  The
+ * host keeps the ordering and symbolic analysis (bit-identical to the
+ * reference's); front assembly, extend-add, the partial front factorizations
+ * and the supernodal triangular solves run as HIP kernels on one gfx950 GPU.
+ *
+ * Conventions
+ *   - Index arrays are 1-based int32 as produced by Fortran; sizes are int64.
+ *   - Return 0 on success; > 0 is the 1-based global (postordered) column at
+ *     which a pivot was not positive (the matrix is not SPD: the reference
+ *     ignores this, DPOTRF info at factorize_subroutines_m.f90:18); < 0 is an
+ *     argument, state or HIP error (MFD_ERR_*).  No call aborts or throws.
+ *   - Host pointers are borrowed for the duration of the call only.  Device
+ *     memory and the HIP stream are owned by the handle.  Calls are
+ *     synchronous on return.  One host thread per handle.
+ */
+#ifndef MFDAG_H
+#define MFDAG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mfd_handle mfd_handle;
+
+/* return codes */
+#define MFD_OK 0
+#define MFD_ERR_ARG (-1)
+#define MFD_ERR_STATE (-2)
+#define MFD_ERR_HIP (-3)
+#define MFD_ERR_NOMEM (-4)
+#define MFD_ERR_REDUCIBLE (-5)
+#define MFD_ERR_INTERNAL (-6)
+#define MFD_ERR_NODEVICE (-7)
+#define MFD_ERR_UNSUPPORTED (-8)
+#define MFD_ERR_NUMERIC (-9)
+
+/* precision forms for mfd_factorize */
+#define MFD_FP64 0     /* plain form: FP64 fronts, FP64 solves */
+#define MFD_FP32_IR 1  /* FP32 fronts on MFMA + FP64 iterative refinement */
+
+/* mfd_create flags */
+#define MFD_FLAG_GRAPH 1    /* replay each schedule from a captured hipGraph */
+#define MFD_FLAG_PROFILE 2  /* time every kernel class with HIP events */
+
+/* timer ids: time_list_m.f90:3-10 */
+#define MFD_TIME_ORDERING 1
+#define MFD_TIME_ANALYZE 2
+#define MFD_TIME_FACTORIZE 3
+#define MFD_TIME_SOLVE 4
+#define MFD_TIME_FORWARD 5
+#define MFD_TIME_BACKWARD 6
+#define MFD_TIME_REAL_FACTORIZE 7
+
+/* kernel classes reported by mfd_get_kernel_time */
+#define MFD_K_ASSEMBLE 0
+#define MFD_K_EXTEND_ADD 1
+#define MFD_K_SMALL_FRONT 2
+#define MFD_K_POTRF 3
+#define MFD_K_TRSM 4
+#define MFD_K_UPDATE 5      /* in-panel trailing update (GEMM/SYRK, K = panel width) */
+#define MFD_K_SCHUR 6       /* Schur-complement update (SYRK, K = front's pivot count) */
+#define MFD_K_SOLVE_FWD 7
+#define MFD_K_SOLVE_BWD 8
+#define MFD_K_REFINE 9
+#define MFD_K_NUM 10
+
+/* Handle; nb is the reference's tile size (options(1), options_m.f90:3),
+   used for the reference-layout views (mfd_export_tiles, array sizes). */
+int mfd_create(mfd_handle** h, int nb, int flags);
+void mfd_destroy(mfd_handle* h);
+
+/* hppllt_analyze (hppllt.f90:24-81, create_data_structure :83-106).
+   ccs_col (n+1) / ccs_row: 1-based CCS of the lower triangle with the
+   diagonal first in each column.  perm (nullable, new -> old, 1-based) takes
+   the place of the METIS ordering (use_metis = 1); NULL analyses the matrix
+   as given (use_metis = 0).  Builds the GPU schedule and allocates all device
+   storage.  Unlike the reference, ccs_row is copied, not retained. */
+int mfd_analyze(mfd_handle* h, int n, const int* ccs_col, const int* ccs_row, int max_zero, const int* perm);
+
+/* The structure given directly, as the reference's driver tests build it
+   (seq_factorize_test.f90:23-35): per node supernode_size, work_size and
+   parent (1-based, 0 for the root, postordered), local_index (sum of
+   work_size entries, create_local_index_m.f90:10-45), and the supernodal CCS
+   of A (a_colptr: n+1 0-based offsets over the global postordered columns;
+   supernodal_index: front-relative 1-based rows, create_supernodal_index_m.f90:10-40). */
+int mfd_set_structure(mfd_handle* h, int num_node, const int* supernode_size, const int* work_size,
+                      const int* parent, const int* local_index, const int64_t* a_colptr,
+                      const int* supernodal_index);
+
+/* hppllt_factorize (hppllt.f90:123-150): a holds the values in the caller's
+   CCS order (after mfd_set_structure: in supernodal CCS order). */
+int mfd_factorize(mfd_handle* h, const double* a, int precision);
+/* Same, with the values already in device memory (HBM-resident inputs). */
+int mfd_factorize_device(mfd_handle* h, const double* d_a, int precision);
+
+/* hppllt_solve (hppllt.f90:152-173) for nrhs right-hand sides: b (ldb x nrhs,
+   column-major, caller numbering) is overwritten with x.  For MFD_FP32_IR
+   factors the solution is refined in FP64 to a normwise backward error of
+   2^-52-ish (see mfd_get_stats). */
+int mfd_solve(mfd_handle* h, double* b, int nrhs, int ldb);
+/* b in the postordered supernodal numbering (between the two permutations
+   of hppllt.f90:157,170). */
+int mfd_solve_permuted(mfd_handle* h, double* b, int nrhs, int ldb);
+/* Device-resident b (caller numbering), ldb >= n. */
+int mfd_solve_device(mfd_handle* h, double* d_b, int nrhs, int ldb);
+
+/* Factor in the reference's tile storage (App. layout of factors_m.f90):
+   supernode array (hppllt_get_array_size(1) entries) and work array
+   (hppllt_get_array_size(2)); either may be NULL.  The border array is
+   scratch in the reference (assembled copies of the border column before its
+   split, matrix_splitter_m.f90:5-46) and is not exported. */
+int mfd_export_tiles(mfd_handle* h, double* supernode, double* work);
+/* Dense front of node (1-based): column-major n x n, n = supernode_size +
+   work_size; lower part holds L (first supernode_size columns) and the
+   update matrix (trailing block). */
+int mfd_export_front(mfd_handle* h, int node, double* out);
+
+/* Getters: hppllt_get_cost / get_nonzero / get_array_size / get_time_info
+   (hppllt.f90:186-211).  Sizes are int64 (the reference's nonzero is int32). */
+int64_t mfd_get_cost(mfd_handle* h);
+int64_t mfd_get_nonzero(mfd_handle* h);
+int64_t mfd_get_array_size(mfd_handle* h, int which);
+double mfd_get_time_info(mfd_handle* h, int id);
+
+/* Structure accessors (for parity checks and the Fortran module):
+   sizes: n, num_node, nnz (entries of the lower CCS), sum of work sizes. */
+int mfd_get_dims(mfd_handle* h, int* n, int* num_node, int64_t* nnz, int64_t* nlidx);
+/* Any pointer may be NULL.  supernode_size/work_size/parent: num_node;
+   perm: n (new -> old); local_index: nlidx; supernodal_index, ccs_perm: nnz;
+   a_colptr: n + 1. */
+int mfd_get_structure(mfd_handle* h, int* supernode_size, int* work_size, int* parent, int* perm,
+                      int* local_index, int* supernodal_index, int64_t* ccs_perm, int64_t* a_colptr);
+
+typedef struct mfd_stats {
+    double analyze_ms, assemble_ms, factor_ms, solve_ms;  /* last calls, wall (HIP events) */
+    double factor_gflops;        /* cost / factor time (cost_calculator_m.f90:9-20) */
+    int precision;               /* of the current factor */
+    int refine_iters;            /* last solve */
+    double backward_error;       /* last solve: max over rhs of ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf) */
+    double refine_residual;      /* last solve: max over rhs of ||b - A x||_2 / ||x||_2 (example.f90:147-180) */
+    int64_t front_bytes;         /* device bytes of the front pool */
+    int64_t device_bytes;        /* all device bytes held by the handle */
+    int num_levels, num_launches_factor, num_launches_solve;
+} mfd_stats;
+int mfd_get_stats(mfd_handle* h, mfd_stats* st);
+
+/* Per-kernel-class timing of the last factorize + solve (needs
+   MFD_FLAG_PROFILE): total ms and number of launches of class k. */
+int mfd_get_kernel_time(mfd_handle* h, int k, double* total_ms, int* launches);
+
+const char* mfd_error_string(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

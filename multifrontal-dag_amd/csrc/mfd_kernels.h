@@ -1,0 +1,44 @@
+// Host-side launchers of the numeric kernels (mfd_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "mfd_plan.h"
+
+namespace mfd {
+
+template <typename T> void launch_fill(T* p, int64_t n, hipStream_t st);
+template <typename T>
+void launch_scatter_a(T* pool, const double* a, const int64_t* dst, const int64_t* src, int64_t nnz, hipStream_t st);
+template <typename T>
+void launch_extend_add(T* pool, const int* lidx, const NodeDev* nodes, const ExtTask* tasks, int count, hipStream_t st);
+template <typename T>
+void launch_small_front(T* pool, const NodeDev* nodes, const int* tasks, int count, int* info, hipStream_t st);
+template <typename T>
+void launch_potrf(T* pool, const NodeDev* nodes, const PanelTask* tasks, int count, int* info, hipStream_t st);
+template <typename T>
+void launch_trsm(T* pool, const NodeDev* nodes, const PanelTask* tasks, int count, hipStream_t st);
+template <typename T> void launch_gemm(T* pool, const GemmTask* tasks, int count, hipStream_t st);
+template <typename T>
+void launch_solve_fwd(const T* pool, double* zpool, const double* b, int ldb, int nrhs, const NodeDev* nodes,
+                      const int* lidx, const int* child_ptr, const int* child_list, const int* tasks, int count,
+                      hipStream_t st);
+template <typename T>
+void launch_solve_bwd(const T* pool, double* zpool, double* x, int ldx, int nrhs, const NodeDev* nodes, const int* lidx,
+                      const int* tasks, int count, hipStream_t st);
+void launch_set_int(int* p, int v, hipStream_t st);
+void launch_residual(double* r, const double* b, const double* x, int n, int nrhs, const int* ptr, const int* col,
+                     const int64_t* src, const double* a, hipStream_t st);
+void launch_absmax(unsigned long long* out, const double* x, int64_t n, int ld, int nrhs, hipStream_t st);
+void launch_norm_inf(unsigned long long* out, int n, const int* ptr, const int64_t* src, const double* a,
+                     hipStream_t st);
+void launch_axpy(double* x, const double* d, int64_t n, hipStream_t st);
+void launch_zero_u64(unsigned long long* p, int n, hipStream_t st);
+void launch_gather_rows(double* out, int ld_out, const double* in, int ld_in, const int* perm, int n, int nrhs,
+                        hipStream_t st);
+void launch_scatter_rows(double* out, int ld_out, const double* in, int ld_in, const int* perm, int n, int nrhs,
+                         hipStream_t st);
+
+}  // namespace mfd

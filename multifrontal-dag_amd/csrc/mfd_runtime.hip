@@ -1,0 +1,893 @@
+// Runtime of the numeric phase: the handle behind the C-ABI (include/mfdag.h),
+// the level-by-level GPU schedule built once per analysis, device memory,
+// optional hipGraph replay, and the FP32 + FP64-refinement precision form.
+//
+// The reference drives the numeric phase node by node in postorder
+// (seq_factorize_m.f90:13-29, seq_forward_m.f90:14-32,
+// seq_backward_m.f90:14-32) or as a StarPU task DAG
+// (starpu_factorize_m.f90:13-31).  Here the elimination tree is cut into
+// levels (a node's level is one more than its highest child's), every level
+// is a short fixed sequence of batched launches over all of its fronts, and
+// the whole sequence is built on the host once and replayed per
+// factorization.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "mfd_analysis.h"
+#include "mfd_kernels.h"
+#include "mfd_layout.h"
+#include "mfdag.h"
+
+using namespace mfd;
+
+namespace {
+
+struct Launch {
+    int kind;       // MFD_K_*
+    int64_t first;  // first task in the kind's task array
+    int count;
+};
+
+double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+#define HIP_TRY(expr)                                \
+    do {                                             \
+        hipError_t e_ = (expr);                      \
+        if (e_ != hipSuccess) {                      \
+            h->last_hip = e_;                        \
+            return MFD_ERR_HIP;                      \
+        }                                            \
+    } while (0)
+
+#define MFD_TRY(expr)         \
+    do {                      \
+        int r_ = (expr);      \
+        if (r_ != MFD_OK) return r_; \
+    } while (0)
+
+}  // namespace
+
+struct mfd_handle {
+    int nb = 0, flags = 0;
+    bool device_ok = false;
+    hipStream_t stream = nullptr;
+    hipError_t last_hip = hipSuccess;
+    int64_t mem_limit = 0, device_bytes = 0, front_bytes = 0;
+    std::vector<void*> allocs;
+
+    bool analyzed = false, factored = false, from_parts = false;
+    int precision = MFD_FP64;
+    Structure S;
+    RefLayout* layout = nullptr;
+
+    // host schedule
+    std::vector<NodeDev> nodes;
+    std::vector<ExtTask> ext;
+    std::vector<int> small;
+    std::vector<PanelTask> potrf, trsm;
+    std::vector<GemmTask> gemm;
+    std::vector<Launch> fseq;
+    std::vector<int> level_ptr, level_nodes, child_ptr, child_list;
+    int64_t pool_elems = 0, z_per_rhs = 0;
+    int num_levels = 0;
+
+    // device copies
+    NodeDev* d_nodes = nullptr;
+    int* d_lidx = nullptr;
+    ExtTask* d_ext = nullptr;
+    int* d_small = nullptr;
+    PanelTask *d_potrf = nullptr, *d_trsm = nullptr;
+    GemmTask* d_gemm = nullptr;
+    int *d_level_nodes = nullptr, *d_child_ptr = nullptr, *d_child_list = nullptr;
+    int64_t *d_dst = nullptr, *d_src = nullptr;
+    int* d_perm = nullptr;  // new -> old, 0-based
+    double* d_pool64 = nullptr;
+    float* d_pool32 = nullptr;
+    double* d_a = nullptr;  // caller-order values (nnz)
+    int* d_info = nullptr;
+    // solve buffers
+    int nrhs_cap = 0;
+    double *d_z = nullptr, *d_bin = nullptr, *d_bp = nullptr, *d_xp = nullptr;
+    // refinement
+    int *d_csr_ptr = nullptr, *d_csr_col = nullptr;
+    int64_t* d_csr_src = nullptr;  // index into the caller-order values of each CSR entry
+    int64_t csr_nnz = 0;
+    double *d_r = nullptr, *d_d = nullptr;
+    unsigned long long* d_norms = nullptr;
+
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    hipEvent_t ev[4] = {};
+    std::vector<hipEvent_t> prof_ev;
+
+    double times[8] = {0};
+    mfd_stats st{};
+    double ktime[MFD_K_NUM] = {0};
+    int kcount[MFD_K_NUM] = {0};
+};
+
+namespace {
+
+template <typename T> int dalloc(mfd_handle* h, T** p, int64_t n)
+{
+    size_t bytes = (size_t)std::max<int64_t>(n, 1) * sizeof(T);
+    if (h->mem_limit > 0 && h->device_bytes + (int64_t)bytes > h->mem_limit) return MFD_ERR_NOMEM;
+    if (hipMalloc((void**)p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return MFD_ERR_NOMEM;
+    }
+    h->device_bytes += (int64_t)bytes;
+    h->allocs.push_back((void*)*p);
+    return MFD_OK;
+}
+
+template <typename T> int upload(mfd_handle* h, T** p, const std::vector<T>& v)
+{
+    MFD_TRY(dalloc(h, p, (int64_t)v.size()));
+    if (!v.empty()) HIP_TRY(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return MFD_OK;
+}
+
+void free_device(mfd_handle* h)
+{
+    for (auto& g : h->graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    for (void* p : h->allocs) (void)hipFree(p);
+    h->allocs.clear();
+    h->device_bytes = 0;
+    h->d_nodes = nullptr;
+    h->d_lidx = nullptr;
+    h->d_ext = nullptr;
+    h->d_small = nullptr;
+    h->d_potrf = h->d_trsm = nullptr;
+    h->d_gemm = nullptr;
+    h->d_level_nodes = h->d_child_ptr = h->d_child_list = nullptr;
+    h->d_dst = h->d_src = nullptr;
+    h->d_perm = nullptr;
+    h->d_pool64 = nullptr;
+    h->d_pool32 = nullptr;
+    h->d_a = nullptr;
+    h->d_info = nullptr;
+    h->nrhs_cap = 0;
+    h->d_z = h->d_bin = h->d_bp = h->d_xp = nullptr;
+    h->d_csr_ptr = h->d_csr_col = nullptr;
+    h->d_csr_src = nullptr;
+    h->d_r = h->d_d = nullptr;
+    h->d_norms = nullptr;
+}
+
+int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+// ------------------------------------------------------------- schedule
+void build_plan(mfd_handle* h)
+{
+    const Structure& S = h->S;
+    const int m = S.num_node;
+    h->nodes.assign(m, NodeDev{});
+    int64_t off = 0, zoff = 0;
+    for (int s = 0; s < m; ++s) {
+        NodeDev& d = h->nodes[s];
+        d.n = S.c[s] + S.w[s];
+        d.c = S.c[s];
+        d.w = S.w[s];
+        d.front = off;
+        d.lidx = S.li_ptr[s];
+        d.z = zoff;
+        d.parent = S.parent[s] - 1;
+        d.first = S.first[s] - 1;
+        off += align_up((int64_t)d.n * d.n, 32);
+        zoff += d.n;
+    }
+    h->pool_elems = off;
+    h->z_per_rhs = zoff;
+
+    std::vector<int> lev(m, 0);
+    for (int s = 0; s < m; ++s)
+        for (int ch : S.children[s]) lev[s] = std::max(lev[s], lev[ch - 1] + 1);
+    h->num_levels = m ? *std::max_element(lev.begin(), lev.end()) + 1 : 0;
+    h->level_ptr.assign(h->num_levels + 1, 0);
+    for (int s = 0; s < m; ++s) h->level_ptr[lev[s] + 1]++;
+    for (int L = 0; L < h->num_levels; ++L) h->level_ptr[L + 1] += h->level_ptr[L];
+    h->level_nodes.assign(m, 0);
+    {
+        std::vector<int> pos(h->level_ptr.begin(), h->level_ptr.end() - 1);
+        for (int s = 0; s < m; ++s) h->level_nodes[pos[lev[s]]++] = s;
+    }
+    h->child_ptr.assign(m + 1, 0);
+    h->child_list.clear();
+    for (int s = 0; s < m; ++s) {
+        for (int ch : S.children[s]) h->child_list.push_back(ch - 1);
+        h->child_ptr[s + 1] = (int)h->child_list.size();
+    }
+
+    h->ext.clear();
+    h->small.clear();
+    h->potrf.clear();
+    h->trsm.clear();
+    h->gemm.clear();
+    h->fseq.clear();
+    auto push = [&](int kind, int64_t first, int64_t end) {
+        if (end > first) h->fseq.push_back(Launch{kind, first, (int)(end - first)});
+    };
+    for (int L = 0; L < h->num_levels; ++L) {
+        const int* lv = h->level_nodes.data() + h->level_ptr[L];
+        const int cnt = h->level_ptr[L + 1] - h->level_ptr[L];
+        // extend-add: the r-th child of every parent of this level in one launch
+        size_t maxch = 0;
+        for (int k = 0; k < cnt; ++k) maxch = std::max(maxch, S.children[lv[k]].size());
+        for (size_t r = 0; r < maxch; ++r) {
+            int64_t first = (int64_t)h->ext.size();
+            for (int k = 0; k < cnt; ++k) {
+                const auto& chs = S.children[lv[k]];
+                if (chs.size() <= r) continue;
+                int ch = chs[r] - 1, nt = (S.w[ch] + kTile - 1) / kTile;
+                for (int ti = 0; ti < nt; ++ti)
+                    for (int tj = 0; tj <= ti; ++tj) h->ext.push_back(ExtTask{ch, ti, tj, 0});
+            }
+            push(MFD_K_EXTEND_ADD, first, (int64_t)h->ext.size());
+        }
+        // small fronts: whole front in LDS
+        int64_t first = (int64_t)h->small.size();
+        std::vector<int> large;
+        for (int k = 0; k < cnt; ++k) {
+            const NodeDev& d = h->nodes[lv[k]];
+            if (d.n <= kSmallFront) h->small.push_back(lv[k]);
+            else large.push_back(lv[k]);
+        }
+        push(MFD_K_SMALL_FRONT, first, (int64_t)h->small.size());
+        // large fronts: panel steps, then one Schur update with K = c
+        int steps = 0;
+        for (int s : large) steps = std::max(steps, (h->nodes[s].c + kPanel - 1) / kPanel);
+        for (int t = 0; t < steps; ++t) {
+            const int k0 = t * kPanel;
+            int64_t f0 = (int64_t)h->potrf.size();
+            for (int s : large) {
+                const NodeDev& d = h->nodes[s];
+                if (d.c > k0) h->potrf.push_back(PanelTask{s, k0, std::min(kPanel, d.c - k0), 0});
+            }
+            push(MFD_K_POTRF, f0, (int64_t)h->potrf.size());
+            int64_t f1 = (int64_t)h->trsm.size();
+            for (int s : large) {
+                const NodeDev& d = h->nodes[s];
+                if (d.c <= k0) continue;
+                int kb = std::min(kPanel, d.c - k0);
+                for (int r0 = k0 + kb; r0 < d.n; r0 += 256) h->trsm.push_back(PanelTask{s, k0, kb, r0});
+            }
+            push(MFD_K_TRSM, f1, (int64_t)h->trsm.size());
+            int64_t f2 = (int64_t)h->gemm.size();
+            for (int s : large) {
+                const NodeDev& d = h->nodes[s];
+                if (d.c <= k0) continue;
+                int kb = std::min(kPanel, d.c - k0);
+                for (int c0 = k0 + kb; c0 < d.c; c0 += kTile)
+                    for (int r0 = c0; r0 < d.n; r0 += kTile) {
+                        GemmTask g{};
+                        g.c = d.front + r0 + (int64_t)c0 * d.n;
+                        g.a = d.front + r0 + (int64_t)k0 * d.n;
+                        g.b = d.front + c0 + (int64_t)k0 * d.n;
+                        g.ldc = g.lda = g.ldb = d.n;
+                        g.m = std::min(kTile, d.n - r0);
+                        g.n = std::min(kTile, d.c - c0);
+                        g.k = kb;
+                        g.tri = r0 == c0;
+                        h->gemm.push_back(g);
+                    }
+            }
+            push(MFD_K_UPDATE, f2, (int64_t)h->gemm.size());
+        }
+        int64_t f3 = (int64_t)h->gemm.size();
+        for (int s : large) {
+            const NodeDev& d = h->nodes[s];
+            for (int c0 = d.c; c0 < d.n; c0 += kTile)
+                for (int r0 = c0; r0 < d.n; r0 += kTile) {
+                    GemmTask g{};
+                    g.c = d.front + r0 + (int64_t)c0 * d.n;
+                    g.a = d.front + r0;
+                    g.b = d.front + c0;
+                    g.ldc = g.lda = g.ldb = d.n;
+                    g.m = std::min(kTile, d.n - r0);
+                    g.n = std::min(kTile, d.n - c0);
+                    g.k = d.c;
+                    g.tri = r0 == c0;
+                    h->gemm.push_back(g);
+                }
+        }
+        push(MFD_K_SCHUR, f3, (int64_t)h->gemm.size());
+    }
+}
+
+// Full symmetric CSR (both triangles) of the postordered matrix for the
+// refinement residual; entry e takes its value from caller entry csr_src[e].
+void build_csr(const Structure& S, std::vector<int>& ptr, std::vector<int>& col, std::vector<int64_t>& src)
+{
+    const int n = S.n;
+    std::vector<int> cnt(n + 1, 0);
+    // rows of a_structure: snidx is front-relative; recover the global row
+    // from the node's first column and row list.
+    std::vector<int> node_of(n, 0);
+    for (int s = 0; s < S.num_node; ++s)
+        for (int j = S.first[s]; j < S.first[s + 1]; ++j) node_of[j - 1] = s;
+    auto global_row = [&](int j, int64_t p) {
+        int s = node_of[j - 1];
+        int r = S.snidx[p];
+        int c = S.c[s];
+        return r <= c ? S.first[s] + r - 1 : S.l_rows[S.l_ptr[s] + (r - c - 1)];
+    };
+    for (int j = 1; j <= n; ++j)
+        for (int64_t p = S.a_colptr[j - 1]; p < S.a_colptr[j]; ++p) {
+            int r = global_row(j, p);
+            cnt[r]++;
+            if (r != j) cnt[j]++;
+        }
+    ptr.assign(n + 1, 0);
+    for (int i = 0; i < n; ++i) ptr[i + 1] = ptr[i] + cnt[i + 1];
+    col.assign(ptr[n], 0);
+    src.assign(ptr[n], 0);
+    std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+    for (int j = 1; j <= n; ++j)
+        for (int64_t p = S.a_colptr[j - 1]; p < S.a_colptr[j]; ++p) {
+            int r = global_row(j, p);
+            int64_t v = S.ccs_perm[p] - 1;
+            col[pos[r - 1]] = j - 1;
+            src[pos[r - 1]++] = v;
+            if (r != j) {
+                col[pos[j - 1]] = r - 1;
+                src[pos[j - 1]++] = v;
+            }
+        }
+}
+
+int setup_device(mfd_handle* h)
+{
+    free_device(h);
+    const Structure& S = h->S;
+    build_plan(h);
+    MFD_TRY(upload(h, &h->d_nodes, h->nodes));
+    MFD_TRY(upload(h, &h->d_lidx, S.lidx));
+    MFD_TRY(upload(h, &h->d_ext, h->ext));
+    MFD_TRY(upload(h, &h->d_small, h->small));
+    MFD_TRY(upload(h, &h->d_potrf, h->potrf));
+    MFD_TRY(upload(h, &h->d_trsm, h->trsm));
+    MFD_TRY(upload(h, &h->d_gemm, h->gemm));
+    MFD_TRY(upload(h, &h->d_level_nodes, h->level_nodes));
+    MFD_TRY(upload(h, &h->d_child_ptr, h->child_ptr));
+    MFD_TRY(upload(h, &h->d_child_list, h->child_list));
+    // assembly map: A entry -> front position (coefficient_setter_m.f90:26-41)
+    const int64_t nnz = (int64_t)S.snidx.size();
+    std::vector<int64_t> dst(nnz), src(nnz);
+    for (int s = 0; s < S.num_node; ++s) {
+        const NodeDev& d = h->nodes[s];
+        for (int j = S.first[s]; j < S.first[s + 1]; ++j) {
+            int64_t q = j - S.first[s];
+            for (int64_t p = S.a_colptr[j - 1]; p < S.a_colptr[j]; ++p) {
+                dst[p] = d.front + (S.snidx[p] - 1) + q * d.n;
+                src[p] = S.ccs_perm[p] - 1;
+            }
+        }
+    }
+    MFD_TRY(upload(h, &h->d_dst, dst));
+    MFD_TRY(upload(h, &h->d_src, src));
+    std::vector<int> perm0(S.n);
+    for (int i = 0; i < S.n; ++i) perm0[i] = S.perm[i] - 1;
+    MFD_TRY(upload(h, &h->d_perm, perm0));
+    MFD_TRY(dalloc(h, &h->d_a, nnz));
+    MFD_TRY(dalloc(h, &h->d_info, 1));
+    MFD_TRY(dalloc(h, &h->d_pool64, h->pool_elems));
+    h->front_bytes = h->pool_elems * (int64_t)sizeof(double);
+    std::vector<int> cptr, ccol;
+    std::vector<int64_t> csrc;
+    build_csr(S, cptr, ccol, csrc);
+    h->csr_nnz = (int64_t)ccol.size();
+    MFD_TRY(upload(h, &h->d_csr_ptr, cptr));
+    MFD_TRY(upload(h, &h->d_csr_col, ccol));
+    MFD_TRY(upload(h, &h->d_csr_src, csrc));
+    return MFD_OK;
+}
+
+int ensure_rhs(mfd_handle* h, int nrhs)
+{
+    if (nrhs <= h->nrhs_cap) return MFD_OK;
+    const int64_t n = h->S.n;
+    MFD_TRY(dalloc(h, &h->d_z, h->z_per_rhs * nrhs));
+    MFD_TRY(dalloc(h, &h->d_bin, n * nrhs));
+    MFD_TRY(dalloc(h, &h->d_bp, n * nrhs));
+    MFD_TRY(dalloc(h, &h->d_xp, n * nrhs));
+    MFD_TRY(dalloc(h, &h->d_r, n * nrhs));
+    MFD_TRY(dalloc(h, &h->d_d, n * nrhs));
+    MFD_TRY(dalloc(h, &h->d_norms, 3 * nrhs + 1));
+    h->nrhs_cap = nrhs;
+    return MFD_OK;
+}
+
+// ------------------------------------------------------------ factorize
+template <typename T> T* pool_of(mfd_handle* h);
+template <> double* pool_of<double>(mfd_handle* h) { return h->d_pool64; }
+template <> float* pool_of<float>(mfd_handle* h) { return h->d_pool32; }
+
+template <typename T> void enqueue_launch(mfd_handle* h, const Launch& l)
+{
+    T* pool = pool_of<T>(h);
+    hipStream_t st = h->stream;
+    switch (l.kind) {
+    case MFD_K_EXTEND_ADD: launch_extend_add<T>(pool, h->d_lidx, h->d_nodes, h->d_ext + l.first, l.count, st); break;
+    case MFD_K_SMALL_FRONT: launch_small_front<T>(pool, h->d_nodes, h->d_small + l.first, l.count, h->d_info, st); break;
+    case MFD_K_POTRF: launch_potrf<T>(pool, h->d_nodes, h->d_potrf + l.first, l.count, h->d_info, st); break;
+    case MFD_K_TRSM: launch_trsm<T>(pool, h->d_nodes, h->d_trsm + l.first, l.count, st); break;
+    case MFD_K_UPDATE:
+    case MFD_K_SCHUR: launch_gemm<T>(pool, h->d_gemm + l.first, l.count, st); break;
+    default: break;
+    }
+}
+
+// The factorization schedule: reset the pivot flag, zero the front pool,
+// scatter A, then every level's launches.  No hipMemsetAsync, so the
+// sequence can be captured into a graph.
+template <typename T> void enqueue_factor(mfd_handle* h, bool profile)
+{
+    hipStream_t st = h->stream;
+    T* pool = pool_of<T>(h);
+    auto mark = [&](int k) {
+        if (!profile) return;
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        (void)hipEventRecord(e, st);
+        h->prof_ev.push_back(e);
+        (void)k;
+    };
+    launch_set_int(h->d_info, INT_MAX, st);
+    mark(0);
+    launch_fill<T>(pool, h->pool_elems, st);
+    launch_scatter_a<T>(pool, h->d_a, h->d_dst, h->d_src, (int64_t)h->S.snidx.size(), st);
+    mark(0);
+    for (const Launch& l : h->fseq) {
+        enqueue_launch<T>(h, l);
+        mark(l.kind);
+    }
+}
+
+void collect_profile(mfd_handle* h, const std::vector<int>& kinds)
+{
+    // prof_ev[0..1] bracket assembly; then one event after each launch
+    if (h->prof_ev.size() < 2) return;
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, h->prof_ev[0], h->prof_ev[1]);
+    h->ktime[MFD_K_ASSEMBLE] += ms;
+    h->kcount[MFD_K_ASSEMBLE] += 2;
+    for (size_t i = 0; i + 2 < h->prof_ev.size() && i < kinds.size(); ++i) {
+        (void)hipEventElapsedTime(&ms, h->prof_ev[i + 1], h->prof_ev[i + 2]);
+        h->ktime[kinds[i]] += ms;
+        h->kcount[kinds[i]] += 1;
+    }
+    for (auto e : h->prof_ev) (void)hipEventDestroy(e);
+    h->prof_ev.clear();
+}
+
+template <typename T> int run_factor(mfd_handle* h)
+{
+    const bool profile = (h->flags & MFD_FLAG_PROFILE) != 0;
+    const bool graph = (h->flags & MFD_FLAG_GRAPH) != 0 && !profile;
+    const int gi = sizeof(T) == 8 ? 0 : 1;
+    HIP_TRY(hipEventRecord(h->ev[0], h->stream));
+    if (graph) {
+        if (!h->graph[gi]) {
+            hipGraph_t g;
+            HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            enqueue_factor<T>(h, false);
+            HIP_TRY(hipStreamEndCapture(h->stream, &g));
+            HIP_TRY(hipGraphInstantiate(&h->graph[gi], g, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(g);
+        }
+        HIP_TRY(hipGraphLaunch(h->graph[gi], h->stream));
+    } else {
+        if (profile)
+            for (int k = 0; k < MFD_K_NUM; ++k) h->ktime[k] = 0, h->kcount[k] = 0;
+        enqueue_factor<T>(h, profile);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(h->ev[1], h->stream));
+    HIP_TRY(hipEventSynchronize(h->ev[1]));
+    if (profile) {
+        std::vector<int> kinds;
+        for (const Launch& l : h->fseq) kinds.push_back(l.kind);
+        collect_profile(h, kinds);
+    }
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
+    h->st.factor_ms = ms;
+    int info = INT_MAX;
+    HIP_TRY(hipMemcpy(&info, h->d_info, sizeof(int), hipMemcpyDeviceToHost));
+    return info == INT_MAX ? MFD_OK : info;
+}
+
+int factorize_common(mfd_handle* h, int precision)
+{
+    if (!h->analyzed) return MFD_ERR_STATE;
+    if (!h->device_ok) return MFD_ERR_NODEVICE;
+    if (precision != MFD_FP64 && precision != MFD_FP32_IR) return MFD_ERR_UNSUPPORTED;
+    h->factored = false;
+    int rc;
+    if (precision == MFD_FP32_IR) {
+        if (!h->d_pool32) MFD_TRY(dalloc(h, &h->d_pool32, h->pool_elems));
+        rc = run_factor<float>(h);
+    } else {
+        rc = run_factor<double>(h);
+    }
+    if (rc < 0) return rc;
+    h->precision = precision;
+    h->factored = rc == MFD_OK;
+    const double cost = (double)mfd::cost(h->S);
+    h->st.factor_gflops = h->st.factor_ms > 0 ? cost / (h->st.factor_ms * 1e6) : 0;
+    h->st.precision = precision;
+    return rc;
+}
+
+// --------------------------------------------------------------- solve
+template <typename T> void enqueue_solve(mfd_handle* h, const double* bp, double* xp, int nrhs)
+{
+    hipStream_t st = h->stream;
+    const T* pool = pool_of<T>(h);
+    const int n = h->S.n;
+    for (int L = 0; L < h->num_levels; ++L)
+        launch_solve_fwd<T>(pool, h->d_z, bp, n, nrhs, h->d_nodes, h->d_lidx, h->d_child_ptr, h->d_child_list,
+                            h->d_level_nodes + h->level_ptr[L], h->level_ptr[L + 1] - h->level_ptr[L], st);
+    for (int L = h->num_levels - 1; L >= 0; --L)
+        launch_solve_bwd<T>(pool, h->d_z, xp, n, nrhs, h->d_nodes, h->d_lidx, h->d_level_nodes + h->level_ptr[L],
+                            h->level_ptr[L + 1] - h->level_ptr[L], st);
+}
+
+// Normwise backward error per right-hand side, max over them:
+// ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf), and the reference's
+// ||b - A x||_2 / ||x||_2 is not computed here (see the host tests).
+int backward_error(mfd_handle* h, const double* bp, const double* xp, double* r, int nrhs, double* eta)
+{
+    const int n = h->S.n;
+    hipStream_t st = h->stream;
+    launch_residual(r, bp, xp, n, nrhs, h->d_csr_ptr, h->d_csr_col, h->d_csr_src, h->d_a, st);
+    launch_zero_u64(h->d_norms, 3 * nrhs + 1, st);
+    launch_absmax(h->d_norms, r, n, n, nrhs, st);
+    launch_absmax(h->d_norms + nrhs, xp, n, n, nrhs, st);
+    launch_absmax(h->d_norms + 2 * nrhs, bp, n, n, nrhs, st);
+    launch_norm_inf(h->d_norms + 3 * nrhs, n, h->d_csr_ptr, h->d_csr_src, h->d_a, st);
+    std::vector<unsigned long long> v(3 * nrhs + 1);
+    HIP_TRY(hipMemcpyAsync(v.data(), h->d_norms, v.size() * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    auto d = [&](int i) {
+        double x;
+        memcpy(&x, &v[i], 8);
+        return x;
+    };
+    double an = d(3 * nrhs), worst = 0;
+    for (int k = 0; k < nrhs; ++k) {
+        double den = an * d(nrhs + k) + d(2 * nrhs + k);
+        double e = den > 0 ? d(k) / den : (d(k) > 0 ? INFINITY : 0.0);
+        if (!(e <= worst)) worst = e;  // NaN propagates
+    }
+    *eta = worst;
+    return MFD_OK;
+}
+
+constexpr int kMaxRefine = 10;
+constexpr double kRefineTarget = 1e-15;
+
+// x for b, both postordered (ld n) in device memory.  FP32 factors are
+// refined in FP64 (residual with the FP64 matrix, correction with the FP32
+// factor) until the backward error reaches the target or stops improving;
+// if it then still exceeds 1e-14, the matrix is refactored in FP64.
+int solve_permuted_device(mfd_handle* h, int nrhs)
+{
+    const int64_t n = h->S.n;
+    hipStream_t st = h->stream;
+    h->st.refine_iters = 0;
+    h->st.backward_error = -1;
+    if (h->precision == MFD_FP64) {
+        enqueue_solve<double>(h, h->d_bp, h->d_xp, nrhs);
+        return MFD_OK;
+    }
+    enqueue_solve<float>(h, h->d_bp, h->d_xp, nrhs);
+    double eta = 0, prev = INFINITY;
+    int it = 0;
+    for (;;) {
+        MFD_TRY(backward_error(h, h->d_bp, h->d_xp, h->d_r, nrhs, &eta));
+        if (!(eta > kRefineTarget) || it >= kMaxRefine || !(eta < 0.5 * prev)) break;
+        prev = eta;
+        enqueue_solve<float>(h, h->d_r, h->d_d, nrhs);
+        for (int k = 0; k < nrhs; ++k) launch_axpy(h->d_xp + k * n, h->d_d + k * n, n, st);
+        ++it;
+    }
+    h->st.refine_iters = it;
+    h->st.backward_error = eta;
+    if (!(eta <= 1e-14)) {
+        // refinement stalled: FP64 refactorization fallback
+        int rc = factorize_common(h, MFD_FP64);
+        if (rc != MFD_OK) return rc;
+        enqueue_solve<double>(h, h->d_bp, h->d_xp, nrhs);
+        MFD_TRY(backward_error(h, h->d_bp, h->d_xp, h->d_r, nrhs, &eta));
+        h->st.backward_error = eta;
+        h->st.refine_iters = -1 - it;  // negative: fell back to FP64
+    }
+    return MFD_OK;
+}
+
+// Copy b (ldb x nrhs) to the device, apply the analysis permutation, solve,
+// undo the permutation and copy back (hppllt.f90:152-173).
+int solve_host(mfd_handle* h, double* b, int nrhs, int ldb, bool permuted, bool on_device)
+{
+    if (!h->factored) return MFD_ERR_STATE;
+    if (!b || nrhs < 1 || ldb < h->S.n) return MFD_ERR_ARG;
+    MFD_TRY(ensure_rhs(h, nrhs));
+    const int n = h->S.n;
+    hipStream_t st = h->stream;
+    double t0 = now_ms();
+    const hipMemcpyKind kin = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const hipMemcpyKind kout = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    HIP_TRY(hipMemcpy2DAsync(h->d_bin, (size_t)n * 8, b, (size_t)ldb * 8, (size_t)n * 8, nrhs, kin, st));
+    HIP_TRY(hipEventRecord(h->ev[2], st));
+    if (permuted)
+        HIP_TRY(hipMemcpyAsync(h->d_bp, h->d_bin, (size_t)n * nrhs * 8, hipMemcpyDeviceToDevice, st));
+    else
+        launch_gather_rows(h->d_bp, n, h->d_bin, n, h->d_perm, n, nrhs, st);
+    MFD_TRY(solve_permuted_device(h, nrhs));
+    if (permuted)
+        HIP_TRY(hipMemcpyAsync(h->d_bin, h->d_xp, (size_t)n * nrhs * 8, hipMemcpyDeviceToDevice, st));
+    else
+        launch_scatter_rows(h->d_bin, n, h->d_xp, n, h->d_perm, n, nrhs, st);
+    HIP_TRY(hipEventRecord(h->ev[3], st));
+    HIP_TRY(hipMemcpy2DAsync(b, (size_t)ldb * 8, h->d_bin, (size_t)n * 8, (size_t)n * 8, nrhs, kout, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipGetLastError());
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, h->ev[2], h->ev[3]));
+    h->st.solve_ms = ms;
+    h->times[MFD_TIME_SOLVE] = (now_ms() - t0) / 1e3;
+    return MFD_OK;
+}
+
+int finish_analysis(mfd_handle* h)
+{
+    delete h->layout;
+    h->layout = new RefLayout(h->S.c, h->S.w, h->nb);
+    h->analyzed = true;
+    h->factored = false;
+    if (!h->device_ok) return MFD_OK;
+    int rc = setup_device(h);
+    if (rc != MFD_OK) {
+        free_device(h);
+        h->analyzed = false;
+    }
+    return rc;
+}
+
+}  // namespace
+
+// ================================================================== C-ABI
+extern "C" {
+
+int mfd_create(mfd_handle** out, int nb, int flags)
+{
+    if (!out || nb < 1) return MFD_ERR_ARG;
+    *out = nullptr;
+    mfd_handle* h = new mfd_handle();
+    h->nb = nb;
+    h->flags = flags;
+    if (const char* lim = getenv("MFD_DEVICE_MEM_LIMIT")) h->mem_limit = atoll(lim);
+    int count = 0;
+    if (hipGetDeviceCount(&count) == hipSuccess && count > 0) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, 0) == hipSuccess && strncmp(prop.gcnArchName, "gfx950", 6) == 0 &&
+            hipSetDevice(0) == hipSuccess && hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) == hipSuccess) {
+            h->device_ok = true;
+            for (auto& e : h->ev) (void)hipEventCreate(&e);
+        }
+    }
+    (void)hipGetLastError();
+    *out = h;
+    return h->device_ok ? MFD_OK : MFD_ERR_NODEVICE;
+}
+
+void mfd_destroy(mfd_handle* h)
+{
+    if (!h) return;
+    if (h->device_ok) {
+        (void)hipStreamSynchronize(h->stream);
+        free_device(h);
+        for (auto e : h->ev)
+            if (e) (void)hipEventDestroy(e);
+        (void)hipStreamDestroy(h->stream);
+    }
+    delete h->layout;
+    delete h;
+}
+
+int mfd_analyze(mfd_handle* h, int n, const int* ccs_col, const int* ccs_row, int max_zero, const int* perm)
+{
+    if (!h) return MFD_ERR_ARG;
+    double t0 = now_ms();
+    h->analyzed = h->factored = false;
+    h->from_parts = false;
+    int rc = analyze(n, ccs_col, ccs_row, max_zero, perm, h->S);
+    if (rc != MFD_OK) return rc;
+    rc = finish_analysis(h);
+    h->times[MFD_TIME_ANALYZE] = (now_ms() - t0) / 1e3;
+    h->st.analyze_ms = now_ms() - t0;
+    return rc;
+}
+
+int mfd_set_structure(mfd_handle* h, int num_node, const int* c, const int* w, const int* parent, const int* lidx,
+                      const int64_t* a_colptr, const int* snidx)
+{
+    if (!h || !a_colptr || !snidx) return MFD_ERR_ARG;
+    h->analyzed = h->factored = false;
+    int rc = structure_from_parts(num_node, c, w, parent, lidx, a_colptr, snidx, h->S);
+    if (rc != MFD_OK) return rc;
+    h->from_parts = true;
+    return finish_analysis(h);
+}
+
+int mfd_factorize(mfd_handle* h, const double* a, int precision)
+{
+    if (!h || !a) return MFD_ERR_ARG;
+    if (!h->analyzed) return MFD_ERR_STATE;
+    if (!h->device_ok) return MFD_ERR_NODEVICE;
+    double t0 = now_ms();
+    const size_t nnz = h->S.snidx.size();
+    HIP_TRY(hipMemcpyAsync(h->d_a, a, nnz * 8, hipMemcpyHostToDevice, h->stream));
+    int rc = factorize_common(h, precision);
+    h->times[MFD_TIME_FACTORIZE] = (now_ms() - t0) / 1e3;
+    h->times[MFD_TIME_REAL_FACTORIZE] = h->st.factor_ms / 1e3;
+    return rc;
+}
+
+int mfd_factorize_device(mfd_handle* h, const double* d_a, int precision)
+{
+    if (!h || !d_a) return MFD_ERR_ARG;
+    if (!h->analyzed) return MFD_ERR_STATE;
+    if (!h->device_ok) return MFD_ERR_NODEVICE;
+    double t0 = now_ms();
+    if (d_a != h->d_a)
+        HIP_TRY(hipMemcpyAsync(h->d_a, d_a, h->S.snidx.size() * 8, hipMemcpyDeviceToDevice, h->stream));
+    int rc = factorize_common(h, precision);
+    h->times[MFD_TIME_FACTORIZE] = (now_ms() - t0) / 1e3;
+    h->times[MFD_TIME_REAL_FACTORIZE] = h->st.factor_ms / 1e3;
+    return rc;
+}
+
+int mfd_solve(mfd_handle* h, double* b, int nrhs, int ldb)
+{
+    if (!h) return MFD_ERR_ARG;
+    return solve_host(h, b, nrhs, ldb, false, false);
+}
+
+int mfd_solve_permuted(mfd_handle* h, double* b, int nrhs, int ldb)
+{
+    if (!h) return MFD_ERR_ARG;
+    return solve_host(h, b, nrhs, ldb, true, false);
+}
+
+int mfd_solve_device(mfd_handle* h, double* d_b, int nrhs, int ldb)
+{
+    if (!h) return MFD_ERR_ARG;
+    return solve_host(h, d_b, nrhs, ldb, false, true);
+}
+
+int mfd_export_front(mfd_handle* h, int node, double* out)
+{
+    if (!h || !out) return MFD_ERR_ARG;
+    if (!h->factored) return MFD_ERR_STATE;
+    if (node < 1 || node > h->S.num_node) return MFD_ERR_ARG;
+    const NodeDev& d = h->nodes[node - 1];
+    const int64_t cnt = (int64_t)d.n * d.n;
+    if (h->precision == MFD_FP64) {
+        HIP_TRY(hipMemcpy(out, h->d_pool64 + d.front, cnt * 8, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> tmp(cnt);
+        HIP_TRY(hipMemcpy(tmp.data(), h->d_pool32 + d.front, cnt * 4, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < cnt; ++i) out[i] = tmp[i];
+    }
+    return MFD_OK;
+}
+
+int mfd_export_tiles(mfd_handle* h, double* supernode, double* work)
+{
+    if (!h) return MFD_ERR_ARG;
+    if (!h->factored) return MFD_ERR_STATE;
+    std::vector<double> front;
+    for (int s = 0; s < h->S.num_node; ++s) {
+        const NodeDev& d = h->nodes[s];
+        front.assign((size_t)d.n * d.n, 0.0);
+        int rc = mfd_export_front(h, s + 1, front.data());
+        if (rc != MFD_OK) return rc;
+        h->layout->put_front(s, front.data(), supernode, work);
+    }
+    return MFD_OK;
+}
+
+int64_t mfd_get_cost(mfd_handle* h) { return h && h->analyzed ? mfd::cost(h->S) : -1; }
+int64_t mfd_get_nonzero(mfd_handle* h) { return h && h->analyzed ? mfd::nonzero(h->S) : -1; }
+int64_t mfd_get_array_size(mfd_handle* h, int which) { return h && h->layout ? h->layout->size(which) : -1; }
+double mfd_get_time_info(mfd_handle* h, int id)
+{
+    if (!h || id < 1 || id > 7) return -1;
+    return h->times[id];
+}
+
+int mfd_get_dims(mfd_handle* h, int* n, int* num_node, int64_t* nnz, int64_t* nlidx)
+{
+    if (!h) return MFD_ERR_ARG;
+    if (!h->analyzed) return MFD_ERR_STATE;
+    if (n) *n = h->S.n;
+    if (num_node) *num_node = h->S.num_node;
+    if (nnz) *nnz = (int64_t)h->S.snidx.size();
+    if (nlidx) *nlidx = (int64_t)h->S.lidx.size();
+    return MFD_OK;
+}
+
+int mfd_get_structure(mfd_handle* h, int* c, int* w, int* parent, int* perm, int* lidx, int* snidx, int64_t* ccs_perm,
+                      int64_t* a_colptr)
+{
+    if (!h) return MFD_ERR_ARG;
+    if (!h->analyzed) return MFD_ERR_STATE;
+    const Structure& S = h->S;
+    auto cp = [](auto* dst, const auto& v) {
+        if (dst) std::copy(v.begin(), v.end(), dst);
+    };
+    cp(c, S.c);
+    cp(w, S.w);
+    cp(parent, S.parent);
+    cp(perm, S.perm);
+    cp(lidx, S.lidx);
+    cp(snidx, S.snidx);
+    cp(ccs_perm, S.ccs_perm);
+    cp(a_colptr, S.a_colptr);
+    return MFD_OK;
+}
+
+int mfd_get_stats(mfd_handle* h, mfd_stats* st)
+{
+    if (!h || !st) return MFD_ERR_ARG;
+    h->st.front_bytes = h->front_bytes;
+    h->st.device_bytes = h->device_bytes;
+    h->st.num_levels = h->num_levels;
+    h->st.num_launches_factor = (int)h->fseq.size() + 3;
+    h->st.num_launches_solve = 2 * h->num_levels + 2;
+    *st = h->st;
+    return MFD_OK;
+}
+
+int mfd_get_kernel_time(mfd_handle* h, int k, double* total_ms, int* launches)
+{
+    if (!h || k < 0 || k >= MFD_K_NUM) return MFD_ERR_ARG;
+    if (total_ms) *total_ms = h->ktime[k];
+    if (launches) *launches = h->kcount[k];
+    return MFD_OK;
+}
+
+const char* mfd_error_string(int code)
+{
+    switch (code) {
+    case MFD_OK: return "ok";
+    case MFD_ERR_ARG: return "invalid argument";
+    case MFD_ERR_STATE: return "call out of order";
+    case MFD_ERR_HIP: return "HIP runtime error";
+    case MFD_ERR_NOMEM: return "device memory budget exhausted";
+    case MFD_ERR_REDUCIBLE: return "matrix is reducible (elimination forest)";
+    case MFD_ERR_INTERNAL: return "internal structure inconsistency";
+    case MFD_ERR_NODEVICE: return "no gfx950 device";
+    case MFD_ERR_UNSUPPORTED: return "unsupported option";
+    case MFD_ERR_NUMERIC: return "non-finite input";
+    default: return code > 0 ? "matrix is not positive definite" : "unknown error";
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,237 @@
+"""Python front end of libmfdag (the MI355X numeric phase of a supernodal
+multifrontal Cholesky solver).
+
+`Solver` wraps one C-ABI handle (include/mfdag.h); `hppllt` mirrors the
+reference's Fortran module API (src/main/hppllt.f90:8-9) on top of it.
+The GPU path has no fallback: if libmfdag.so is missing or no gfx950 device
+is present, the numeric calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libmfdag.so")
+
+MFD_OK = 0
+MFD_ERR_NODEVICE = -7
+FP64 = 0
+FP32_IR = 1
+FLAG_GRAPH = 1
+FLAG_PROFILE = 2
+KERNEL_CLASSES = ["assemble", "extend_add", "small_front", "potrf", "trsm", "update", "schur", "solve_fwd",
+                  "solve_bwd", "refine"]
+
+_lib = None
+
+
+class MfdError(RuntimeError):
+    def __init__(self, code, what):
+        self.code = code
+        msg = lib().mfd_error_string(code).decode() if _lib is not None else str(code)
+        super().__init__(f"{what}: {msg} (code {code})")
+
+
+class NotPositiveDefinite(MfdError):
+    pass
+
+
+class MfdStats(C.Structure):
+    _fields_ = [("analyze_ms", C.c_double), ("assemble_ms", C.c_double), ("factor_ms", C.c_double),
+                ("solve_ms", C.c_double), ("factor_gflops", C.c_double), ("precision", C.c_int),
+                ("refine_iters", C.c_int), ("backward_error", C.c_double), ("refine_residual", C.c_double),
+                ("front_bytes", C.c_int64), ("device_bytes", C.c_int64), ("num_levels", C.c_int),
+                ("num_launches_factor", C.c_int), ("num_launches_solve", C.c_int)]
+
+
+def lib():
+    """Loads libmfdag.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} is missing: run multifrontal-dag_amd/build_lib.py")
+        L = C.CDLL(LIB_PATH)
+        P, i, i64, d = C.c_void_p, C.c_int, C.c_int64, C.c_double
+        ip = C.POINTER(C.c_int)
+        lp = C.POINTER(C.c_int64)
+        dp = C.POINTER(C.c_double)
+        sig = {
+            "mfd_create": (i, [C.POINTER(P), i, i]),
+            "mfd_destroy": (None, [P]),
+            "mfd_analyze": (i, [P, i, ip, ip, i, ip]),
+            "mfd_set_structure": (i, [P, i, ip, ip, ip, ip, lp, ip]),
+            "mfd_factorize": (i, [P, dp, i]),
+            "mfd_factorize_device": (i, [P, C.c_void_p, i]),
+            "mfd_solve": (i, [P, dp, i, i]),
+            "mfd_solve_permuted": (i, [P, dp, i, i]),
+            "mfd_solve_device": (i, [P, C.c_void_p, i, i]),
+            "mfd_export_tiles": (i, [P, dp, dp]),
+            "mfd_export_front": (i, [P, i, dp]),
+            "mfd_get_cost": (i64, [P]),
+            "mfd_get_nonzero": (i64, [P]),
+            "mfd_get_array_size": (i64, [P, i]),
+            "mfd_get_time_info": (d, [P, i]),
+            "mfd_get_dims": (i, [P, ip, ip, lp, lp]),
+            "mfd_get_structure": (i, [P, ip, ip, ip, ip, ip, ip, lp, lp]),
+            "mfd_get_stats": (i, [P, C.POINTER(MfdStats)]),
+            "mfd_get_kernel_time": (i, [P, i, dp, ip]),
+            "mfd_error_string": (C.c_char_p, [i]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct)) if a is not None else None
+
+
+def _i32(a):
+    return np.ascontiguousarray(np.asarray(a), dtype=np.int32)
+
+
+class Solver:
+    """One analysis + factorization + solves (one C-ABI handle).
+
+    nb is the reference's tile size, used only for the reference-layout
+    views (export_tiles, array_size).  With host_only=True the handle may be
+    created on a machine without a GPU (analysis and structure queries work;
+    numeric calls raise)."""
+
+    def __init__(self, nb=64, graph=False, profile=False, host_only=False):
+        L = lib()
+        h = C.c_void_p()
+        flags = (FLAG_GRAPH if graph else 0) | (FLAG_PROFILE if profile else 0)
+        rc = L.mfd_create(C.byref(h), int(nb), flags)
+        self.h = h
+        self.nb = int(nb)
+        self.device = rc == MFD_OK
+        if rc not in (MFD_OK, MFD_ERR_NODEVICE) or (rc == MFD_ERR_NODEVICE and not host_only):
+            L.mfd_destroy(h)
+            self.h = None
+            raise MfdError(rc, "mfd_create")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mfd_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _check(self, rc, what):
+        if rc > 0:
+            raise NotPositiveDefinite(rc, what)
+        if rc < 0:
+            raise MfdError(rc, what)
+        return rc
+
+    # -------------------------------------------------------------- analysis
+    def analyze(self, ccs_col, ccs_row, max_zero=0, perm=None):
+        """1-based lower CCS (diagonal first); perm: new -> old, 1-based."""
+        ccs_col = _i32(ccs_col)
+        ccs_row = _i32(ccs_row)
+        n = len(ccs_col) - 1
+        p = _i32(perm) if perm is not None else None
+        self._check(lib().mfd_analyze(self.h, n, _ptr(ccs_col, C.c_int), _ptr(ccs_row, C.c_int), int(max_zero),
+                                      _ptr(p, C.c_int)), "mfd_analyze")
+        self._dims()
+
+    def set_structure(self, supernode_size, work_size, parent, local_index, a_colptr, supernodal_index):
+        c, w, pa = _i32(supernode_size), _i32(work_size), _i32(parent)
+        li = _i32(local_index) if len(local_index) else np.zeros(1, np.int32)
+        cp = np.ascontiguousarray(a_colptr, dtype=np.int64)
+        sx = _i32(supernodal_index)
+        self._check(lib().mfd_set_structure(self.h, len(c), _ptr(c, C.c_int), _ptr(w, C.c_int), _ptr(pa, C.c_int),
+                                            _ptr(li, C.c_int), _ptr(cp, C.c_int64), _ptr(sx, C.c_int)),
+                    "mfd_set_structure")
+        self._dims()
+
+    def _dims(self):
+        n, m = C.c_int(), C.c_int()
+        nnz, nl = C.c_int64(), C.c_int64()
+        self._check(lib().mfd_get_dims(self.h, C.byref(n), C.byref(m), C.byref(nnz), C.byref(nl)), "mfd_get_dims")
+        self.n, self.num_node, self.nnz, self.nlidx = n.value, m.value, nnz.value, nl.value
+
+    def structure(self):
+        m, n, nnz, nl = self.num_node, self.n, self.nnz, self.nlidx
+        out = dict(supernode_size=np.zeros(m, np.int32), work_size=np.zeros(m, np.int32),
+                   parent=np.zeros(m, np.int32), perm=np.zeros(n, np.int32),
+                   local_index=np.zeros(max(nl, 1), np.int32), supernodal_index=np.zeros(max(nnz, 1), np.int32),
+                   ccs_perm=np.zeros(max(nnz, 1), np.int64), a_colptr=np.zeros(n + 1, np.int64))
+        self._check(lib().mfd_get_structure(
+            self.h, *[_ptr(out[k], C.c_int) for k in ("supernode_size", "work_size", "parent", "perm",
+                                                       "local_index", "supernodal_index")],
+            _ptr(out["ccs_perm"], C.c_int64), _ptr(out["a_colptr"], C.c_int64)), "mfd_get_structure")
+        out["local_index"] = out["local_index"][:nl]
+        out["supernodal_index"] = out["supernodal_index"][:nnz]
+        out["ccs_perm"] = out["ccs_perm"][:nnz]
+        return out
+
+    def cost(self):
+        return lib().mfd_get_cost(self.h)
+
+    def nonzero(self):
+        return lib().mfd_get_nonzero(self.h)
+
+    def array_size(self, which):
+        return lib().mfd_get_array_size(self.h, int(which))
+
+    # -------------------------------------------------------------- numeric
+    def factorize(self, a, precision=FP64):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.size != self.nnz:
+            raise ValueError(f"expected {self.nnz} values, got {a.size}")
+        return self._check(lib().mfd_factorize(self.h, _ptr(a, C.c_double), int(precision)), "mfd_factorize")
+
+    def factorize_device(self, d_a_ptr, precision=FP64):
+        return self._check(lib().mfd_factorize_device(self.h, C.c_void_p(d_a_ptr), int(precision)),
+                           "mfd_factorize_device")
+
+    def solve(self, b, permuted=False):
+        """b: (n,) or (n, nrhs); returns x (a new array)."""
+        x = np.array(b, dtype=np.float64, order="F", copy=True)
+        one = x.ndim == 1
+        if one:
+            x = x.reshape(-1, 1, order="F")
+        f = lib().mfd_solve_permuted if permuted else lib().mfd_solve
+        self._check(f(self.h, _ptr(x, C.c_double), x.shape[1], x.shape[0]), "mfd_solve")
+        return x[:, 0] if one else x
+
+    def solve_device(self, d_b_ptr, nrhs, ldb):
+        return self._check(lib().mfd_solve_device(self.h, C.c_void_p(d_b_ptr), int(nrhs), int(ldb)),
+                           "mfd_solve_device")
+
+    def export_tiles(self):
+        sn = np.zeros(max(self.array_size(1), 1))
+        wk = np.zeros(max(self.array_size(2), 1))
+        self._check(lib().mfd_export_tiles(self.h, _ptr(sn, C.c_double), _ptr(wk, C.c_double)), "mfd_export_tiles")
+        return sn[: self.array_size(1)], wk[: self.array_size(2)]
+
+    def front(self, node):
+        st = self.structure()
+        n = int(st["supernode_size"][node - 1] + st["work_size"][node - 1])
+        out = np.zeros(n * n)
+        self._check(lib().mfd_export_front(self.h, int(node), _ptr(out, C.c_double)), "mfd_export_front")
+        return out.reshape(n, n, order="F")
+
+    def stats(self):
+        s = MfdStats()
+        self._check(lib().mfd_get_stats(self.h, C.byref(s)), "mfd_get_stats")
+        return {k: getattr(s, k) for k, _ in MfdStats._fields_}
+
+    def kernel_times(self):
+        out = {}
+        for k, name in enumerate(KERNEL_CLASSES):
+            ms, cnt = C.c_double(), C.c_int()
+            lib().mfd_get_kernel_time(self.h, k, C.byref(ms), C.byref(cnt))
+            out[name] = (ms.value, cnt.value)
+        return out
+
+    def time_info(self, i):
+        return lib().mfd_get_time_info(self.h, int(i))

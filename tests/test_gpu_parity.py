@@ -1,0 +1,220 @@
+"""GPU parity: the HIP numeric phase against the oracle (the C restatement of
+the reference's seq_factorize / seq_forward / seq_backward) and against the
+reference's own known answers.  Tolerances: factor fronts and solutions
+within 1e-12 relative (normwise per front / per solution), backward error
+<= 1e-14; integer-valued known answers exactly."""
+import sys
+
+import numpy as np
+import pytest
+
+import mfdag
+from mfdag import problems as P
+from oracle import oracle as O
+
+from parity import front_errors, oracle_for, tile_mask_compare
+from test_oracle_kats import check_three_front_factors
+
+pytestmark = pytest.mark.gpu
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def three_front(kats, nb, graph=False):
+    t = kats["three_front"]
+    s = mfdag.Solver(nb=nb, graph=graph)
+    lens = np.array(t["supernodal_index_lengths"], np.int64)
+    colptr = np.concatenate([[0], np.cumsum(lens)])
+    s.set_structure(t["supernode_size"], t["work_size"], t["parent"], t["local_index"], colptr,
+                    t["supernodal_index"])
+    return s, np.array(t["val"], float)
+
+
+@pytest.mark.parametrize("nb", [2, 3])
+def test_three_front_kat_exact(kats, nb):
+    """seq_factorize_test.f90 (13 x 13, integer factors): exact tiles."""
+    log("three_front nb", nb)
+    s, val = three_front(kats, nb)
+    assert s.factorize(val) == 0
+    sn, _ = s.export_tiles()
+    f = O.OracleFactor(kats["three_front"]["supernode_size"], kats["three_front"]["work_size"],
+                       kats["three_front"]["parent"], nb, kats["three_front"]["local_index"])
+
+    def tile(node, i, j):
+        off, r, c = f.tile(1, node, i, j)
+        return sn[off:off + r * c]
+    check_three_front_factors(kats, nb, tile)
+
+
+def test_three_front_solve(kats):
+    """Solve with the factor of the 13 x 13 problem for x = 1 (the known
+    answer of seq_forward_test / seq_backward_test)."""
+    s, val = three_front(kats, 2)
+    s.factorize(val)
+    st = s.structure()
+    f, _, _ = oracle_for(s, val)
+    b = np.ones(13)
+    xo = f.solve(b.copy())
+    x = s.solve(b, permuted=True)
+    assert np.allclose(x, xo, rtol=1e-13, atol=1e-13)
+    # x = A^{-1} (A 1): build A 1 from the oracle's dense front data
+    A = assemble_dense_from_parts(kats)
+    x1 = s.solve(A @ np.ones(13), permuted=True)
+    assert np.allclose(x1, 1.0, rtol=0, atol=1e-12)
+
+
+def assemble_dense_from_parts(kats):
+    t = kats["three_front"]
+    c, w, par, li = t["supernode_size"], t["work_size"], t["parent"], t["local_index"]
+    first = np.concatenate([[1], 1 + np.cumsum(c)])
+    lptr = np.concatenate([[0], np.cumsum(w)])
+    rows_of = {}
+    for s in range(len(c) - 1, -1, -1):
+        p = par[s]
+        rows = []
+        for k in range(w[s]):
+            pos = li[lptr[s] + k]
+            rows.append(first[p - 1] + pos - 1 if pos <= c[p - 1] else rows_of[p - 1][pos - c[p - 1] - 1])
+        rows_of[s] = rows
+    n = sum(c)
+    A = np.zeros((n, n))
+    lens = t["supernodal_index_lengths"]
+    k = 0
+    col = 0
+    for s in range(len(c)):
+        for j in range(c[s]):
+            for _ in range(lens[col]):
+                r = t["supernodal_index"][k]
+                g = first[s] + r - 1 if r <= c[s] else rows_of[s][r - c[s] - 1]
+                A[g - 1, col] = t["val"][k]
+                A[col, g - 1] = t["val"][k]
+                k += 1
+            col += 1
+    return A
+
+
+CASES = [
+    ("lap2d_20", lambda: P.laplacian_2d(20), 0, 8),
+    ("lap2d_100", lambda: P.laplacian_2d(100), 16, 64),
+    ("wlap3d_8", lambda: P.weighted_laplacian_3d(8, seed=3), 0, 16),
+    ("wlap3d_12x10x9", lambda: P.weighted_laplacian_3d(12, 10, 9, seed=4), 32, 5),
+    ("st27_7", lambda: P.stencil27_3d(7, seed=5), 8, 7),
+    ("wlap3d_16", lambda: P.weighted_laplacian_3d(16, seed=6), 64, 32),
+]
+
+
+@pytest.mark.parametrize("name,make,max_zero,nb", CASES)
+def test_factor_and_solve_vs_oracle(name, make, max_zero, nb):
+    log("case", name, "max_zero", max_zero, "nb", nb)
+    p = make()
+    s = mfdag.Solver(nb=nb)
+    s.analyze(p.ccs_col, p.ccs_row, max_zero, p.perm)
+    assert s.factorize(p.val) == 0
+    f, st, info = oracle_for(s, p.val)
+    assert info == 0
+    errs = front_errors(s, f, st)
+    assert errs.max() <= 1e-12, (name, errs.max(), int(errs.argmax()) + 1)
+    worst, cnt = tile_mask_compare(s, f)
+    assert cnt > 0 and worst <= 1e-12, (name, worst)
+    xt, b = P.rhs(p, 1, seed=1)
+    x = s.solve(b[:, 0])
+    # oracle solve in the permuted numbering
+    perm = st["perm"] - 1
+    xo = np.empty(p.n)
+    xo[perm] = f.solve(b[perm, 0].copy())
+    assert np.linalg.norm(x - xo, np.inf) <= 1e-12 * np.linalg.norm(xo, np.inf), name
+    assert P.backward_error(p, x, b[:, 0]) <= 1e-14
+
+
+def test_multiple_rhs_and_device_entry_points():
+    p = P.weighted_laplacian_3d(10, seed=7)
+    s = mfdag.Solver(nb=16)
+    s.analyze(p.ccs_col, p.ccs_row, 8, p.perm)
+    s.factorize(p.val)
+    xt, b = P.rhs(p, 4, seed=2)
+    x = s.solve(b)
+    for k in range(4):
+        xk = s.solve(b[:, k])
+        assert np.array_equal(xk, x[:, k])
+    assert P.backward_error(p, x, b) <= 1e-14
+
+
+def test_graph_replay_matches_eager():
+    """hipGraph replays (several, with eager work in between) give the
+    bitwise eager result."""
+    p = P.weighted_laplacian_3d(14, seed=8)
+    eager = mfdag.Solver(nb=32)
+    eager.analyze(p.ccs_col, p.ccs_row, 16, p.perm)
+    eager.factorize(p.val)
+    fronts = [eager.front(k) for k in range(1, eager.num_node + 1)]
+    g = mfdag.Solver(nb=32, graph=True)
+    g.analyze(p.ccs_col, p.ccs_row, 16, p.perm)
+    xt, b = P.rhs(p, 1, seed=3)
+    for rep in range(3):
+        g.factorize(p.val * (1.0 + rep))
+        eager.factorize(p.val * (1.0 + rep))
+        fronts = [eager.front(k) for k in range(1, eager.num_node + 1)]
+        for k in range(1, g.num_node + 1):
+            assert np.array_equal(np.tril(g.front(k)), np.tril(fronts[k - 1])), (rep, k)
+        assert np.array_equal(g.solve(b[:, 0]), eager.solve(b[:, 0]))
+
+
+def test_fp32_refinement_reaches_fp64_accuracy():
+    p = P.weighted_laplacian_3d(16, seed=9)
+    s = mfdag.Solver(nb=64)
+    s.analyze(p.ccs_col, p.ccs_row, 32, p.perm)
+    s.factorize(p.val, precision=mfdag.FP32_IR)
+    xt, b = P.rhs(p, 2, seed=4)
+    x = s.solve(b)
+    st = s.stats()
+    log("fp32 ir iters", st["refine_iters"], "eta", st["backward_error"])
+    assert st["refine_iters"] >= 1
+    assert st["backward_error"] <= 1e-14
+    assert P.backward_error(p, x, b) <= 1e-14
+
+
+def test_not_positive_definite_reports_column():
+    p = P.laplacian_2d(12)
+    val = p.val.copy()
+    diag_pos = p.ccs_col[:-1] - 1
+    val[diag_pos[50]] = -4.0   # column 51 (caller numbering) loses positivity
+    s = mfdag.Solver(nb=8)
+    s.analyze(p.ccs_col, p.ccs_row, 0, p.perm)
+    with pytest.raises(mfdag.NotPositiveDefinite) as e:
+        s.factorize(val)
+    f, st, info = oracle_for(s, val)
+    assert info > 0
+    assert e.value.code >= 1
+    with pytest.raises(mfdag.MfdError):
+        s.solve(np.ones(p.n))   # no valid factor
+
+
+def test_memory_budget_is_an_error(monkeypatch):
+    monkeypatch.setenv("MFD_DEVICE_MEM_LIMIT", "100000")
+    p = P.weighted_laplacian_3d(12, seed=1)
+    s = mfdag.Solver(nb=16)
+    with pytest.raises(mfdag.MfdError) as e:
+        s.analyze(p.ccs_col, p.ccs_row, 0, p.perm)
+    assert e.value.code == -4
+
+
+@pytest.mark.parametrize("n", [1, 40, 65, 200])
+def test_dense_front_matches_lapack(n):
+    """A dense matrix is one front (c = n, w = 0): exercises the small-front,
+    panel and update kernels against numpy's Cholesky."""
+    rng = np.random.default_rng(n)
+    m = rng.standard_normal((n, n))
+    Ad = m @ m.T + n * np.eye(n)
+    rows, cols = np.tril_indices(n)
+    cc, rr, vv = P.lower_ccs(n, rows, cols, Ad[rows, cols])
+    s = mfdag.Solver(nb=16)
+    s.analyze(cc, rr, 0)
+    s.factorize(vv)
+    assert s.num_node == 1
+    F = s.front(1)
+    perm = s.structure()["perm"] - 1
+    L = np.linalg.cholesky(Ad[np.ix_(perm, perm)])
+    assert np.abs(np.tril(F) - L).max() <= 1e-12 * np.abs(L).max()
