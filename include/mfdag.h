@@ -154,6 +154,12 @@ int mfd_get_stats(mfd_handle* h, mfd_stats* st);
 /* Per-kernel-class timing of the last factorize + solve (needs
    MFD_FLAG_PROFILE): total ms and number of launches of class k. */
 int mfd_get_kernel_time(mfd_handle* h, int k, double* total_ms, int* launches);
+/* Algorithmic work of kernel class k over one factorization (k < MFD_K_SOLVE_FWD)
+   or one single-RHS solve (solve classes) in the current schedule: flops
+   (2 per multiply-add of the entries the kernel must produce) and the HBM
+   bytes the class must move at minimum (each operand read once, each result
+   written once) for an element size of elem_bytes. */
+int mfd_get_kernel_work(mfd_handle* h, int k, int elem_bytes, double* flops, double* bytes);
 
 const char* mfd_error_string(int code);
 

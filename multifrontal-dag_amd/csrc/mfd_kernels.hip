@@ -29,16 +29,20 @@ constexpr int kThreads = 256;
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
+// D(16x16) += A(16x4) B(4x16); lane l supplies A[l%16][l/16] and
+// B[l/16][l%16].  The accumulator layouts differ (measured on gfx950):
+//   f64: element v of lane l is D[l/16 + 4v][l%16]
+//   f32: element v of lane l is D[4(l/16) + v][l%16]
 template <typename T> struct Mfma;
 template <> struct Mfma<double> {
     typedef d4 acc_t;
-    // D(16x16) += A(16x4) B(4x16); lane l holds A[l%16][l/16] and B[l/16][l%16];
-    // accumulator element v of lane l is D[4*(l/16)+v][l%16].
     static __device__ acc_t mma(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    static __device__ int row(int lane, int v) { return (lane >> 4) + 4 * v; }
 };
 template <> struct Mfma<float> {
     typedef f4 acc_t;
     static __device__ acc_t mma(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    static __device__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
 __device__ inline double dsqrt(double x) { return sqrt(x); }
@@ -240,13 +244,14 @@ __global__ __launch_bounds__(kThreads) void k_gemm(T* pool, const GemmTask* task
         }
         __syncthreads();
     }
-    const int r0 = wr + 4 * (lane >> 4), q0 = wc + (lane & 15);
+    const int q0 = wc + (lane & 15);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        Cs[q0][r0 + v] = c00[v];
-        Cs[q0 + 16][r0 + v] = c01[v];
-        Cs[q0][r0 + 16 + v] = c10[v];
-        Cs[q0 + 16][r0 + 16 + v] = c11[v];
+        const int r = wr + Mfma<T>::row(lane, v);
+        Cs[q0][r] = c00[v];
+        Cs[q0 + 16][r] = c01[v];
+        Cs[q0][r + 16] = c10[v];
+        Cs[q0 + 16][r + 16] = c11[v];
     }
     __syncthreads();
     T* C = pool + t.c;

@@ -873,6 +873,73 @@ int mfd_get_kernel_time(mfd_handle* h, int k, double* total_ms, int* launches)
     return MFD_OK;
 }
 
+int mfd_get_kernel_work(mfd_handle* h, int k, int s, double* flops, double* bytes)
+{
+    if (!h || k < 0 || k >= MFD_K_NUM || s < 1) return MFD_ERR_ARG;
+    if (!h->analyzed) return MFD_ERR_STATE;
+    double f = 0, b = 0;
+    auto tri = [](double x) { return x * (x + 1) / 2; };
+    switch (k) {
+    case MFD_K_ASSEMBLE: {
+        double nnz = (double)h->S.snidx.size();
+        b = s * (double)h->pool_elems + nnz * (8 + 8 + 8 + s);
+        break;
+    }
+    case MFD_K_EXTEND_ADD:
+        for (const auto& d : h->nodes)
+            if (d.parent >= 0) {
+                double e = tri(d.w);
+                f += e;
+                b += e * 3 * s + 4.0 * d.w;
+            }
+        break;
+    case MFD_K_SMALL_FRONT:
+        for (int s0 : h->small) {
+            const NodeDev& d = h->nodes[s0];
+            for (int64_t m = d.w + 1; m <= d.n; ++m) f += (double)m * m;
+            b += 2.0 * s * tri(d.n);
+        }
+        break;
+    case MFD_K_POTRF:
+        for (const auto& t : h->potrf) {
+            f += (double)t.kb * t.kb * t.kb / 3.0;
+            b += 2.0 * s * tri(t.kb);
+        }
+        break;
+    case MFD_K_TRSM:
+        for (const auto& t : h->trsm) {
+            double rows = std::min(256, h->nodes[t.node].n - t.row0);
+            f += rows * t.kb * t.kb;
+            b += s * (2.0 * rows * t.kb + tri(t.kb));
+        }
+        break;
+    case MFD_K_UPDATE:
+    case MFD_K_SCHUR:
+        for (const Launch& l : h->fseq) {
+            if (l.kind != k) continue;
+            for (int64_t i = l.first; i < l.first + l.count; ++i) {
+                const GemmTask& g = h->gemm[i];
+                double e = g.tri ? (double)g.n * g.m - tri(g.n - 1) : (double)g.m * g.n;
+                f += 2.0 * g.k * e;
+                b += s * ((double)g.m * g.k + (double)g.n * g.k + 2.0 * e);
+            }
+        }
+        break;
+    case MFD_K_SOLVE_FWD:
+    case MFD_K_SOLVE_BWD:
+        for (const auto& d : h->nodes) {
+            double lent = tri(d.c) + (double)d.w * d.c;
+            f += 2.0 * lent;
+            b += s * lent + 8.0 * 3 * d.n;
+        }
+        break;
+    default: break;
+    }
+    if (flops) *flops = f;
+    if (bytes) *bytes = b;
+    return MFD_OK;
+}
+
 const char* mfd_error_string(int code)
 {
     switch (code) {

@@ -78,6 +78,7 @@ def lib():
             "mfd_get_structure": (i, [P, ip, ip, ip, ip, ip, ip, lp, lp]),
             "mfd_get_stats": (i, [P, C.POINTER(MfdStats)]),
             "mfd_get_kernel_time": (i, [P, i, dp, ip]),
+            "mfd_get_kernel_work": (i, [P, i, i, dp, dp]),
             "mfd_error_string": (C.c_char_p, [i]),
         }
         for name, (res, args) in sig.items():
@@ -231,6 +232,17 @@ class Solver:
             ms, cnt = C.c_double(), C.c_int()
             lib().mfd_get_kernel_time(self.h, k, C.byref(ms), C.byref(cnt))
             out[name] = (ms.value, cnt.value)
+        return out
+
+    def kernel_work(self, elem_bytes=8):
+        """Algorithmic (flops, bytes) per kernel class for one factorization /
+        one single-RHS solve."""
+        out = {}
+        for k, name in enumerate(KERNEL_CLASSES):
+            f, b = C.c_double(), C.c_double()
+            self._check(lib().mfd_get_kernel_work(self.h, k, int(elem_bytes), C.byref(f), C.byref(b)),
+                        "mfd_get_kernel_work")
+            out[name] = (f.value, b.value)
         return out
 
     def time_info(self, i):
