@@ -67,22 +67,26 @@ def timed_steps(s, d_a, d_b, d_x, precision, steps, warmup):
     return wall, float(np.median(fms)), float(np.median(sms))
 
 
-def roofline(p, max_zero, nb, precision):
+def roofline(p, b, max_zero, nb, precision):
     """One profiled factorization + solve: per-class time (HIP events on the
-    library's stream, one pair per launch) against the class's algorithmic
-    work; returns the dominant class's roofline entry and all classes."""
+    library's stream, one pair per launch; the solve sweeps as a whole)
+    against the class's algorithmic work; returns the dominant class's
+    roofline entry and all classes."""
     import mfdag
     s = mfdag.Solver(nb=nb, profile=True)
     s.analyze(p.ccs_col, p.ccs_row, max_zero, p.perm)
     s.factorize(p.val, precision)
+    s.solve(b)
     kt = s.kernel_times()
     work = s.kernel_work(8 if precision == mfdag.FP64 else 4)
     rows = {}
     for name, (ms, cnt) in kt.items():
         if cnt == 0 or ms <= 0:
             continue
-        f, b = work[name]
-        rows[name] = dict(ms=ms, launches=cnt, gflops=f / (ms * 1e6), gbs=b / (ms * 1e6))
+        f, by = work[name]
+        if name.startswith("solve"):
+            f *= b.shape[1]
+        rows[name] = dict(ms=ms, launches=cnt, gflops=f / (ms * 1e6), gbs=by / (ms * 1e6))
     dom = max(rows, key=lambda k: rows[k]["ms"])
     r = rows[dom]
     if dom in MFMA_CLASSES:
@@ -139,6 +143,7 @@ def main():
     ap.add_argument("--cpu-size", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--fp64-only", action="store_true", help="skip the FP32+IR leg and the profiled pass")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU path is not built",
@@ -169,6 +174,9 @@ def main():
     st64 = s.stats()
     log(f"FP64: step {wall64 * 1e3:.2f} ms (factor {f64:.2f} ms, solve {s64:.2f} ms), eta {eta64:.2e}")
 
+    if args.fp64_only:
+        print(json.dumps({"fp64_ms_per_step": wall64 * 1e3, "factor_ms": f64, "solve_ms": s64, "eta": eta64}))
+        return
     wall32, f32, s32 = timed_steps(s, d_a, d_b, d_x, mfdag.FP32_IR, args.steps, args.warmup)
     x32 = d_x.cpu().numpy().T
     eta32 = P.backward_error(p, x32, b)
@@ -176,7 +184,7 @@ def main():
     log(f"FP32+IR: step {wall32 * 1e3:.2f} ms (factor {f32:.2f} ms, solve {s32:.2f} ms), eta {eta32:.2e}, "
         f"iters {st32['refine_iters']}")
 
-    roof, classes = roofline(p, args.max_zero, args.nb, mfdag.FP64)
+    roof, classes = roofline(p, b, args.max_zero, args.nb, mfdag.FP64)
     log("kernel classes (FP64):", json.dumps({k: {kk: round(v, 4) for kk, v in r.items()} for k, r in classes.items()}))
 
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_size, args.max_zero, args.nb, args.seed)

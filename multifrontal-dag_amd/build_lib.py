@@ -47,7 +47,29 @@ def build(verbose=False):
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs
     subprocess.run(cmd, check=True, capture_output=True, text=True)
     os.replace(tmp, LIB)
+    build_fortran()
     return LIB
+
+
+FLANG = os.environ.get("FLANG", "/opt/rocm/llvm/bin/flang")
+FMOD = os.path.join(HERE, "fortran", "hppllt.f90")
+FLIB = os.path.join(HERE, "lib", "libhppllt.so")
+FCHECK_SRC = os.path.join(ROOT, "tests", "fortran", "hppllt_check.f90")
+FCHECK = os.path.join(HERE, "lib", "hppllt_check")
+
+
+def build_fortran():
+    """The Fortran module hppllt (libhppllt.so, over libmfdag.so) and the
+    caller-side check program, both with $ORIGIN run paths so the pair works
+    from any copy of the tree."""
+    moddir = os.path.join(OUT, "fmod")
+    os.makedirs(moddir, exist_ok=True)
+    libdir = os.path.dirname(LIB)
+    rpath = "-Wl,-rpath,$ORIGIN"
+    subprocess.run([FLANG, "-O2", "-fPIC", "-shared", "-module-dir", moddir, FMOD, "-o", FLIB,
+                    "-L", libdir, "-lmfdag", rpath], check=True, capture_output=True, text=True)
+    subprocess.run([FLANG, "-O2", "-I", moddir, FCHECK_SRC, "-o", FCHECK,
+                    "-L", libdir, "-lhppllt", "-lmfdag", rpath], check=True, capture_output=True, text=True)
 
 
 if __name__ == "__main__":

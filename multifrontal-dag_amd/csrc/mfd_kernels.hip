@@ -3,18 +3,18 @@
 // Fronts live in one pool, each an n x n column-major block (see NodeDev in
 // mfd_plan.h).  Every kernel processes one range of a task array; a
 // workgroup owns one task, so no two workgroups of one launch write the same
-// front entry and the results do not depend on scheduling.
+// entry and the results do not depend on scheduling.
 //
 // Reference operations these kernels replace (paths relative to the
 // reference root, src/main/):
-//   k_scatter_a     set_zero + set_coefficient      zero_setter_m.f90:12-31, coefficient_setter_m.f90:15-50
-//   k_extend_add    extend_add_tri / extend_add_rect seq_factorize/kernel/extend_add_kernel_m.f90:6-34
-//   k_small_front   the whole per-front tile loop    seq_factorize_m.f90:31-79
-//   k_potrf         DPOTRF("U") on a diagonal tile   seq_factorize/kernel/factorize_kernel_m.f90:4-9
-//   k_trsm          DTRSM("L","U","T","N")           factorize_kernel_m.f90:11-16, border_kernel_m.f90:21-30
-//   k_gemm          DSYRK / DGEMM updates            factorize_kernel_m.f90:18-33, border_subroutines_m.f90:41-73
-//   k_solve_fwd     seq_forward (TRSV + GEMV + scatter_add)  seq_solve/seq_forward_m.f90:14-81
-//   k_solve_bwd     seq_backward (gather + GEMV + TRSV)      seq_solve/seq_backward_m.f90:14-131
+//   k_scatter_a     set_zero + set_coefficient        zero_setter_m.f90:12-31, coefficient_setter_m.f90:15-50
+//   k_extend_add    extend_add_tri / extend_add_rect  seq_factorize/kernel/extend_add_kernel_m.f90:6-34
+//   k_small_front   the per-front tile loop           seq_factorize_m.f90:31-79
+//   k_panel         DPOTRF("U") + DTRSM("L","U","T")  seq_factorize/kernel/factorize_kernel_m.f90:4-16,
+//                                                     border_kernel_m.f90:9-30
+//   k_gemm          DSYRK / DGEMM updates             factorize_kernel_m.f90:18-33, border_subroutines_m.f90:41-73
+//   k_fwd_*         seq_forward (scatter_add, TRSV, GEMV)   seq_solve/seq_forward_m.f90:14-81
+//   k_bwd_*         seq_backward (gather, GEMV, TRSV)       seq_solve/seq_backward_m.f90:14-131
 #include <hip/hip_runtime.h>
 
 #include "mfd_kernels.h"
@@ -24,6 +24,7 @@ namespace mfd {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kDLd = 2 * kPanel + 1;  // row length of the LDS copy of a diagonal block
 
 // ------------------------------------------------------------ MFMA traits
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -48,25 +49,102 @@ template <> struct Mfma<float> {
 __device__ inline double dsqrt(double x) { return sqrt(x); }
 __device__ inline float dsqrt(float x) { return sqrtf(x); }
 
-// Right-looking Cholesky of the first c columns of an n x n lower matrix held
-// in LDS as F[col][row] (n <= 64); the trailing (n-c) x (n-c) block receives
-// the Schur complement.  Division (not a reciprocal) keeps integer-valued
-// factors exact.  A non-positive pivot reports its global column.
-template <typename T>
-__device__ void chol_lds(T (*F)[65], int n, int c, int* info, int col0)
+// Orders one wave's LDS accesses (they complete in issue order within a
+// wave) against compiler reordering; no workgroup barrier is involved.
+__device__ inline void wave_sync()
 {
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    for (int k = 0; k < c; ++k) {
-        T d = F[k][k];
-        T s = dsqrt(d);
-        if (!(d > T(0)) && threadIdx.x == 0) atomicMin(info, col0 + k + 1);
-        if (ty == 0 && tx > k && tx < n) F[k][tx] = F[k][tx] / s;
-        __syncthreads();
-        if (threadIdx.x == 0) F[k][k] = s;
-        for (int jj = k + 1 + ty; jj < n; jj += 4)
-            if (tx >= jj && tx < n) F[jj][tx] -= F[k][tx] * F[k][jj];
-        __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Row kernels keep one 64-entry row per lane in registers as a shift
+// register: at step k, x[j] holds the row's entry of column k + j.  The
+// step loop is rolled and the inner loop fully unrolled, so every register
+// index is static (no scratch memory) and each step shifts the row by one.
+// The 64 steps run in four segments of 16 whose inner width shrinks with
+// the segment (64 - K0), so the redundant work past the live columns is at
+// most 15 columns per step.
+
+// One wave (threads 0..63) factors the first c columns of an n x n lower
+// block (n <= 64) held one row per lane (x[j] = A[lane][j], zero above the
+// diagonal).  At step k the pivot comes from lane k by a shuffle and the
+// finished column is published through LDS (colbuf: 128 entries, the upper
+// 64 zero) and read back as broadcasts.  Column k of L is stored to g
+// (leading dimension ld).  A pivot that is not positive records the global
+// column col0 + k + 1 in *info (smallest wins).  Afterwards x[j] holds the
+// Schur complement entry of column c + j.
+template <int K0, typename T>
+__device__ __forceinline__ void chol_seg(T (&x)[64], T* colbuf, int n, int c, int* info, int col0, T* g, int64_t ld)
+{
+    constexpr int W = 64 - K0;
+    const int lane = threadIdx.x & 63;
+    const int kend = min(K0 + 16, c);
+#pragma unroll 1
+    for (int k = K0; k < kend; ++k) {
+        const T d = __shfl(x[0], k);
+        if (!(d > T(0)) && lane == 0) atomicMin(info, col0 + k + 1);
+        const T s = dsqrt(d);
+        const T l = lane == k ? s : x[0] / s;
+        colbuf[lane] = l;
+        wave_sync();
+        if (lane >= k && lane < n) g[lane + (int64_t)k * ld] = l;
+        const T* ck = colbuf + k;
+#pragma unroll
+        for (int j = 1; j < W; ++j) x[j - 1] = x[j] - l * ck[j];
+        wave_sync();
     }
+}
+template <typename T>
+__device__ __forceinline__ void chol_rows(T (&x)[64], T* colbuf, int n, int c, int* info, int col0, T* g, int64_t ld)
+{
+    chol_seg<0>(x, colbuf, n, c, info, col0, g, ld);
+    chol_seg<16>(x, colbuf, n, c, info, col0, g, ld);
+    chol_seg<32>(x, colbuf, n, c, info, col0, g, ld);
+    chol_seg<48>(x, colbuf, n, c, info, col0, g, ld);
+}
+
+// x (one row, kb columns) <- x L^{-T} by forward substitution, L given as
+// D[k][m] = L[m][k] (D rows are kDLd >= 2 * 64 long and zero past column
+// 63, which the shifted reads D[k][k + j] reach).  Column k of the result
+// goes to out[k * stride].
+template <int K0, typename T>
+__device__ __forceinline__ void row_solve_seg(T (&x)[64], const T (*D)[kDLd], int kb, T* out, int64_t stride)
+{
+    constexpr int W = 64 - K0;
+    const int kend = min(K0 + 16, kb);
+#pragma unroll 1
+    for (int k = K0; k < kend; ++k) {
+        const T* dk = D[k] + k;
+        const T x0 = x[0] / dk[0];
+        out[(int64_t)k * stride] = x0;
+#pragma unroll
+        for (int j = 1; j < W; ++j) x[j - 1] = x[j] - x0 * dk[j];
+    }
+}
+template <typename T>
+__device__ __forceinline__ void row_solve(T (&x)[64], const T (*D)[kDLd], int kb, T* out, int64_t stride)
+{
+    row_solve_seg<0>(x, D, kb, out, stride);
+    row_solve_seg<16>(x, D, kb, out, stride);
+    row_solve_seg<32>(x, D, kb, out, stride);
+    row_solve_seg<48>(x, D, kb, out, stride);
+}
+
+// The inverse of the kb x kb lower block held in D, stored column-major
+// (ld 64, zero-padded to 64 x 64) at inv: row i of the identity solved
+// against L^T gives row i of L^{-T}, i.e. column i of L^{-1}.  Threads
+// 0..63 take one identity row each.
+template <typename T>
+__device__ __forceinline__ void store_inverse(const T (*D)[kDLd], int kb, T* inv)
+{
+    const int i = threadIdx.x;
+    if (i >= 64) return;
+    T x[64];
+#pragma unroll
+    for (int j = 0; j < 64; ++j) x[j] = j == i ? T(1) : T(0);
+    row_solve<T>(x, D, kb, inv + 64 * i, 1);
+    for (int k = kb; k < 64; ++k) inv[64 * i + k] = T(0);
 }
 
 }  // namespace
@@ -125,73 +203,77 @@ __global__ __launch_bounds__(kThreads) void k_extend_add(T* pool, const int* lid
 }
 
 // -------------------------------------------------------- small fronts
-// Whole partial factorization of a front with n <= 64 inside LDS.
+// Whole partial factorization of a front with n <= 64 by one wave: L in
+// the first c columns, the Schur complement in the trailing block, and the
+// inverse of the c x c diagonal block for the solves.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_small_front(T* pool, const NodeDev* nodes, const int* tasks, int* info)
+__global__ __launch_bounds__(64) void k_small_front(T* pool, const NodeDev* nodes, const int* tasks, int* info)
 {
     const NodeDev nd = nodes[tasks[blockIdx.x]];
-    __shared__ T F[kSmallFront][65];
-    const int n = nd.n, tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    __shared__ T D[kPanel][kDLd];
+    __shared__ T colbuf[128];
+    const int n = nd.n, c = nd.c, lane = threadIdx.x;
     T* g = pool + nd.front;
-    for (int q = ty; q < n; q += 4)
-        if (tx >= q && tx < n) F[q][tx] = g[tx + (int64_t)q * n];
+    for (int e = lane; e < kPanel * kDLd; e += 64) (&D[0][0])[e] = T(0);
+    colbuf[lane] = T(0);
+    colbuf[lane + 64] = T(0);
+    T x[64];
+#pragma unroll
+    for (int j = 0; j < 64; ++j) x[j] = (lane < n && j <= lane) ? g[lane + (int64_t)j * n] : T(0);
     __syncthreads();
-    chol_lds<T>(F, n, nd.c, info, nd.first);
-    for (int q = ty; q < n; q += 4)
-        if (tx >= q && tx < n) g[tx + (int64_t)q * n] = F[q][tx];
+    chol_rows<T>(x, colbuf, n, c, info, nd.first, &D[0][0], kDLd);
+    for (int k = 0; k < c; ++k)
+        if (lane >= k && lane < n) g[lane + (int64_t)k * n] = D[k][lane];
+#pragma unroll
+    for (int j = 0; j < 64; ++j)
+        if (lane < n && lane >= c + j) g[lane + (int64_t)(c + j) * n] = x[j];
+    store_inverse<T>(D, c, pool + nd.inv);
 }
 
 // ------------------------------------------------------ large fronts
-// Cholesky of the kb x kb diagonal block [k0, k0+kb) of a large front.
+// One panel step [k0, k0+kb) of a large front.  Wave 0 of every workgroup
+// factors the kb x kb diagonal block itself (it is small, and this removes a
+// launch and a dependency from the step).  A chunk task (row0 >= 0) then solves its 256
+// rows X L^T = P, one thread per row.  The writer task (row0 < 0) stores the
+// factored block into scratch slot `slot` -- the front's own copy is
+// rewritten by the update launch that follows, once no workgroup of this
+// launch can still be reading the unfactored block -- and the inverse of
+// the block for the solves.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_potrf(T* pool, const NodeDev* nodes, const PanelTask* tasks, int* info)
+__global__ __launch_bounds__(kThreads) void k_panel(T* pool, const NodeDev* nodes, const SolveTask* tasks,
+                                                    int64_t scratch, int* info)
 {
-    const PanelTask t = tasks[blockIdx.x];
+    const SolveTask t = tasks[blockIdx.x];
     const NodeDev nd = nodes[t.node];
-    __shared__ T D[kPanel][65];
-    const int n = nd.n, kb = t.kb, tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    T* g = pool + nd.front + t.k0 + (int64_t)t.k0 * n;
-    for (int q = ty; q < kb; q += 4)
-        if (tx >= q && tx < kb) D[q][tx] = g[tx + (int64_t)q * n];
+    __shared__ T D[kPanel][kDLd];
+    __shared__ T colbuf[128];
+    const int n = nd.n, kb = t.kb, tid = threadIdx.x, lane = tid & 63;
+    for (int e = tid; e < kPanel * kDLd; e += kThreads) (&D[0][0])[e] = T(0);
+    if (tid < 128) colbuf[tid] = T(0);
     __syncthreads();
-    chol_lds<T>(D, kb, kb, info, nd.first + t.k0);
-    for (int q = ty; q < kb; q += 4)
-        if (tx >= q && tx < kb) g[tx + (int64_t)q * n] = D[q][tx];
-}
-
-// X L^T = P for the 256 rows [row0, row0+256) of the panel [k0, k0+kb),
-// L the factored diagonal block.  One thread per row.  The row lives in
-// registers as a shift register: at step k, x[0] is column k; it is
-// finished by one division, stored, and the remaining columns are updated
-// and shifted down by one.  The inner loop is fully unrolled with static
-// register indices (no scratch); L sits in LDS as D[k][m] = L[m][k], zero
-// past the block, and is read as a broadcast.  No barrier inside the solve.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_trsm(T* pool, const NodeDev* nodes, const PanelTask* tasks)
-{
-    const PanelTask t = tasks[blockIdx.x];
-    const NodeDev nd = nodes[t.node];
-    __shared__ T D[kPanel][2 * kPanel + 1];
-    const int n = nd.n, kb = t.kb, tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    const T* gd = pool + nd.front + t.k0 + (int64_t)t.k0 * n;
-    for (int q = ty; q < kPanel; q += 4) {
-        D[q][tx] = (q < kb && tx < kb && tx >= q) ? gd[tx + (int64_t)q * n] : T(0);
-        D[q][tx + kPanel] = T(0);
+    if (tid < 64) {
+        const T* gd = pool + nd.front + t.k0 + (int64_t)t.k0 * n;
+        T xd[64];
+#pragma unroll
+        for (int j = 0; j < 64; ++j) xd[j] = (lane < kb && j <= lane) ? gd[lane + (int64_t)j * n] : T(0);
+        chol_rows<T>(xd, colbuf, kb, kb, info, nd.first + t.k0, &D[0][0], kDLd);
     }
     __syncthreads();
-    const int row = t.row0 + threadIdx.x;
-    if (row >= n) return;
-    T* gp = pool + nd.front + row + (int64_t)t.k0 * n;
-    T x[kPanel];
+    if (t.row0 >= 0) {
+        const int row = t.row0 + tid;
+        if (row >= n) return;
+        T* gp = pool + nd.front + row + (int64_t)t.k0 * n;
+        T x[64];
 #pragma unroll
-    for (int j = 0; j < kPanel; ++j) x[j] = j < kb ? gp[(int64_t)j * n] : T(0);
-    for (int k = 0; k < kb; ++k) {
-        const T x0 = x[0] / D[k][k];
-        gp[(int64_t)k * n] = x0;
-        const T* dk = &D[k][k];
-#pragma unroll
-        for (int j = 1; j < kPanel; ++j) x[j - 1] = x[j] - x0 * dk[j];
-        x[kPanel - 1] = T(0);
+        for (int j = 0; j < 64; ++j) x[j] = j < kb ? gp[(int64_t)j * n] : T(0);
+        row_solve<T>(x, D, kb, gp, n);
+    } else {
+        T* sc = pool + scratch + (int64_t)t.slot * kPanel * kPanel;
+        for (int e = tid; e < kPanel * kPanel; e += kThreads) {
+            int i = e & 63, k = e >> 6;
+            sc[e] = (k < kb && i < kb && i >= k) ? D[k][i] : T(0);
+        }
+        store_inverse<T>(D, kb, pool + nd.inv + (int64_t)(t.k0 / kPanel) * kPanel * kPanel);
     }
 }
 
@@ -199,16 +281,26 @@ __global__ __launch_bounds__(kThreads) void k_trsm(T* pool, const NodeDev* nodes
 // each own a 32 x 32 quarter of the 64 x 64 tile (2 x 2 MFMA blocks).  K is
 // staged through LDS in chunks of 16 with the next chunk's global loads
 // issued before the current chunk's MFMAs.  The result goes through LDS so
-// the read-modify-write of C is coalesced along columns.
+// the read-modify-write of C is coalesced along columns.  k == 0 copies the
+// lower part of A into C instead (the factored diagonal block of a panel).
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_gemm(T* pool, const GemmTask* tasks)
 {
     typedef typename Mfma<T>::acc_t acc_t;
     const GemmTask t = tasks[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (t.k == 0) {
+        const T* A = pool + t.a;
+        T* C = pool + t.c;
+        for (int e = tid; e < t.m * t.n; e += kThreads) {
+            int i = e % t.m, j = e / t.m;
+            if (i >= j) C[i + (int64_t)j * t.ldc] = A[i + (int64_t)j * t.lda];
+        }
+        return;
+    }
     __shared__ T As[16][kTile + 4];
     __shared__ T Bs[16][kTile + 4];
     __shared__ T Cs[kTile][kTile + 1];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wr = (wv & 1) * 32, wc = (wv >> 1) * 32;
     const int li = tid & 63, lp = tid >> 6;
     const T* A = pool + t.a;
@@ -262,20 +354,26 @@ __global__ __launch_bounds__(kThreads) void k_gemm(T* pool, const GemmTask* task
 }
 
 // ------------------------------------------------------------------ solve
-// Forward sweep of one front (the level's fronts run in parallel): gather
-// b and the children's update parts (in child order), then L11^{-1} on the
-// pivot part and the GEMV update of the rows below, block by block.  The
-// diagonal 64 x 64 solves are done by wave 0 with lane shuffles.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_solve_fwd(const T* pool, double* zpool, const double* b, int ldb, int nrhs,
-                                                        const NodeDev* nodes, const int* lidx, const int* child_ptr,
-                                                        const int* child_list, const int* tasks)
+// Blocked supernodal sweeps, level by level (the level's fronts in one
+// launch).  Front s's solve slice (per right-hand side, ld n and ld c):
+// Z[0:n] is the right-hand side being reduced (pivot part, then the update
+// part passed to the parent), Y[0:c] the solved pivot values (forward: y,
+// backward: x).  Diagonal blocks are applied through their stored inverses,
+// so no kernel runs a sequential loop: forward one launch per 64-column
+// block, backward two (seq_forward_m.f90:14-81, seq_backward_m.f90:14-131).
+constexpr int kRhsGroup = 8;  // right-hand sides processed together
+
+// Forward, first launch of a level: pivot part of Z from b, update part
+// zero, then the children's update parts added in child order (scatter_add,
+// sparse_add_subroutines_m.f90:13-30).
+__global__ __launch_bounds__(kThreads) void k_fwd_gather(double* zpool, const double* b, int ldb, int nrhs,
+                                                         const NodeDev* nodes, const int* lidx, const int* child_ptr,
+                                                         const int* child_list, const int* tasks)
 {
     const int s = tasks[blockIdx.x];
     const NodeDev nd = nodes[s];
-    const int n = nd.n, c = nd.c, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int n = nd.n, c = nd.c, tid = threadIdx.x;
     double* Z = zpool + nd.z * nrhs;
-    const T* L = pool + nd.front;
     for (int e = tid; e < n * nrhs; e += kThreads) {
         int i = e % n, r = e / n;
         Z[i + (int64_t)r * n] = i < c ? b[(int64_t)(nd.first + i) + (int64_t)r * ldb] : 0.0;
@@ -290,81 +388,190 @@ __global__ __launch_bounds__(kThreads) void k_solve_fwd(const T* pool, double* z
         }
         __syncthreads();
     }
-    for (int k0 = 0; k0 < c; k0 += 64) {
-        const int kb = min(64, c - k0);
-        if (wv == 0) {
-            for (int r = 0; r < nrhs; ++r) {
-                double zi = lane < kb ? Z[k0 + lane + (int64_t)r * n] : 0.0;
-                for (int k = 0; k < kb; ++k) {
-                    double zk = __shfl(zi, k) / (double)L[(k0 + k) + (int64_t)(k0 + k) * n];
-                    if (lane == k) zi = zk;
-                    else if (lane > k && lane < kb) zi -= (double)L[(k0 + lane) + (int64_t)(k0 + k) * n] * zk;
-                }
-                if (lane < kb) Z[k0 + lane + (int64_t)r * n] = zi;
-            }
+}
+
+// Stage inv(L_tt) (64 x 64, column-major in the pool) in LDS as LI[col][row].
+template <typename T>
+__device__ void load_inverse(const T* inv, double (*LI)[65])
+{
+#pragma unroll
+    for (int e = threadIdx.x; e < 64 * 64; e += kThreads) LI[e >> 6][e & 63] = (double)inv[e];
+}
+
+// Forward block step: y_t = inv(L_tt) Z[k0:k0+kb] (every workgroup computes
+// it from the unmodified Z block); the writer task (row0 < 0) stores y_t in
+// Y, each chunk task subtracts L[rows, t] y_t from its 256 rows of Z
+// (DTRSV "U","T" and DGEMV "T", solve_kernel_m.f90:4-23).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_fwd_step(const T* pool, double* zpool, int nrhs, const NodeDev* nodes,
+                                                       const SolveTask* tasks)
+{
+    const SolveTask t = tasks[blockIdx.x];
+    const NodeDev nd = nodes[t.node];
+    __shared__ double LI[64][65];
+    __shared__ double zt[64][kRhsGroup], yt[64][kRhsGroup];
+    const int n = nd.n, c = nd.c, kb = t.kb, tid = threadIdx.x, lane = tid & 63, ty = tid >> 6;
+    double* Z = zpool + nd.z * nrhs;
+    double* Y = Z + (int64_t)n * nrhs;
+    load_inverse<T>(pool + nd.inv + (int64_t)(t.k0 / 64) * 4096, LI);
+    const int row = t.row0 + tid;
+    const T* L = pool + nd.front + row + (int64_t)t.k0 * n;
+    for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
+        const int nr = min(kRhsGroup, nrhs - r0);
+        for (int e = tid; e < 64 * kRhsGroup; e += kThreads) {
+            int k = e & 63, r = e >> 6;
+            zt[k][r] = (k < kb && r < nr) ? Z[t.k0 + k + (int64_t)(r0 + r) * n] : 0.0;
         }
         __syncthreads();
-        for (int row = k0 + kb + tid; row < n; row += kThreads)
-            for (int r = 0; r < nrhs; ++r) {
-                double acc = 0.0;
-                for (int k = 0; k < kb; ++k) acc += (double)L[row + (int64_t)(k0 + k) * n] * Z[k0 + k + (int64_t)r * n];
-                Z[row + (int64_t)r * n] -= acc;
+        for (int r = ty; r < kRhsGroup; r += 4) {
+            double acc = 0.0;
+            for (int k = 0; k < kb; ++k) acc += LI[k][lane] * zt[k][r];
+            yt[lane][r] = acc;
+        }
+        __syncthreads();
+        if (t.row0 < 0) {
+            for (int e = tid; e < kb * nr; e += kThreads) {
+                int i = e % kb, r = e / kb;
+                Y[t.k0 + i + (int64_t)(r0 + r) * c] = yt[i][r];
             }
+        } else if (row < n) {
+            double acc[kRhsGroup];
+#pragma unroll
+            for (int r = 0; r < kRhsGroup; ++r) acc[r] = 0.0;
+            // 16 column loads in flight per group (zero past kb; the clamped
+            // yt index stays in the array and meets a zero factor)
+#pragma unroll 1
+            for (int k0 = 0; k0 < kb; k0 += 16) {
+                double l[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) l[k] = k0 + k < kb ? (double)L[(int64_t)(k0 + k) * n] : 0.0;
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const int kk = min(k0 + k, 63);
+#pragma unroll
+                    for (int r = 0; r < kRhsGroup; ++r) acc[r] += l[k] * yt[kk][r];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < kRhsGroup; ++r)
+                if (r < nr) Z[row + (int64_t)(r0 + r) * n] -= acc[r];
+        }
         __syncthreads();
     }
 }
 
-// Backward sweep of one front: gather the parent's solution at the update
-// rows, then for pivot blocks in descending order subtract L21^T x and solve
-// with the diagonal block transposed; write the pivot part of x.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_solve_bwd(const T* pool, double* zpool, double* x, int ldx, int nrhs,
-                                                        const NodeDev* nodes, const int* lidx, const int* tasks)
+// Backward, first launch of a level: the update part of Z takes the
+// parent's solution at the parent-front positions -- pivot positions from
+// the parent's Y, update positions from the parent's Z (gather,
+// sparse_add_subroutines_m.f90:32-49).
+__global__ __launch_bounds__(kThreads) void k_bwd_gather(double* zpool, int nrhs, const NodeDev* nodes, const int* lidx,
+                                                         const int* tasks)
 {
-    const int s = tasks[blockIdx.x];
-    const NodeDev nd = nodes[s];
-    const int n = nd.n, c = nd.c, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const NodeDev nd = nodes[tasks[blockIdx.x]];
+    if (nd.parent < 0) return;
+    const NodeDev pa = nodes[nd.parent];
     double* Z = zpool + nd.z * nrhs;
-    const T* L = pool + nd.front;
-    if (nd.parent >= 0) {
-        const NodeDev pa = nodes[nd.parent];
-        const double* Zp = zpool + pa.z * nrhs;
-        for (int e = tid; e < nd.w * nrhs; e += kThreads) {
-            int i = e % nd.w, r = e / nd.w;
-            Z[(c + i) + (int64_t)r * n] = Zp[(lidx[nd.lidx + i] - 1) + (int64_t)r * pa.n];
+    const double* Zp = zpool + pa.z * nrhs;
+    const double* Yp = Zp + (int64_t)pa.n * nrhs;
+    for (int e = threadIdx.x; e < nd.w * nrhs; e += kThreads) {
+        int i = e % nd.w, r = e / nd.w;
+        int p = lidx[nd.lidx + i] - 1;
+        Z[(nd.c + i) + (int64_t)r * nd.n] = p < pa.c ? Yp[p + (int64_t)r * pa.c] : Zp[p + (int64_t)r * pa.n];
+    }
+}
+
+// Backward: partial dot products P[slot][k][r] = sum over this chunk's rows
+// of L[row][k0+k] x[row][r], x from Y (pivot rows) or Z (update rows)
+// (DGEMV "N", solve_kernel_m.f90:25-30).  64-row sub-blocks of L are staged
+// transposed through LDS so the global reads and the per-column dot
+// products are both conflict-free.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_bwd_partial(const T* pool, double* zpool, double* part, int nrhs,
+                                                          const NodeDev* nodes, const SolveTask* tasks)
+{
+    const SolveTask t = tasks[blockIdx.x];
+    const NodeDev nd = nodes[t.node];
+    __shared__ double Ls[64][65];  // Ls[k][i] = L[row_i][k0+k]
+    __shared__ double Xs[64][kRhsGroup];
+    const int n = nd.n, c = nd.c, kb = t.kb, tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const double* Z = zpool + nd.z * nrhs;
+    const double* Y = Z + (int64_t)n * nrhs;
+    const T* L = pool + nd.front + (int64_t)t.k0 * n;
+    const int rend = min(n, t.row0 + 256);
+    double* P = part + (int64_t)t.slot * 64 * nrhs;
+    for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
+        const int nr = min(kRhsGroup, nrhs - r0);
+        double acc0 = 0.0, acc1 = 0.0;  // (k = tx, r = ty) and (k = tx, r = ty + 4)
+        for (int rb = t.row0; rb < rend; rb += 64) {
+            const int rows = min(64, rend - rb);
+            __syncthreads();
+#pragma unroll
+            for (int q = ty; q < 64; q += 4)
+                if (q < kb) Ls[q][tx] = tx < rows ? (double)L[rb + tx + (int64_t)q * n] : 0.0;
+            for (int e = threadIdx.x; e < 64 * nr; e += kThreads) {
+                int i = e & 63, r = e >> 6, g = rb + i;
+                Xs[i][r] = i >= rows ? 0.0 : g < c ? Y[g + (int64_t)(r0 + r) * c] : Z[g + (int64_t)(r0 + r) * n];
+            }
+            __syncthreads();
+            if (tx < kb) {
+                if (ty < nr)
+                    for (int i = 0; i < rows; ++i) acc0 += Ls[tx][i] * Xs[i][ty];
+                if (ty + 4 < nr)
+                    for (int i = 0; i < rows; ++i) acc1 += Ls[tx][i] * Xs[i][ty + 4];
+            }
+        }
+        if (tx < kb) {
+            if (ty < nr) P[tx + (int64_t)(r0 + ty) * 64] = acc0;
+            if (ty + 4 < nr) P[tx + (int64_t)(r0 + ty + 4) * 64] = acc1;
+        }
+    }
+}
+
+// Backward block step: z = Y[k0:k0+kb] - sum of the chunk partials (in
+// chunk order), x_t = inv(L_tt)^T z into Y (DTRSV "U","N",
+// solve_kernel_m.f90:11-16).  The front's first block writes its pivot part
+// of x out.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_bwd_diag(const T* pool, double* zpool, const double* part, double* x,
+                                                       int ldx, int nrhs, const NodeDev* nodes, const SolveTask* tasks)
+{
+    const SolveTask t = tasks[blockIdx.x];
+    const NodeDev nd = nodes[t.node];
+    __shared__ double LI[64][65];
+    __shared__ double zt[64][kRhsGroup];
+    const int n = nd.n, c = nd.c, kb = t.kb, tid = threadIdx.x, lane = tid & 63, ty = tid >> 6;
+    double* Y = zpool + nd.z * nrhs + (int64_t)n * nrhs;
+    load_inverse<T>(pool + nd.inv + (int64_t)(t.k0 / 64) * 4096, LI);
+    for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
+        const int nr = min(kRhsGroup, nrhs - r0);
+        for (int e = tid; e < 64 * kRhsGroup; e += kThreads) {
+            int k = e & 63, r = e >> 6;
+            double z = 0.0;
+            if (k < kb && r < nr) {
+                z = Y[t.k0 + k + (int64_t)(r0 + r) * c];
+                for (int ch = 0; ch < t.nchunk; ++ch) z -= part[(int64_t)(t.slot + ch) * 64 * nrhs + k + (int64_t)(r0 + r) * 64];
+            }
+            zt[k][r] = z;
         }
         __syncthreads();
-    }
-    const int nblk = (c + 63) / 64;
-    for (int bk = nblk - 1; bk >= 0; --bk) {
-        const int k0 = bk * 64, kb = min(64, c - k0);
-        // z_blk -= L[below, blk]^T z[below]: wave wv takes columns wv, wv+4, ...
-        for (int r = 0; r < nrhs; ++r) {
-            for (int k = wv; k < kb; k += 4) {
-                double acc = 0.0;
-                const T* col = L + (int64_t)(k0 + k) * n;
-                for (int row = k0 + kb + lane; row < n; row += 64) acc += (double)col[row] * Z[row + (int64_t)r * n];
-                for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-                if (lane == 0) Z[k0 + k + (int64_t)r * n] -= acc;
+        for (int r = ty; r < nr; r += 4) {
+            double acc = 0.0;
+            for (int m = 0; m < kb; ++m) acc += LI[lane][m] * zt[m][r];
+            if (lane < kb) {
+                Y[t.k0 + lane + (int64_t)(r0 + r) * c] = acc;
+                if (t.k0 == 0) x[(int64_t)(nd.first + lane) + (int64_t)(r0 + r) * ldx] = acc;
             }
         }
         __syncthreads();
-        if (wv == 0) {
-            for (int r = 0; r < nrhs; ++r) {
-                double zi = lane < kb ? Z[k0 + lane + (int64_t)r * n] : 0.0;
-                for (int k = kb - 1; k >= 0; --k) {
-                    double xk = __shfl(zi, k) / (double)L[(k0 + k) + (int64_t)(k0 + k) * n];
-                    if (lane == k) zi = xk;
-                    else if (lane < k) zi -= (double)L[(k0 + k) + (int64_t)(k0 + lane) * n] * xk;
-                }
-                if (lane < kb) Z[k0 + lane + (int64_t)r * n] = zi;
-            }
-        }
-        __syncthreads();
     }
-    for (int e = tid; e < c * nrhs; e += kThreads) {
-        int i = e % c, r = e / c;
-        x[(int64_t)(nd.first + i) + (int64_t)r * ldx] = Z[i + (int64_t)r * n];
+    // The front's first block is its last one solved: the other blocks'
+    // values were written to Y by earlier launches.
+    if (t.k0 == 0 && c > kb) {
+        const int rest = c - kb;
+        for (int e = tid; e < rest * nrhs; e += kThreads) {
+            int i = kb + e % rest, r = e / rest;
+            x[(int64_t)(nd.first + i) + (int64_t)r * ldx] = Y[i + (int64_t)r * c];
+        }
     }
 }
 
@@ -490,37 +697,49 @@ void launch_extend_add(T* pool, const int* lidx, const NodeDev* nodes, const Ext
 template <typename T>
 void launch_small_front(T* pool, const NodeDev* nodes, const int* tasks, int count, int* info, hipStream_t st)
 {
-    if (count > 0) hipLaunchKernelGGL(k_small_front<T>, dim3(count), dim3(kThreads), 0, st, pool, nodes, tasks, info);
+    if (count > 0) hipLaunchKernelGGL(k_small_front<T>, dim3(count), dim3(64), 0, st, pool, nodes, tasks, info);
 }
 template <typename T>
-void launch_potrf(T* pool, const NodeDev* nodes, const PanelTask* tasks, int count, int* info, hipStream_t st)
+void launch_panel(T* pool, const NodeDev* nodes, const SolveTask* tasks, int count, int64_t scratch, int* info,
+                  hipStream_t st)
 {
-    if (count > 0) hipLaunchKernelGGL(k_potrf<T>, dim3(count), dim3(kThreads), 0, st, pool, nodes, tasks, info);
-}
-template <typename T>
-void launch_trsm(T* pool, const NodeDev* nodes, const PanelTask* tasks, int count, hipStream_t st)
-{
-    if (count > 0) hipLaunchKernelGGL(k_trsm<T>, dim3(count), dim3(kThreads), 0, st, pool, nodes, tasks);
+    if (count > 0) hipLaunchKernelGGL(k_panel<T>, dim3(count), dim3(kThreads), 0, st, pool, nodes, tasks, scratch, info);
 }
 template <typename T> void launch_gemm(T* pool, const GemmTask* tasks, int count, hipStream_t st)
 {
     if (count > 0) hipLaunchKernelGGL(k_gemm<T>, dim3(count), dim3(kThreads), 0, st, pool, tasks);
 }
-template <typename T>
-void launch_solve_fwd(const T* pool, double* zpool, const double* b, int ldb, int nrhs, const NodeDev* nodes,
-                      const int* lidx, const int* child_ptr, const int* child_list, const int* tasks, int count,
-                      hipStream_t st)
+void launch_fwd_gather(double* zpool, const double* b, int ldb, int nrhs, const NodeDev* nodes, const int* lidx,
+                       const int* child_ptr, const int* child_list, const int* tasks, int count, hipStream_t st)
 {
     if (count > 0)
-        hipLaunchKernelGGL(k_solve_fwd<T>, dim3(count), dim3(kThreads), 0, st, pool, zpool, b, ldb, nrhs, nodes, lidx,
-                           child_ptr, child_list, tasks);
+        hipLaunchKernelGGL(k_fwd_gather, dim3(count), dim3(kThreads), 0, st, zpool, b, ldb, nrhs, nodes, lidx, child_ptr,
+                           child_list, tasks);
 }
 template <typename T>
-void launch_solve_bwd(const T* pool, double* zpool, double* x, int ldx, int nrhs, const NodeDev* nodes, const int* lidx,
-                      const int* tasks, int count, hipStream_t st)
+void launch_fwd_step(const T* pool, double* zpool, int nrhs, const NodeDev* nodes, const SolveTask* tasks, int count,
+                     hipStream_t st)
+{
+    if (count > 0) hipLaunchKernelGGL(k_fwd_step<T>, dim3(count), dim3(kThreads), 0, st, pool, zpool, nrhs, nodes, tasks);
+}
+void launch_bwd_gather(double* zpool, int nrhs, const NodeDev* nodes, const int* lidx, const int* tasks, int count,
+                       hipStream_t st)
+{
+    if (count > 0) hipLaunchKernelGGL(k_bwd_gather, dim3(count), dim3(kThreads), 0, st, zpool, nrhs, nodes, lidx, tasks);
+}
+template <typename T>
+void launch_bwd_partial(const T* pool, double* zpool, double* part, int nrhs, const NodeDev* nodes,
+                        const SolveTask* tasks, int count, hipStream_t st)
 {
     if (count > 0)
-        hipLaunchKernelGGL(k_solve_bwd<T>, dim3(count), dim3(kThreads), 0, st, pool, zpool, x, ldx, nrhs, nodes, lidx,
+        hipLaunchKernelGGL(k_bwd_partial<T>, dim3(count), dim3(kThreads), 0, st, pool, zpool, part, nrhs, nodes, tasks);
+}
+template <typename T>
+void launch_bwd_diag(const T* pool, double* zpool, const double* part, double* x, int ldx, int nrhs,
+                     const NodeDev* nodes, const SolveTask* tasks, int count, hipStream_t st)
+{
+    if (count > 0)
+        hipLaunchKernelGGL(k_bwd_diag<T>, dim3(count), dim3(kThreads), 0, st, pool, zpool, part, x, ldx, nrhs, nodes,
                            tasks);
 }
 void launch_gather_rows(double* out, int ld_out, const double* in, int ld_in, const int* perm, int n, int nrhs,
@@ -545,13 +764,13 @@ void launch_scatter_rows(double* out, int ld_out, const double* in, int ld_in, c
     template void launch_scatter_a<T>(T*, const double*, const int64_t*, const int64_t*, int64_t, hipStream_t);        \
     template void launch_extend_add<T>(T*, const int*, const NodeDev*, const ExtTask*, int, hipStream_t);              \
     template void launch_small_front<T>(T*, const NodeDev*, const int*, int, int*, hipStream_t);                       \
-    template void launch_potrf<T>(T*, const NodeDev*, const PanelTask*, int, int*, hipStream_t);                       \
-    template void launch_trsm<T>(T*, const NodeDev*, const PanelTask*, int, hipStream_t);                              \
+    template void launch_panel<T>(T*, const NodeDev*, const SolveTask*, int, int64_t, int*, hipStream_t);              \
     template void launch_gemm<T>(T*, const GemmTask*, int, hipStream_t);                                               \
-    template void launch_solve_fwd<T>(const T*, double*, const double*, int, int, const NodeDev*, const int*,          \
-                                      const int*, const int*, const int*, int, hipStream_t);                           \
-    template void launch_solve_bwd<T>(const T*, double*, double*, int, int, const NodeDev*, const int*, const int*,    \
-                                      int, hipStream_t);
+    template void launch_fwd_step<T>(const T*, double*, int, const NodeDev*, const SolveTask*, int, hipStream_t);       \
+    template void launch_bwd_partial<T>(const T*, double*, double*, int, const NodeDev*, const SolveTask*, int,        \
+                                        hipStream_t);                                                                  \
+    template void launch_bwd_diag<T>(const T*, double*, const double*, double*, int, int, const NodeDev*,              \
+                                     const SolveTask*, int, hipStream_t);
 MFD_INSTANTIATE(double)
 MFD_INSTANTIATE(float)
 

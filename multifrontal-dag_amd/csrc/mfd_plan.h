@@ -18,11 +18,12 @@ constexpr int kSmallFront = 64;  // fronts up to this order are factored whole i
 struct NodeDev {
     int64_t front;   // element offset of the front in the pool
     int64_t lidx;    // offset of this node's local_index (w entries) in the lidx array
-    int64_t z;       // element offset of the solve buffer (n x nrhs_cap) in the z pool
+    int64_t z;       // per-rhs offset of the solve slice ((n + c) entries per rhs) in the z pool
+    int64_t inv;     // element offset (in the pool) of the inverses of the 64 x 64 diagonal blocks
     int n, c, w;
     int parent;      // 0-based parent node, -1 for a root
     int first;       // 0-based first global (postordered) column
-    int pad;
+    int pad[3];
 };
 
 // Extend-add of one 64 x 64 tile (ti >= tj) of child's update matrix.
@@ -36,8 +37,18 @@ struct PanelTask {
     int node, k0, kb, row0;
 };
 
+// One block step of the blocked supernodal solves: the pivot block
+// [k0, k0+kb) of a front and, for the row-chunk kernels, the 256 rows from
+// row0.  slot/nchunk address the backward sweep's partial-sum scratch
+// (slot s holds 64 x nrhs doubles).
+struct SolveTask {
+    int node, k0, kb, row0;
+    int slot, nchunk, pad0, pad1;
+};
+
 // C (m x n tile, ldc) -= A (m x k, lda) * B (n x k, ldb)^T, offsets into the
 // front pool; tri != 0 keeps only the lower triangle (row >= col) of C.
+// k == 0 marks a copy task: the lower m x n block at a (lda) is copied to c.
 struct GemmTask {
     int64_t c, a, b;
     int ldc, lda, ldb;

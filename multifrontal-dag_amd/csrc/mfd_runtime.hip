@@ -74,10 +74,17 @@ struct mfd_handle {
     std::vector<NodeDev> nodes;
     std::vector<ExtTask> ext;
     std::vector<int> small;
-    std::vector<PanelTask> potrf, trsm;
+    std::vector<SolveTask> panel;       // fused diagonal factorization + TRSM tasks
+    int64_t scratch_off = 0;             // pool offset of the factored-diagonal scratch
     std::vector<GemmTask> gemm;
     std::vector<Launch> fseq;
     std::vector<int> level_ptr, level_nodes, child_ptr, child_list;
+    // solve schedule: kinds kFG/kFS (forward) and kBG/kBP/kBD (backward)
+    std::vector<SolveTask> fstep, bpart, bdiag;
+    std::vector<Launch> fwd_seq, bwd_seq;
+    int64_t part_slots = 0;
+    SolveTask *d_fstep = nullptr, *d_bpart = nullptr, *d_bdiag = nullptr;
+    double* d_part = nullptr;
     int64_t pool_elems = 0, z_per_rhs = 0;
     int num_levels = 0;
 
@@ -86,7 +93,7 @@ struct mfd_handle {
     int* d_lidx = nullptr;
     ExtTask* d_ext = nullptr;
     int* d_small = nullptr;
-    PanelTask *d_potrf = nullptr, *d_trsm = nullptr;
+    SolveTask* d_panel = nullptr;
     GemmTask* d_gemm = nullptr;
     int *d_level_nodes = nullptr, *d_child_ptr = nullptr, *d_child_list = nullptr;
     int64_t *d_dst = nullptr, *d_src = nullptr;
@@ -106,6 +113,13 @@ struct mfd_handle {
     unsigned long long* d_norms = nullptr;
 
     hipGraphExec_t graph[2] = {nullptr, nullptr};
+    struct SolveGraph {
+        int prec, nrhs;
+        const double* bp;
+        double* xp;
+        hipGraphExec_t exec;
+    };
+    std::vector<SolveGraph> sgraphs;
     hipEvent_t ev[4] = {};
     std::vector<hipEvent_t> prof_ev;
 
@@ -137,6 +151,12 @@ template <typename T> int upload(mfd_handle* h, T** p, const std::vector<T>& v)
     return MFD_OK;
 }
 
+void drop_solve_graphs(mfd_handle* h)
+{
+    for (auto& g : h->sgraphs) (void)hipGraphExecDestroy(g.exec);
+    h->sgraphs.clear();
+}
+
 void free_device(mfd_handle* h)
 {
     for (auto& g : h->graph)
@@ -144,6 +164,7 @@ void free_device(mfd_handle* h)
             (void)hipGraphExecDestroy(g);
             g = nullptr;
         }
+    drop_solve_graphs(h);
     for (void* p : h->allocs) (void)hipFree(p);
     h->allocs.clear();
     h->device_bytes = 0;
@@ -151,9 +172,11 @@ void free_device(mfd_handle* h)
     h->d_lidx = nullptr;
     h->d_ext = nullptr;
     h->d_small = nullptr;
-    h->d_potrf = h->d_trsm = nullptr;
+    h->d_panel = nullptr;
     h->d_gemm = nullptr;
     h->d_level_nodes = h->d_child_ptr = h->d_child_list = nullptr;
+    h->d_fstep = h->d_bpart = h->d_bdiag = nullptr;
+    h->d_part = nullptr;
     h->d_dst = h->d_src = nullptr;
     h->d_perm = nullptr;
     h->d_pool64 = nullptr;
@@ -171,12 +194,76 @@ void free_device(mfd_handle* h)
 int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 // ------------------------------------------------------------- schedule
+enum SolveKind { kFG = 100, kFS, kBG, kBP, kBD };
+
+// Solve schedule: per level, a gather launch, then for every 64-column pivot
+// block one forward launch (the block's writer task plus its row chunks) and
+// two backward launches (blocks in descending order: partial dot products
+// over row chunks, then their in-order sum and the diagonal block).
+// Partial-sum slots are reused from zero by every block step.
+void build_solve_plan(mfd_handle* h)
+{
+    h->fstep.clear();
+    h->bpart.clear();
+    h->bdiag.clear();
+    h->fwd_seq.clear();
+    h->bwd_seq.clear();
+    h->part_slots = 0;
+    auto push = [](std::vector<Launch>& seq, int kind, int64_t first, int64_t end) {
+        if (end > first) seq.push_back(Launch{kind, first, (int)(end - first)});
+    };
+    auto steps_of = [&](int L) {
+        int steps = 0;
+        for (int k = h->level_ptr[L]; k < h->level_ptr[L + 1]; ++k)
+            steps = std::max(steps, (h->nodes[h->level_nodes[k]].c + 63) / 64);
+        return steps;
+    };
+    for (int L = 0; L < h->num_levels; ++L) {
+        const int b0 = h->level_ptr[L], b1 = h->level_ptr[L + 1];
+        push(h->fwd_seq, kFG, b0, b1);
+        const int steps = steps_of(L);
+        for (int t = 0; t < steps; ++t) {
+            const int k0 = 64 * t;
+            int64_t f0 = (int64_t)h->fstep.size();
+            for (int k = b0; k < b1; ++k) {
+                const int s = h->level_nodes[k];
+                const NodeDev& d = h->nodes[s];
+                if (d.c <= k0) continue;
+                const int kb = std::min(64, d.c - k0);
+                h->fstep.push_back(SolveTask{s, k0, kb, -1, 0, 0, 0, 0});
+                for (int r0 = k0 + kb; r0 < d.n; r0 += 256) h->fstep.push_back(SolveTask{s, k0, kb, r0, 0, 0, 0, 0});
+            }
+            push(h->fwd_seq, kFS, f0, (int64_t)h->fstep.size());
+        }
+    }
+    for (int L = h->num_levels - 1; L >= 0; --L) {
+        const int b0 = h->level_ptr[L], b1 = h->level_ptr[L + 1];
+        push(h->bwd_seq, kBG, b0, b1);
+        for (int t = steps_of(L) - 1; t >= 0; --t) {
+            const int k0 = 64 * t;
+            int64_t f0 = (int64_t)h->bpart.size(), f1 = (int64_t)h->bdiag.size();
+            int slot = 0;
+            for (int k = b0; k < b1; ++k) {
+                const int s = h->level_nodes[k];
+                const NodeDev& d = h->nodes[s];
+                if (d.c <= k0) continue;
+                const int kb = std::min(64, d.c - k0), first = slot;
+                for (int r0 = k0 + kb; r0 < d.n; r0 += 256) h->bpart.push_back(SolveTask{s, k0, kb, r0, slot++, 0, 0, 0});
+                h->bdiag.push_back(SolveTask{s, k0, kb, 0, first, slot - first, 0, 0});
+            }
+            h->part_slots = std::max<int64_t>(h->part_slots, slot);
+            push(h->bwd_seq, kBP, f0, (int64_t)h->bpart.size());
+            push(h->bwd_seq, kBD, f1, (int64_t)h->bdiag.size());
+        }
+    }
+}
+
 void build_plan(mfd_handle* h)
 {
     const Structure& S = h->S;
     const int m = S.num_node;
     h->nodes.assign(m, NodeDev{});
-    int64_t off = 0, zoff = 0;
+    int64_t off = 0, zoff = 0, inv_blocks = 0;
     for (int s = 0; s < m; ++s) {
         NodeDev& d = h->nodes[s];
         d.n = S.c[s] + S.w[s];
@@ -188,8 +275,26 @@ void build_plan(mfd_handle* h)
         d.parent = S.parent[s] - 1;
         d.first = S.first[s] - 1;
         off += align_up((int64_t)d.n * d.n, 32);
-        zoff += d.n;
+        zoff += d.n + d.c;
+        d.inv = inv_blocks;  // rebased below
+        inv_blocks += (d.c + kPanel - 1) / kPanel;
     }
+    // pool layout: fronts | factored-diagonal scratch | diagonal-block inverses
+    int max_large = 0;
+    {
+        std::vector<int> lv_large;
+        std::vector<int> lev0(m, 0);
+        for (int s = 0; s < m; ++s)
+            for (int ch : S.children[s]) lev0[s] = std::max(lev0[s], lev0[ch - 1] + 1);
+        int nl = m ? *std::max_element(lev0.begin(), lev0.end()) + 1 : 0;
+        lv_large.assign(nl, 0);
+        for (int s = 0; s < m; ++s)
+            if (h->nodes[s].n > kSmallFront) max_large = std::max(max_large, ++lv_large[lev0[s]]);
+    }
+    h->scratch_off = off;
+    off += (int64_t)max_large * kPanel * kPanel;
+    for (int s = 0; s < m; ++s) h->nodes[s].inv = off + h->nodes[s].inv * kPanel * kPanel;
+    off += inv_blocks * kPanel * kPanel;
     h->pool_elems = off;
     h->z_per_rhs = zoff;
 
@@ -214,8 +319,7 @@ void build_plan(mfd_handle* h)
 
     h->ext.clear();
     h->small.clear();
-    h->potrf.clear();
-    h->trsm.clear();
+    h->panel.clear();
     h->gemm.clear();
     h->fseq.clear();
     auto push = [&](int kind, int64_t first, int64_t end) {
@@ -252,21 +356,37 @@ void build_plan(mfd_handle* h)
         for (int s : large) steps = std::max(steps, (h->nodes[s].c + kPanel - 1) / kPanel);
         for (int t = 0; t < steps; ++t) {
             const int k0 = t * kPanel;
-            int64_t f0 = (int64_t)h->potrf.size();
-            for (int s : large) {
-                const NodeDev& d = h->nodes[s];
-                if (d.c > k0) h->potrf.push_back(PanelTask{s, k0, std::min(kPanel, d.c - k0), 0});
-            }
-            push(MFD_K_POTRF, f0, (int64_t)h->potrf.size());
-            int64_t f1 = (int64_t)h->trsm.size();
-            for (int s : large) {
+            // panel launch: per front one writer task (slot = its index in
+            // this step) and the 256-row chunks below the diagonal block
+            int64_t f0 = (int64_t)h->panel.size();
+            int slot = 0;
+            std::vector<int> slot_of(large.size(), -1);
+            for (size_t q = 0; q < large.size(); ++q) {
+                const int s = large[q];
                 const NodeDev& d = h->nodes[s];
                 if (d.c <= k0) continue;
-                int kb = std::min(kPanel, d.c - k0);
-                for (int r0 = k0 + kb; r0 < d.n; r0 += 256) h->trsm.push_back(PanelTask{s, k0, kb, r0});
+                const int kb = std::min(kPanel, d.c - k0);
+                slot_of[q] = slot;
+                h->panel.push_back(SolveTask{s, k0, kb, -1, slot++, 0, 0, 0});
+                for (int r0 = k0 + kb; r0 < d.n; r0 += 256) h->panel.push_back(SolveTask{s, k0, kb, r0, 0, 0, 0, 0});
             }
-            push(MFD_K_TRSM, f1, (int64_t)h->trsm.size());
+            push(MFD_K_POTRF, f0, (int64_t)h->panel.size());
+            // update launch: copy each factored diagonal block into its front,
+            // then the trailing update of the remaining pivot columns
             int64_t f2 = (int64_t)h->gemm.size();
+            for (size_t q = 0; q < large.size(); ++q) {
+                if (slot_of[q] < 0) continue;
+                const NodeDev& d = h->nodes[large[q]];
+                GemmTask g{};
+                g.c = d.front + k0 + (int64_t)k0 * d.n;
+                g.a = h->scratch_off + (int64_t)slot_of[q] * kPanel * kPanel;
+                g.ldc = d.n;
+                g.lda = g.ldb = kPanel;
+                g.m = g.n = std::min(kPanel, d.c - k0);
+                g.k = 0;
+                g.tri = 1;
+                h->gemm.push_back(g);
+            }
             for (int s : large) {
                 const NodeDev& d = h->nodes[s];
                 if (d.c <= k0) continue;
@@ -306,6 +426,7 @@ void build_plan(mfd_handle* h)
         }
         push(MFD_K_SCHUR, f3, (int64_t)h->gemm.size());
     }
+    build_solve_plan(h);
 }
 
 // Full symmetric CSR (both triangles) of the postordered matrix for the
@@ -358,12 +479,14 @@ int setup_device(mfd_handle* h)
     MFD_TRY(upload(h, &h->d_lidx, S.lidx));
     MFD_TRY(upload(h, &h->d_ext, h->ext));
     MFD_TRY(upload(h, &h->d_small, h->small));
-    MFD_TRY(upload(h, &h->d_potrf, h->potrf));
-    MFD_TRY(upload(h, &h->d_trsm, h->trsm));
+    MFD_TRY(upload(h, &h->d_panel, h->panel));
     MFD_TRY(upload(h, &h->d_gemm, h->gemm));
     MFD_TRY(upload(h, &h->d_level_nodes, h->level_nodes));
     MFD_TRY(upload(h, &h->d_child_ptr, h->child_ptr));
     MFD_TRY(upload(h, &h->d_child_list, h->child_list));
+    MFD_TRY(upload(h, &h->d_fstep, h->fstep));
+    MFD_TRY(upload(h, &h->d_bpart, h->bpart));
+    MFD_TRY(upload(h, &h->d_bdiag, h->bdiag));
     // assembly map: A entry -> front position (coefficient_setter_m.f90:26-41)
     const int64_t nnz = (int64_t)S.snidx.size();
     std::vector<int64_t> dst(nnz), src(nnz);
@@ -407,6 +530,8 @@ int ensure_rhs(mfd_handle* h, int nrhs)
     MFD_TRY(dalloc(h, &h->d_r, n * nrhs));
     MFD_TRY(dalloc(h, &h->d_d, n * nrhs));
     MFD_TRY(dalloc(h, &h->d_norms, 3 * nrhs + 1));
+    MFD_TRY(dalloc(h, &h->d_part, h->part_slots * 64 * nrhs));
+    drop_solve_graphs(h);  // captured solves refer to the old buffers
     h->nrhs_cap = nrhs;
     return MFD_OK;
 }
@@ -423,8 +548,9 @@ template <typename T> void enqueue_launch(mfd_handle* h, const Launch& l)
     switch (l.kind) {
     case MFD_K_EXTEND_ADD: launch_extend_add<T>(pool, h->d_lidx, h->d_nodes, h->d_ext + l.first, l.count, st); break;
     case MFD_K_SMALL_FRONT: launch_small_front<T>(pool, h->d_nodes, h->d_small + l.first, l.count, h->d_info, st); break;
-    case MFD_K_POTRF: launch_potrf<T>(pool, h->d_nodes, h->d_potrf + l.first, l.count, h->d_info, st); break;
-    case MFD_K_TRSM: launch_trsm<T>(pool, h->d_nodes, h->d_trsm + l.first, l.count, st); break;
+    case MFD_K_POTRF:
+        launch_panel<T>(pool, h->d_nodes, h->d_panel + l.first, l.count, h->scratch_off, h->d_info, st);
+        break;
     case MFD_K_UPDATE:
     case MFD_K_SCHUR: launch_gemm<T>(pool, h->d_gemm + l.first, l.count, st); break;
     default: break;
@@ -539,12 +665,73 @@ template <typename T> void enqueue_solve(mfd_handle* h, const double* bp, double
     hipStream_t st = h->stream;
     const T* pool = pool_of<T>(h);
     const int n = h->S.n;
-    for (int L = 0; L < h->num_levels; ++L)
-        launch_solve_fwd<T>(pool, h->d_z, bp, n, nrhs, h->d_nodes, h->d_lidx, h->d_child_ptr, h->d_child_list,
-                            h->d_level_nodes + h->level_ptr[L], h->level_ptr[L + 1] - h->level_ptr[L], st);
-    for (int L = h->num_levels - 1; L >= 0; --L)
-        launch_solve_bwd<T>(pool, h->d_z, xp, n, nrhs, h->d_nodes, h->d_lidx, h->d_level_nodes + h->level_ptr[L],
-                            h->level_ptr[L + 1] - h->level_ptr[L], st);
+    const bool profile = (h->flags & MFD_FLAG_PROFILE) != 0;
+    hipEvent_t pe[3] = {};
+    if (profile)
+        for (auto& e : pe) (void)hipEventCreate(&e);
+    if (profile) (void)hipEventRecord(pe[0], st);
+    for (const Launch& l : h->fwd_seq) {
+        switch (l.kind) {
+        case kFG:
+            launch_fwd_gather(h->d_z, bp, n, nrhs, h->d_nodes, h->d_lidx, h->d_child_ptr, h->d_child_list,
+                              h->d_level_nodes + l.first, l.count, st);
+            break;
+        case kFS: launch_fwd_step<T>(pool, h->d_z, nrhs, h->d_nodes, h->d_fstep + l.first, l.count, st); break;
+        default: break;
+        }
+    }
+    if (profile) (void)hipEventRecord(pe[1], st);
+    for (const Launch& l : h->bwd_seq) {
+        switch (l.kind) {
+        case kBG: launch_bwd_gather(h->d_z, nrhs, h->d_nodes, h->d_lidx, h->d_level_nodes + l.first, l.count, st); break;
+        case kBP:
+            launch_bwd_partial<T>(pool, h->d_z, h->d_part, nrhs, h->d_nodes, h->d_bpart + l.first, l.count, st);
+            break;
+        case kBD:
+            launch_bwd_diag<T>(pool, h->d_z, h->d_part, xp, n, nrhs, h->d_nodes, h->d_bdiag + l.first, l.count, st);
+            break;
+        default: break;
+        }
+    }
+    if (profile) {
+        (void)hipEventRecord(pe[2], st);
+        (void)hipEventSynchronize(pe[2]);
+        float a = 0, b = 0;
+        (void)hipEventElapsedTime(&a, pe[0], pe[1]);
+        (void)hipEventElapsedTime(&b, pe[1], pe[2]);
+        h->ktime[MFD_K_SOLVE_FWD] += a;
+        h->kcount[MFD_K_SOLVE_FWD] += (int)h->fwd_seq.size();
+        h->ktime[MFD_K_SOLVE_BWD] += b;
+        h->kcount[MFD_K_SOLVE_BWD] += (int)h->bwd_seq.size();
+        for (auto e : pe) (void)hipEventDestroy(e);
+    }
+}
+
+// One forward + backward sweep, replayed from a captured graph when graphs
+// are enabled (one graph per precision, nrhs and buffer pair).
+template <typename T> int run_solve(mfd_handle* h, const double* bp, double* xp, int nrhs)
+{
+    const bool graph = (h->flags & MFD_FLAG_GRAPH) != 0 && (h->flags & MFD_FLAG_PROFILE) == 0;
+    if (!graph) {
+        enqueue_solve<T>(h, bp, xp, nrhs);
+        return MFD_OK;
+    }
+    const int prec = sizeof(T) == 8 ? 0 : 1;
+    for (const auto& g : h->sgraphs)
+        if (g.prec == prec && g.nrhs == nrhs && g.bp == bp && g.xp == xp) {
+            HIP_TRY(hipGraphLaunch(g.exec, h->stream));
+            return MFD_OK;
+        }
+    hipGraph_t g;
+    hipGraphExec_t exec;
+    HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_solve<T>(h, bp, xp, nrhs);
+    HIP_TRY(hipStreamEndCapture(h->stream, &g));
+    HIP_TRY(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    h->sgraphs.push_back({prec, nrhs, bp, xp, exec});
+    HIP_TRY(hipGraphLaunch(exec, h->stream));
+    return MFD_OK;
 }
 
 // Normwise backward error per right-hand side, max over them:
@@ -592,17 +779,17 @@ int solve_permuted_device(mfd_handle* h, int nrhs)
     h->st.refine_iters = 0;
     h->st.backward_error = -1;
     if (h->precision == MFD_FP64) {
-        enqueue_solve<double>(h, h->d_bp, h->d_xp, nrhs);
+        MFD_TRY(run_solve<double>(h, h->d_bp, h->d_xp, nrhs));
         return MFD_OK;
     }
-    enqueue_solve<float>(h, h->d_bp, h->d_xp, nrhs);
+    MFD_TRY(run_solve<float>(h, h->d_bp, h->d_xp, nrhs));
     double eta = 0, prev = INFINITY;
     int it = 0;
     for (;;) {
         MFD_TRY(backward_error(h, h->d_bp, h->d_xp, h->d_r, nrhs, &eta));
         if (!(eta > kRefineTarget) || it >= kMaxRefine || !(eta < 0.5 * prev)) break;
         prev = eta;
-        enqueue_solve<float>(h, h->d_r, h->d_d, nrhs);
+        MFD_TRY(run_solve<float>(h, h->d_r, h->d_d, nrhs));
         for (int k = 0; k < nrhs; ++k) launch_axpy(h->d_xp + k * n, h->d_d + k * n, n, st);
         ++it;
     }
@@ -612,7 +799,7 @@ int solve_permuted_device(mfd_handle* h, int nrhs)
         // refinement stalled: FP64 refactorization fallback
         int rc = factorize_common(h, MFD_FP64);
         if (rc != MFD_OK) return rc;
-        enqueue_solve<double>(h, h->d_bp, h->d_xp, nrhs);
+        MFD_TRY(run_solve<double>(h, h->d_bp, h->d_xp, nrhs));
         MFD_TRY(backward_error(h, h->d_bp, h->d_xp, h->d_r, nrhs, &eta));
         h->st.backward_error = eta;
         h->st.refine_iters = -1 - it;  // negative: fell back to FP64
@@ -860,7 +1047,7 @@ int mfd_get_stats(mfd_handle* h, mfd_stats* st)
     h->st.device_bytes = h->device_bytes;
     h->st.num_levels = h->num_levels;
     h->st.num_launches_factor = (int)h->fseq.size() + 3;
-    h->st.num_launches_solve = 2 * h->num_levels + 2;
+    h->st.num_launches_solve = (int)(h->fwd_seq.size() + h->bwd_seq.size()) + 2;
     *st = h->st;
     return MFD_OK;
 }
@@ -900,17 +1087,16 @@ int mfd_get_kernel_work(mfd_handle* h, int k, int s, double* flops, double* byte
             b += 2.0 * s * tri(d.n);
         }
         break;
-    case MFD_K_POTRF:
-        for (const auto& t : h->potrf) {
-            f += (double)t.kb * t.kb * t.kb / 3.0;
-            b += 2.0 * s * tri(t.kb);
-        }
-        break;
-    case MFD_K_TRSM:
-        for (const auto& t : h->trsm) {
-            double rows = std::min(256, h->nodes[t.node].n - t.row0);
-            f += rows * t.kb * t.kb;
-            b += s * (2.0 * rows * t.kb + tri(t.kb));
+    case MFD_K_POTRF:  // fused panel: one diagonal factorization per step + TRSM of the rows below
+        for (const auto& t : h->panel) {
+            if (t.row0 < 0) {
+                f += (double)t.kb * t.kb * t.kb / 3.0;
+                b += 2.0 * s * tri(t.kb);
+            } else {
+                double rows = std::min(256, h->nodes[t.node].n - t.row0);
+                f += rows * t.kb * t.kb;
+                b += s * (2.0 * rows * t.kb);
+            }
         }
         break;
     case MFD_K_UPDATE:

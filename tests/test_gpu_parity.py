@@ -125,8 +125,26 @@ def test_factor_and_solve_vs_oracle(name, make, max_zero, nb):
     perm = st["perm"] - 1
     xo = np.empty(p.n)
     xo[perm] = f.solve(b[perm, 0].copy())
-    assert np.linalg.norm(x - xo, np.inf) <= 1e-12 * np.linalg.norm(xo, np.inf), name
-    assert P.backward_error(p, x, b[:, 0]) <= 1e-14
+    eta, eta_o = P.backward_error(p, x, b[:, 0]), P.backward_error(p, xo, b[:, 0])
+    assert eta <= 1e-14 and eta_o <= 1e-14
+    # Two backward-stable solutions differ by up to ~ 2 kappa eta; the
+    # bound is 1e-12 unless the condition number makes that unreachable.
+    tol = max(1e-12, 4.0 * condition_number(p) * max(eta, eta_o))
+    rel = np.linalg.norm(x - xo, np.inf) / np.linalg.norm(xo, np.inf)
+    log(f"{name}: x rel diff {rel:.2e} (tol {tol:.2e}), eta {eta:.2e}")
+    assert rel <= tol, (name, rel, tol)
+
+
+def condition_number(p):
+    """2-norm condition number of the SPD matrix (extreme eigenvalues)."""
+    from scipy.sparse.linalg import eigsh
+    A = p.scipy()
+    if p.n <= 2000:
+        ev = np.linalg.eigvalsh(A.toarray())
+        return ev[-1] / ev[0]
+    hi = eigsh(A, k=1, which="LA", return_eigenvectors=False)[0]
+    lo = eigsh(A, k=1, sigma=0, which="LM", return_eigenvectors=False)[0]
+    return hi / lo
 
 
 def test_multiple_rhs_and_device_entry_points():
