@@ -140,11 +140,23 @@ def main():
     ap.add_argument("--max-zero", type=int, default=16384)
     ap.add_argument("--nb", type=int, default=64)
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--cpu-size", type=int, default=24)
+    ap.add_argument("--cpu-size", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--fp64-only", action="store_true", help="skip the FP32+IR leg and the profiled pass")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device twice (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        nbytes = args.copy_probe << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": nbytes, "copies": 2}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU path is not built",
               file=sys.stderr)

@@ -62,8 +62,8 @@ typedef struct mfd_handle mfd_handle;
 #define MFD_K_ASSEMBLE 0
 #define MFD_K_EXTEND_ADD 1
 #define MFD_K_SMALL_FRONT 2
-#define MFD_K_POTRF 3       /* panel step: diagonal-block Cholesky fused with the TRSM below it */
-#define MFD_K_TRSM 4        /* (reserved; the TRSM runs inside MFD_K_POTRF launches) */
+#define MFD_K_POTRF 3       /* diagonal block: its share of the previous update, Cholesky, inverse */
+#define MFD_K_TRSM 4        /* panel TRSM: rows below a diagonal block times its inverse (MFMA) */
 #define MFD_K_UPDATE 5      /* in-panel trailing update (GEMM/SYRK, K = panel width) */
 #define MFD_K_SCHUR 6       /* Schur-complement update (SYRK, K = front's pivot count) */
 #define MFD_K_SOLVE_FWD 7

@@ -10,6 +10,7 @@
 namespace mfd {
 
 template <typename T> void launch_fill(T* p, int64_t n, hipStream_t st);
+template <typename T> void launch_fill_lower(T* pool, const NodeDev* nodes, const int2* tasks, int count, hipStream_t st);
 template <typename T>
 void launch_scatter_a(T* pool, const double* a, const int64_t* dst, const int64_t* src, int64_t nnz, hipStream_t st);
 template <typename T>
@@ -17,9 +18,9 @@ void launch_extend_add(T* pool, const int* lidx, const NodeDev* nodes, const Ext
 template <typename T>
 void launch_small_front(T* pool, const NodeDev* nodes, const int* tasks, int count, int* info, hipStream_t st);
 template <typename T>
-void launch_panel(T* pool, const NodeDev* nodes, const SolveTask* tasks, int count, int64_t scratch, int* info,
-                  hipStream_t st);
+void launch_diag(T* pool, const NodeDev* nodes, const DiagTask* tasks, int count, int* info, hipStream_t st);
 template <typename T> void launch_gemm(T* pool, const GemmTask* tasks, int count, hipStream_t st);
+template <typename T> void launch_gemm128(T* pool, const GemmTask* tasks, int count, hipStream_t st);
 void launch_fwd_gather(double* zpool, const double* b, int ldb, int nrhs, const NodeDev* nodes, const int* lidx,
                        const int* child_ptr, const int* child_list, const int* tasks, int count, hipStream_t st);
 template <typename T>
@@ -28,10 +29,7 @@ void launch_fwd_step(const T* pool, double* zpool, int nrhs, const NodeDev* node
 void launch_bwd_gather(double* zpool, int nrhs, const NodeDev* nodes, const int* lidx, const int* tasks, int count,
                        hipStream_t st);
 template <typename T>
-void launch_bwd_partial(const T* pool, double* zpool, double* part, int nrhs, const NodeDev* nodes,
-                        const SolveTask* tasks, int count, hipStream_t st);
-template <typename T>
-void launch_bwd_diag(const T* pool, double* zpool, const double* part, double* x, int ldx, int nrhs,
+void launch_bwd_step(const T* pool, double* zpool, double* part, unsigned* counters, double* x, int ldx, int nrhs,
                      const NodeDev* nodes, const SolveTask* tasks, int count, hipStream_t st);
 void launch_set_int(int* p, int v, hipStream_t st);
 void launch_residual(double* r, const double* b, const double* x, int n, int nrhs, const int* ptr, const int* col,

@@ -10,9 +10,10 @@
 //   k_scatter_a     set_zero + set_coefficient        zero_setter_m.f90:12-31, coefficient_setter_m.f90:15-50
 //   k_extend_add    extend_add_tri / extend_add_rect  seq_factorize/kernel/extend_add_kernel_m.f90:6-34
 //   k_small_front   the per-front tile loop           seq_factorize_m.f90:31-79
-//   k_panel         DPOTRF("U") + DTRSM("L","U","T")  seq_factorize/kernel/factorize_kernel_m.f90:4-16,
-//                                                     border_kernel_m.f90:9-30
-//   k_gemm          DSYRK / DGEMM updates             factorize_kernel_m.f90:18-33, border_subroutines_m.f90:41-73
+//   k_diag          DPOTRF("U") of a diagonal block   seq_factorize/kernel/factorize_kernel_m.f90:4-9,
+//                                                     border_kernel_m.f90:9-19
+//   k_gemm          DTRSM (mode 1), DSYRK / DGEMM     factorize_kernel_m.f90:11-33, border_kernel_m.f90:21-30,
+//                   updates (mode 0)                  border_subroutines_m.f90:41-73
 //   k_fwd_*         seq_forward (scatter_add, TRSV, GEMV)   seq_solve/seq_forward_m.f90:14-81
 //   k_bwd_*         seq_backward (gather, GEMV, TRSV)       seq_solve/seq_backward_m.f90:14-131
 #include <hip/hip_runtime.h>
@@ -24,7 +25,7 @@ namespace mfd {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kDLd = 2 * kPanel + 1;  // row length of the LDS copy of a diagonal block
+constexpr int kDLd = kPanel + 1;  // row length of the LDS copy D of a diagonal block
 
 // ------------------------------------------------------------ MFMA traits
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -49,102 +50,179 @@ template <> struct Mfma<float> {
 __device__ inline double dsqrt(double x) { return sqrt(x); }
 __device__ inline float dsqrt(float x) { return sqrtf(x); }
 
+// p[i] for a uniform base p and a 32-bit unsigned index: the address is
+// base + zero-extended byte offset, which keeps one VGPR per address when
+// many loads are in flight.
+template <typename T> __device__ __forceinline__ T ld_off(const T* p, unsigned i)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + (size_t)(i * (unsigned)sizeof(T)));
+}
+
+// Value of v in lane src (uniform src; only its low 6 bits select the lane).
+__device__ __forceinline__ double bcast(double v, int src)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, src);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float bcast(float v, int src)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+
 // Orders one wave's LDS accesses (they complete in issue order within a
 // wave) against compiler reordering; no workgroup barrier is involved.
-__device__ inline void wave_sync()
+__device__ __forceinline__ void wave_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Row kernels keep one 64-entry row per lane in registers as a shift
-// register: at step k, x[j] holds the row's entry of column k + j.  The
-// step loop is rolled and the inner loop fully unrolled, so every register
-// index is static (no scratch memory) and each step shifts the row by one.
-// The 64 steps run in four segments of 16 whose inner width shrinks with
-// the segment (64 - K0), so the redundant work past the live columns is at
-// most 15 columns per step.
+// ------------------------------------------- dense 64 x 64 partial Cholesky
+// A front (or diagonal block) of order n <= 64 in LDS, factored in its first
+// c columns by a blocked right-looking algorithm with 16-column panels:
+//   - each panel is factored by one wave, one row per lane, in a 16-step
+//     chain (the entries of the finished column come from lanes 0..15 by
+//     readlane); the same chain inverts the panel's 16 x 16 diagonal block
+//     (identity rows solved against its L^T);
+//   - the rank-16 update of the trailing blocks runs on MFMA across the four
+//     waves;
+//   - the off-diagonal 16 x 16 blocks of L^{-1} follow on MFMA, by distance
+//     from the diagonal: X_ij = -inv(L_ii) sum_{k=j}^{i-1} L_ik X_kj.
+// Layout: A[col][row] holds the lower triangle (zero past n); afterwards it
+// holds L in columns < c and the Schur complement in the trailing block.
+// X[col][row] = L^{-1}[row][col] for row, col < c (other entries are
+// scratch).  Upper-triangle entries of A are scratch too.
+template <typename T> struct Front64 {
+    T A[64][65];
+    T X[64][65];
+    T Ts[4][16][17];
+};
 
-// One wave (threads 0..63) factors the first c columns of an n x n lower
-// block (n <= 64) held one row per lane (x[j] = A[lane][j], zero above the
-// diagonal).  At step k the pivot comes from lane k by a shuffle and the
-// finished column is published through LDS (colbuf: 128 entries, the upper
-// 64 zero) and read back as broadcasts.  Column k of L is stored to g
-// (leading dimension ld).  A pivot that is not positive records the global
-// column col0 + k + 1 in *info (smallest wins).  Afterwards x[j] holds the
-// Schur complement entry of column c + j.
-template <int K0, typename T>
-__device__ __forceinline__ void chol_seg(T (&x)[64], T* colbuf, int n, int c, int* info, int col0, T* g, int64_t ld)
+template <typename T>
+__device__ __forceinline__ void panel16(Front64<T>& S, int b16, int cb, int n, int* info, int col0)
 {
-    constexpr int W = 64 - K0;
-    const int lane = threadIdx.x & 63;
-    const int kend = min(K0 + 16, c);
-#pragma unroll 1
-    for (int k = K0; k < kend; ++k) {
-        const T d = __shfl(x[0], k);
-        if (!(d > T(0)) && lane == 0) atomicMin(info, col0 + k + 1);
-        const T s = dsqrt(d);
-        const T l = lane == k ? s : x[0] / s;
-        colbuf[lane] = l;
-        wave_sync();
-        if (lane >= k && lane < n) g[lane + (int64_t)k * ld] = l;
-        const T* ck = colbuf + k;
+    const int r = threadIdx.x & 63, row = b16 + r;
+    const bool live = row < n;
+    T x[16], y[16];
 #pragma unroll
-        for (int j = 1; j < W; ++j) x[j - 1] = x[j] - l * ck[j];
-        wave_sync();
+    for (int j = 0; j < 16; ++j) {
+        x[j] = (live && (r >= 16 || j <= r)) ? S.A[b16 + j][row] : T(0);
+        y[j] = j == r ? T(1) : T(0);
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        if (k < cb) {
+            const T d = bcast(x[k], k);
+            if (!(d > T(0)) && r == 0) atomicMin(info, col0 + b16 + k + 1);
+            const T sd = dsqrt(d);
+            const T l = r == k ? sd : x[k] / sd;
+            x[k] = l;
+            const T y0 = y[k] / sd;
+            y[k] = y0;
+#pragma unroll
+            for (int j = k + 1; j < 16; ++j) {
+                const T lj = bcast(l, j);
+                x[j] -= l * lj;
+                y[j] -= y0 * lj;
+            }
+        }
+    }
+    if (live) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) S.A[b16 + j][row] = x[j];
+    }
+    if (r < 16) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) S.X[b16 + r][b16 + k] = y[k];
     }
 }
-template <typename T>
-__device__ __forceinline__ void chol_rows(T (&x)[64], T* colbuf, int n, int c, int* info, int col0, T* g, int64_t ld)
-{
-    chol_seg<0>(x, colbuf, n, c, info, col0, g, ld);
-    chol_seg<16>(x, colbuf, n, c, info, col0, g, ld);
-    chol_seg<32>(x, colbuf, n, c, info, col0, g, ld);
-    chol_seg<48>(x, colbuf, n, c, info, col0, g, ld);
-}
 
-// x (one row, kb columns) <- x L^{-T} by forward substitution, L given as
-// D[k][m] = L[m][k] (D rows are kDLd >= 2 * 64 long and zero past column
-// 63, which the shifted reads D[k][k + j] reach).  Column k of the result
-// goes to out[k * stride].
-template <int K0, typename T>
-__device__ __forceinline__ void row_solve_seg(T (&x)[64], const T (*D)[kDLd], int kb, T* out, int64_t stride)
+// A_IJ -= L_Ib L_Jb^T over the cb pivot columns of panel b (one wave).
+template <typename T>
+__device__ __forceinline__ void trailing16(Front64<T>& S, int b16, int cb, int I, int J)
 {
-    constexpr int W = 64 - K0;
-    const int kend = min(K0 + 16, kb);
-#pragma unroll 1
-    for (int k = K0; k < kend; ++k) {
-        const T* dk = D[k] + k;
-        const T x0 = x[0] / dk[0];
-        out[(int64_t)k * stride] = x0;
+    typedef typename Mfma<T>::acc_t acc_t;
+    const int lane = threadIdx.x & 63, li = lane & 15;
+    acc_t acc = {0, 0, 0, 0};
 #pragma unroll
-        for (int j = 1; j < W; ++j) x[j - 1] = x[j] - x0 * dk[j];
+    for (int q = 0; q < 4; ++q) {
+        const int kk = 4 * q + (lane >> 4);
+        const bool ok = kk < cb;
+        const T a = ok ? S.A[b16 + kk][16 * I + li] : T(0);
+        const T b = ok ? S.A[b16 + kk][16 * J + li] : T(0);
+        acc = Mfma<T>::mma(a, b, acc);
     }
-}
-template <typename T>
-__device__ __forceinline__ void row_solve(T (&x)[64], const T (*D)[kDLd], int kb, T* out, int64_t stride)
-{
-    row_solve_seg<0>(x, D, kb, out, stride);
-    row_solve_seg<16>(x, D, kb, out, stride);
-    row_solve_seg<32>(x, D, kb, out, stride);
-    row_solve_seg<48>(x, D, kb, out, stride);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) S.A[16 * J + li][16 * I + Mfma<T>::row(lane, v)] -= acc[v];
 }
 
-// The inverse of the kb x kb lower block held in D, stored column-major
-// (ld 64, zero-padded to 64 x 64) at inv: row i of the identity solved
-// against L^T gives row i of L^{-T}, i.e. column i of L^{-1}.  Threads
-// 0..63 take one identity row each.
+// X_ij = -inv(L_ii) sum_{k=j}^{i-1} L_ik X_kj (one wave; staging in Ts[w]).
 template <typename T>
-__device__ __forceinline__ void store_inverse(const T (*D)[kDLd], int kb, T* inv)
+__device__ __forceinline__ void inverse16(Front64<T>& S, int i, int j, int w)
 {
-    const int i = threadIdx.x;
-    if (i >= 64) return;
-    T x[64];
+    typedef typename Mfma<T>::acc_t acc_t;
+    const int lane = threadIdx.x & 63, li = lane & 15;
+    acc_t acc = {0, 0, 0, 0};
+    for (int k = j; k < i; ++k) {
 #pragma unroll
-    for (int j = 0; j < 64; ++j) x[j] = j == i ? T(1) : T(0);
-    row_solve<T>(x, D, kb, inv + 64 * i, 1);
-    for (int k = kb; k < 64; ++k) inv[64 * i + k] = T(0);
+        for (int q = 0; q < 4; ++q) {
+            const int kk = 4 * q + (lane >> 4);
+            acc = Mfma<T>::mma(S.A[16 * k + kk][16 * i + li], S.X[16 * j + li][16 * k + kk], acc);
+        }
+    }
+    wave_sync();
+#pragma unroll
+    for (int v = 0; v < 4; ++v) S.Ts[w][Mfma<T>::row(lane, v)][li] = acc[v];
+    wave_sync();
+    acc_t acc2 = {0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int m = 4 * q + (lane >> 4);
+        acc2 = Mfma<T>::mma(S.X[16 * i + m][16 * i + li], S.Ts[w][m][li], acc2);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) S.X[16 * j + li][16 * i + Mfma<T>::row(lane, v)] = -acc2[v];
+}
+
+// All 256 threads: loads the lower n x n block at g (leading dimension ld),
+// factors its first c columns, writes L and the Schur complement back and
+// stores L^{-T} of the c x c factor column-major at inv (64 x 64,
+// zero-padded: inv[r + 64 q] = L^{-1}[q][r]).  A pivot that is not positive
+// records the global column col0 + k + 1 in *info (smallest wins).
+template <typename T>
+__device__ __forceinline__ void factor_dense64(Front64<T>& S, T* g, int64_t ld, int n, int c, int* info, int col0,
+                                               T* inv)
+{
+    const int tid = threadIdx.x, wv = tid >> 6;
+    for (int e = tid; e < 64 * 64; e += kThreads) {
+        const int row = e & 63, col = e >> 6;
+        S.A[col][row] = (row < n && col <= row) ? g[row + (int64_t)col * ld] : T(0);
+        S.X[col][row] = T(0);
+    }
+    __syncthreads();
+    const int nblk = (c + 15) >> 4, nall = (n + 15) >> 4;
+    for (int b = 0; b < nblk; ++b) {
+        const int b16 = 16 * b, cb = min(16, c - b16);
+        if (wv == 0) panel16<T>(S, b16, cb, n, info, col0);
+        __syncthreads();
+        int idx = 0;
+        for (int J = b + 1; J < nall; ++J)
+            for (int I = J; I < nall; ++I, ++idx)
+                if ((idx & 3) == wv) trailing16<T>(S, b16, cb, I, J);
+        __syncthreads();
+    }
+    for (int d = 1; d < nblk; ++d) {
+        for (int j = wv; j + d < nblk; j += 4) inverse16<T>(S, j + d, j, wv);
+        __syncthreads();
+    }
+    for (int e = tid; e < 64 * 64; e += kThreads) {
+        const int row = e & 63, col = e >> 6;
+        if (row < n && col <= row) g[row + (int64_t)col * ld] = S.A[col][row];
+        inv[e] = (row < c && col < c) ? S.X[row][col] : T(0);
+    }
 }
 
 }  // namespace
@@ -156,6 +234,19 @@ __global__ __launch_bounds__(kThreads) void k_fill(T* p, int64_t n)
     int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * kThreads;
     for (; i < n; i += stride) p[i] = T(0);
+}
+
+// Zeroes the lower part (row >= col) of 64 columns of one front: task
+// (node, first column).  Only the lower triangle of a front is ever read.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_fill_lower(T* pool, const NodeDev* nodes, const int2* tasks)
+{
+    const int2 t = tasks[blockIdx.x];
+    const NodeDev nd = nodes[t.x];
+    const int n = nd.n, jend = min(t.y + 64, n);
+    T* g = pool + nd.front;
+    for (int j = t.y; j < jend; ++j)
+        for (int i = j + threadIdx.x; i < n; i += kThreads) g[i + (int64_t)j * n] = T(0);
 }
 
 // pool[dst[k]] = a[src[k]]: each stored A entry lands on exactly one front
@@ -203,132 +294,75 @@ __global__ __launch_bounds__(kThreads) void k_extend_add(T* pool, const int* lid
 }
 
 // -------------------------------------------------------- small fronts
-// Whole partial factorization of a front with n <= 64 by one wave: L in
+// Whole partial factorization of a front with n <= 64 in one workgroup: L in
 // the first c columns, the Schur complement in the trailing block, and the
 // inverse of the c x c diagonal block for the solves.
 template <typename T>
-__global__ __launch_bounds__(64) void k_small_front(T* pool, const NodeDev* nodes, const int* tasks, int* info)
+__global__ __launch_bounds__(kThreads) void k_small_front(T* pool, const NodeDev* nodes, const int* tasks, int* info)
 {
     const NodeDev nd = nodes[tasks[blockIdx.x]];
-    __shared__ T D[kPanel][kDLd];
-    __shared__ T colbuf[128];
-    const int n = nd.n, c = nd.c, lane = threadIdx.x;
-    T* g = pool + nd.front;
-    for (int e = lane; e < kPanel * kDLd; e += 64) (&D[0][0])[e] = T(0);
-    colbuf[lane] = T(0);
-    colbuf[lane + 64] = T(0);
-    T x[64];
-#pragma unroll
-    for (int j = 0; j < 64; ++j) x[j] = (lane < n && j <= lane) ? g[lane + (int64_t)j * n] : T(0);
-    __syncthreads();
-    chol_rows<T>(x, colbuf, n, c, info, nd.first, &D[0][0], kDLd);
-    for (int k = 0; k < c; ++k)
-        if (lane >= k && lane < n) g[lane + (int64_t)k * n] = D[k][lane];
-#pragma unroll
-    for (int j = 0; j < 64; ++j)
-        if (lane < n && lane >= c + j) g[lane + (int64_t)(c + j) * n] = x[j];
-    store_inverse<T>(D, c, pool + nd.inv);
+    __shared__ Front64<T> S;
+    factor_dense64<T>(S, pool + nd.front, nd.n, nd.n, nd.c, info, nd.first, pool + nd.inv);
 }
 
 // ------------------------------------------------------ large fronts
-// One panel step [k0, k0+kb) of a large front.  Wave 0 of every workgroup
-// factors the kb x kb diagonal block itself (it is small, and this removes a
-// launch and a dependency from the step).  A chunk task (row0 >= 0) then solves its 256
-// rows X L^T = P, one thread per row.  The writer task (row0 < 0) stores the
-// factored block into scratch slot `slot` -- the front's own copy is
-// rewritten by the update launch that follows, once no workgroup of this
-// launch can still be reading the unfactored block -- and the inverse of
-// the block for the solves.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_panel(T* pool, const NodeDev* nodes, const SolveTask* tasks,
-                                                    int64_t scratch, int* info)
-{
-    const SolveTask t = tasks[blockIdx.x];
-    const NodeDev nd = nodes[t.node];
-    __shared__ T D[kPanel][kDLd];
-    __shared__ T colbuf[128];
-    const int n = nd.n, kb = t.kb, tid = threadIdx.x, lane = tid & 63;
-    for (int e = tid; e < kPanel * kDLd; e += kThreads) (&D[0][0])[e] = T(0);
-    if (tid < 128) colbuf[tid] = T(0);
-    __syncthreads();
-    if (tid < 64) {
-        const T* gd = pool + nd.front + t.k0 + (int64_t)t.k0 * n;
-        T xd[64];
-#pragma unroll
-        for (int j = 0; j < 64; ++j) xd[j] = (lane < kb && j <= lane) ? gd[lane + (int64_t)j * n] : T(0);
-        chol_rows<T>(xd, colbuf, kb, kb, info, nd.first + t.k0, &D[0][0], kDLd);
-    }
-    __syncthreads();
-    if (t.row0 >= 0) {
-        const int row = t.row0 + tid;
-        if (row >= n) return;
-        T* gp = pool + nd.front + row + (int64_t)t.k0 * n;
-        T x[64];
-#pragma unroll
-        for (int j = 0; j < 64; ++j) x[j] = j < kb ? gp[(int64_t)j * n] : T(0);
-        row_solve<T>(x, D, kb, gp, n);
-    } else {
-        T* sc = pool + scratch + (int64_t)t.slot * kPanel * kPanel;
-        for (int e = tid; e < kPanel * kPanel; e += kThreads) {
-            int i = e & 63, k = e >> 6;
-            sc[e] = (k < kb && i < kb && i >= k) ? D[k][i] : T(0);
-        }
-        store_inverse<T>(D, kb, pool + nd.inv + (int64_t)(t.k0 / kPanel) * kPanel * kPanel);
-    }
-}
+// LDS staging of one 64 x 64 MFMA tile product.
+template <typename T> struct TileLds {
+    T As[16][kTile + 4];
+    T Bs[16][kTile + 4];
+};
 
-// C -= A B^T on one tile (see GemmTask), FP64/FP32 MFMA 16x16x4.  Four waves
-// each own a 32 x 32 quarter of the 64 x 64 tile (2 x 2 MFMA blocks).  K is
-// staged through LDS in chunks of 16 with the next chunk's global loads
-// issued before the current chunk's MFMAs.  The result goes through LDS so
-// the read-modify-write of C is coalesced along columns.  k == 0 copies the
-// lower part of A into C instead (the factored diagonal block of a panel).
+// Cs[col][row] = sum_kk A[row][kk] B[col][kk] over a 64 x 64 tile (rows
+// past m and columns past n give 0), FP64/FP32 MFMA 16x16x4.  Four waves
+// each own a 32 x 32 quarter (2 x 2 MFMA blocks).  K is staged through LDS in
+// chunks of 16 with the next chunk's global loads issued before the current
+// chunk's MFMAs.  Every read of A and B happens before the function
+// returns, so the caller may overwrite A afterwards.  Ends with a barrier.
+//
+// tb != 0 takes B transposed in memory (B[col][kk] at B + col * ldb + kk);
+// its chunks are then read 16 entries at a time along kk.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_gemm(T* pool, const GemmTask* tasks)
+__device__ __forceinline__ void tile_product(const T* A, int lda, const T* B, int ldb, bool tb, int m, int n, int k,
+                                             TileLds<T>& L, T (*Cs)[kTile + 1])
 {
     typedef typename Mfma<T>::acc_t acc_t;
-    const GemmTask t = tasks[blockIdx.x];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    if (t.k == 0) {
-        const T* A = pool + t.a;
-        T* C = pool + t.c;
-        for (int e = tid; e < t.m * t.n; e += kThreads) {
-            int i = e % t.m, j = e / t.m;
-            if (i >= j) C[i + (int64_t)j * t.ldc] = A[i + (int64_t)j * t.lda];
-        }
-        return;
-    }
-    __shared__ T As[16][kTile + 4];
-    __shared__ T Bs[16][kTile + 4];
-    __shared__ T Cs[kTile][kTile + 1];
     const int wr = (wv & 1) * 32, wc = (wv >> 1) * 32;
     const int li = tid & 63, lp = tid >> 6;
-    const T* A = pool + t.a;
-    const T* B = pool + t.b;
     acc_t c00 = {0, 0, 0, 0}, c01 = {0, 0, 0, 0}, c10 = {0, 0, 0, 0}, c11 = {0, 0, 0, 0};
     T ra[4], rb[4];
+    // transposed-B chunk: element e = tid + 256 p is (col e / 16, kk e % 16)
+    const int tcol = tid >> 4, tkk = tid & 15;
     auto load = [&](int k0) {
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
             int kk = k0 + lp + 4 * p;
-            ra[p] = (li < t.m && kk < t.k) ? A[li + (int64_t)kk * t.lda] : T(0);
-            rb[p] = (li < t.n && kk < t.k) ? B[li + (int64_t)kk * t.ldb] : T(0);
+            ra[p] = (li < m && kk < k) ? A[li + (int64_t)kk * lda] : T(0);
+            if (tb) {
+                int col = tcol + 16 * p, kt = k0 + tkk;
+                rb[p] = (col < n && kt < k) ? B[(int64_t)col * ldb + kt] : T(0);
+            } else {
+                rb[p] = (li < n && kk < k) ? B[li + (int64_t)kk * ldb] : T(0);
+            }
         }
     };
     load(0);
-    for (int k0 = 0; k0 < t.k; k0 += 16) {
+    for (int k0 = 0; k0 < k; k0 += 16) {
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
-            As[lp + 4 * p][li] = ra[p];
-            Bs[lp + 4 * p][li] = rb[p];
+            L.As[lp + 4 * p][li] = ra[p];
+            if (tb)
+                L.Bs[tkk][tcol + 16 * p] = rb[p];
+            else
+                L.Bs[lp + 4 * p][li] = rb[p];
         }
         __syncthreads();
-        if (k0 + 16 < t.k) load(k0 + 16);
+        if (k0 + 16 < k) load(k0 + 16);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int kr = q * 4 + (lane >> 4);
-            const T a0 = As[kr][wr + (lane & 15)], a1 = As[kr][wr + 16 + (lane & 15)];
-            const T b0 = Bs[kr][wc + (lane & 15)], b1 = Bs[kr][wc + 16 + (lane & 15)];
+            const T a0 = L.As[kr][wr + (lane & 15)], a1 = L.As[kr][wr + 16 + (lane & 15)];
+            const T b0 = L.Bs[kr][wc + (lane & 15)], b1 = L.Bs[kr][wc + 16 + (lane & 15)];
             c00 = Mfma<T>::mma(a0, b0, c00);
             c01 = Mfma<T>::mma(a0, b1, c01);
             c10 = Mfma<T>::mma(a1, b0, c10);
@@ -346,11 +380,134 @@ __global__ __launch_bounds__(kThreads) void k_gemm(T* pool, const GemmTask* task
         Cs[q0 + 16][r + 16] = c11[v];
     }
     __syncthreads();
+}
+
+// One GemmTask per workgroup (see mfd_plan.h): C -= A B^T (mode 0: the
+// DSYRK/DGEMM updates) or C = A B^T with B transposed in memory (mode 1: the
+// panel TRSM as a product with the stored L^{-T} of the diagonal block, C
+// and A the same rows).
+// The result goes through LDS so the write of C is coalesced along columns.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_gemm(T* pool, const GemmTask* tasks)
+{
+    const GemmTask t = tasks[blockIdx.x];
+    __shared__ TileLds<T> L;
+    __shared__ T Cs[kTile][kTile + 1];
+    tile_product<T>(pool + t.a, t.lda, pool + t.b, t.ldb, t.mode == 1, t.m, t.n, t.k, L, Cs);
     T* C = pool + t.c;
-    const int row = tid & 63;
-    if (row < t.m)
+    const int tid = threadIdx.x, row = tid & 63;
+    if (row >= t.m) return;
+    if (t.mode == 0) {
         for (int col = tid >> 6; col < t.n; col += 4)
             if (!t.tri || row >= col) C[row + (int64_t)col * t.ldc] -= Cs[col][row];
+    } else {
+        for (int col = tid >> 6; col < t.n; col += 4) C[row + (int64_t)col * t.ldc] = Cs[col][row];
+    }
+}
+
+// C -= A B^T on a 128 x 128 tile (GemmTask with m, n <= 128, mode 0): the
+// DSYRK/DGEMM updates of large fronts.  Each of the four waves owns a
+// 64 x 64 quarter as 4 x 4 MFMA 16x16x4 blocks, so every LDS operand read
+// feeds four MFMAs.  K is staged through LDS in chunks of 16, the next
+// chunk's global loads in flight during the current chunk's MFMAs.  A
+// diagonal tile (tri) skips the quarter above the diagonal.  The result is
+// written back one quarter at a time through LDS, coalesced along columns.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_gemm128(T* pool, const GemmTask* tasks)
+{
+    typedef typename Mfma<T>::acc_t acc_t;
+    constexpr int TB = 128, LD = TB + 4;
+    const GemmTask t = tasks[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, li = lane & 15;
+    __shared__ union {
+        struct {
+            T As[16][LD];
+            T Bs[16][LD];
+        } k;
+        T Cs[64][65];
+    } u;
+    const int wr = (wv & 1) * 64, wc = (wv >> 1) * 64;
+    const bool skip = t.tri && wc > wr;  // quarter strictly above the diagonal
+    const T* A = pool + t.a;
+    const T* B = pool + t.b;
+    acc_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+    const int lr = tid & 127, lk = tid >> 7;
+    T ra[8], rb[8];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int p = 0; p < 8; ++p) {
+            const int kk = k0 + lk + 2 * p;
+            ra[p] = (lr < t.m && kk < t.k) ? A[lr + (int64_t)kk * t.lda] : T(0);
+            rb[p] = (lr < t.n && kk < t.k) ? B[lr + (int64_t)kk * t.ldb] : T(0);
+        }
+    };
+    load(0);
+    for (int k0 = 0; k0 < t.k; k0 += 16) {
+#pragma unroll
+        for (int p = 0; p < 8; ++p) {
+            u.k.As[lk + 2 * p][lr] = ra[p];
+            u.k.Bs[lk + 2 * p][lr] = rb[p];
+        }
+        __syncthreads();
+        if (k0 + 16 < t.k) load(k0 + 16);
+        if (!skip) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int kr = 4 * q + (lane >> 4);
+                T a[4], b[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    a[i] = u.k.As[kr][wr + 16 * i + li];
+                    b[i] = u.k.Bs[kr][wc + 16 * i + li];
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::mma(a[i], b[j], acc[i][j]);
+            }
+        }
+        __syncthreads();
+    }
+    T* C = pool + t.c;
+    for (int w = 0; w < 4; ++w) {
+        const int qr = (w & 1) * 64, qc = (w >> 1) * 64;
+        if (t.tri && qc > qr) continue;
+        if (qr >= t.m || qc >= t.n) continue;
+        if (wv == w) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) u.Cs[16 * j + li][16 * i + Mfma<T>::row(lane, v)] = acc[i][j][v];
+        }
+        __syncthreads();
+        const int row = tid & 63, gr = qr + row;
+        if (gr < t.m)
+            for (int col = tid >> 6; col < 64; col += 4) {
+                const int gc = qc + col;
+                if (gc < t.n && (!t.tri || gr >= gc)) C[gr + (int64_t)gc * t.ldc] -= u.Cs[col][row];
+            }
+        __syncthreads();
+    }
+}
+
+// Diagonal block [k0, k0+kb) of a large front, already updated by every
+// earlier panel (DPOTRF("U"), factorize_kernel_m.f90:4-9): L written back
+// into the front, and the inverse of L stored for the panel TRSM and the
+// solves.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_diag(T* pool, const NodeDev* nodes, const DiagTask* tasks, int* info)
+{
+    const DiagTask t = tasks[blockIdx.x];
+    const NodeDev nd = nodes[t.node];
+    __shared__ Front64<T> S;
+    factor_dense64<T>(S, pool + nd.front + t.k0 + (int64_t)t.k0 * nd.n, nd.n, t.kb, t.kb, info, nd.first + t.k0,
+                      pool + nd.inv + (int64_t)(t.k0 / kPanel) * kPanel * kPanel);
 }
 
 // ------------------------------------------------------------------ solve
@@ -390,18 +547,21 @@ __global__ __launch_bounds__(kThreads) void k_fwd_gather(double* zpool, const do
     }
 }
 
-// Stage inv(L_tt) (64 x 64, column-major in the pool) in LDS as LI[col][row].
+// Stage inv(L_tt) in LDS as LI[col][row] = L^{-1}[row][col] (the pool holds
+// L^{-T} column-major, i.e. inv[r + 64 c] = L^{-1}[c][r]).
 template <typename T>
 __device__ void load_inverse(const T* inv, double (*LI)[65])
 {
 #pragma unroll
-    for (int e = threadIdx.x; e < 64 * 64; e += kThreads) LI[e >> 6][e & 63] = (double)inv[e];
+    for (int e = threadIdx.x; e < 64 * 64; e += kThreads) LI[e & 63][e >> 6] = (double)inv[e];
 }
 
 // Forward block step: y_t = inv(L_tt) Z[k0:k0+kb] (every workgroup computes
 // it from the unmodified Z block); the writer task (row0 < 0) stores y_t in
 // Y, each chunk task subtracts L[rows, t] y_t from its 256 rows of Z
-// (DTRSV "U","T" and DGEMV "T", solve_kernel_m.f90:4-23).
+// (DTRSV "U","T" and DGEMV "T", solve_kernel_m.f90:4-23).  Every global
+// read of the step (the inverse, the chunk's 64 columns of L, Z) is issued
+// before the first wait.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_fwd_step(const T* pool, double* zpool, int nrhs, const NodeDev* nodes,
                                                        const SolveTask* tasks)
@@ -413,11 +573,26 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(const T* pool, double* zp
     const int n = nd.n, c = nd.c, kb = t.kb, tid = threadIdx.x, lane = tid & 63, ty = tid >> 6;
     double* Z = zpool + nd.z * nrhs;
     double* Y = Z + (int64_t)n * nrhs;
-    load_inverse<T>(pool + nd.inv + (int64_t)(t.k0 / 64) * 4096, LI);
+    const T* inv = pool + nd.inv + (int64_t)(t.k0 / 64) * 4096;
     const int row = t.row0 + tid;
-    const T* L = pool + nd.front + row + (int64_t)t.k0 * n;
+    const bool chunk = t.row0 >= 0 && row < n;
+    const T* Lb = pool + nd.front + (int64_t)t.k0 * n;
+    double li[16];
+#pragma unroll
+    for (int p = 0; p < 16; ++p) li[p] = (double)inv[tid + kThreads * p];
+    double l[64];
+#pragma unroll
+    for (int k = 0; k < 64; ++k) l[k] = (chunk && k < kb) ? (double)ld_off(Lb, (unsigned)(row + k * n)) : 0.0;
+#pragma unroll
+    for (int p = 0; p < 16; ++p) {
+        const int e = tid + kThreads * p;
+        LI[e & 63][e >> 6] = li[p];
+    }
     for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
         const int nr = min(kRhsGroup, nrhs - r0);
+        double zr[kRhsGroup];
+#pragma unroll
+        for (int r = 0; r < kRhsGroup; ++r) zr[r] = (chunk && r < nr) ? Z[row + (int64_t)(r0 + r) * n] : 0.0;
         for (int e = tid; e < 64 * kRhsGroup; e += kThreads) {
             int k = e & 63, r = e >> 6;
             zt[k][r] = (k < kb && r < nr) ? Z[t.k0 + k + (int64_t)(r0 + r) * n] : 0.0;
@@ -434,27 +609,21 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(const T* pool, double* zp
                 int i = e % kb, r = e / kb;
                 Y[t.k0 + i + (int64_t)(r0 + r) * c] = yt[i][r];
             }
-        } else if (row < n) {
-            double acc[kRhsGroup];
-#pragma unroll
-            for (int r = 0; r < kRhsGroup; ++r) acc[r] = 0.0;
-            // 16 column loads in flight per group (zero past kb; the clamped
-            // yt index stays in the array and meets a zero factor)
+        } else if (chunk) {
 #pragma unroll 1
-            for (int k0 = 0; k0 < kb; k0 += 16) {
-                double l[16];
+            for (int r = 0; r < nr; ++r) {
+                double acc0 = 0.0, acc1 = 0.0;
 #pragma unroll
-                for (int k = 0; k < 16; ++k) l[k] = k0 + k < kb ? (double)L[(int64_t)(k0 + k) * n] : 0.0;
-#pragma unroll
-                for (int k = 0; k < 16; ++k) {
-                    const int kk = min(k0 + k, 63);
-#pragma unroll
-                    for (int r = 0; r < kRhsGroup; ++r) acc[r] += l[k] * yt[kk][r];
+                for (int k = 0; k < 64; k += 2) {
+                    acc0 += l[k] * yt[k][r];
+                    acc1 += l[k + 1] * yt[k + 1][r];
                 }
-            }
+                double z = 0.0;
 #pragma unroll
-            for (int r = 0; r < kRhsGroup; ++r)
-                if (r < nr) Z[row + (int64_t)(r0 + r) * n] -= acc[r];
+                for (int q = 0; q < kRhsGroup; ++q)
+                    if (q == r) z = zr[q];
+                Z[row + (int64_t)(r0 + r) * n] = z - (acc0 + acc1);
+            }
         }
         __syncthreads();
     }
@@ -480,68 +649,82 @@ __global__ __launch_bounds__(kThreads) void k_bwd_gather(double* zpool, int nrhs
     }
 }
 
-// Backward: partial dot products P[slot][k][r] = sum over this chunk's rows
-// of L[row][k0+k] x[row][r], x from Y (pivot rows) or Z (update rows)
-// (DGEMV "N", solve_kernel_m.f90:25-30).  64-row sub-blocks of L are staged
-// transposed through LDS so the global reads and the per-column dot
-// products are both conflict-free.
+// Backward block step for one front (DGEMV "N" + DTRSV "U","N",
+// solve_kernel_m.f90:11-16,25-30).  A chunk task (row0 >= 0) forms the
+// partial products P[slot][k][r] = sum over its 256 rows of L[row][k0+k]
+// x[row][r] (x from Y on pivot rows, from Z on update rows) on MFMA: the
+// chunk's L columns are read once into registers, then staged 64 rows at a
+// time through LDS.  It then counts itself done on the group's counter; the
+// last chunk of the group (or the single task of a front with no rows below
+// the block, row0 < 0) forms z = Y[k0:k0+kb] - sum of the partials in chunk
+// order and x_t = inv(L_tt)^T z, written to Y and, for the front's first
+// block, to x with the rest of the front's pivot part.  The partial sums
+// are added in a fixed order, so the result does not depend on which chunk
+// finishes last; the last chunk resets the counter for the next step.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_bwd_partial(const T* pool, double* zpool, double* part, int nrhs,
-                                                          const NodeDev* nodes, const SolveTask* tasks)
+__global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zpool, double* part, unsigned* counters,
+                                                       double* x, int ldx, int nrhs, const NodeDev* nodes,
+                                                       const SolveTask* tasks)
 {
     const SolveTask t = tasks[blockIdx.x];
     const NodeDev nd = nodes[t.node];
-    __shared__ double Ls[64][65];  // Ls[k][i] = L[row_i][k0+k]
-    __shared__ double Xs[64][kRhsGroup];
-    const int n = nd.n, c = nd.c, kb = t.kb, tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    const double* Z = zpool + nd.z * nrhs;
-    const double* Y = Z + (int64_t)n * nrhs;
-    const T* L = pool + nd.front + (int64_t)t.k0 * n;
-    const int rend = min(n, t.row0 + 256);
-    double* P = part + (int64_t)t.slot * 64 * nrhs;
-    for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
-        const int nr = min(kRhsGroup, nrhs - r0);
-        double acc0 = 0.0, acc1 = 0.0;  // (k = tx, r = ty) and (k = tx, r = ty + 4)
-        for (int rb = t.row0; rb < rend; rb += 64) {
-            const int rows = min(64, rend - rb);
-            __syncthreads();
-#pragma unroll
-            for (int q = ty; q < 64; q += 4)
-                if (q < kb) Ls[q][tx] = tx < rows ? (double)L[rb + tx + (int64_t)q * n] : 0.0;
-            for (int e = threadIdx.x; e < 64 * nr; e += kThreads) {
-                int i = e & 63, r = e >> 6, g = rb + i;
-                Xs[i][r] = i >= rows ? 0.0 : g < c ? Y[g + (int64_t)(r0 + r) * c] : Z[g + (int64_t)(r0 + r) * n];
-            }
-            __syncthreads();
-            if (tx < kb) {
-                if (ty < nr)
-                    for (int i = 0; i < rows; ++i) acc0 += Ls[tx][i] * Xs[i][ty];
-                if (ty + 4 < nr)
-                    for (int i = 0; i < rows; ++i) acc1 += Ls[tx][i] * Xs[i][ty + 4];
-            }
-        }
-        if (tx < kb) {
-            if (ty < nr) P[tx + (int64_t)(r0 + ty) * 64] = acc0;
-            if (ty + 4 < nr) P[tx + (int64_t)(r0 + ty + 4) * 64] = acc1;
-        }
-    }
-}
-
-// Backward block step: z = Y[k0:k0+kb] - sum of the chunk partials (in
-// chunk order), x_t = inv(L_tt)^T z into Y (DTRSV "U","N",
-// solve_kernel_m.f90:11-16).  The front's first block writes its pivot part
-// of x out.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_bwd_diag(const T* pool, double* zpool, const double* part, double* x,
-                                                       int ldx, int nrhs, const NodeDev* nodes, const SolveTask* tasks)
-{
-    const SolveTask t = tasks[blockIdx.x];
-    const NodeDev nd = nodes[t.node];
+    const int n = nd.n, c = nd.c, kb = t.kb, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    __shared__ double Ls[64][65];  // Ls[k][i]: column k0 + k of L, row i of the current 64-row block
+    __shared__ double Xs[256][kRhsGroup + 1];
     __shared__ double LI[64][65];
     __shared__ double zt[64][kRhsGroup];
-    const int n = nd.n, c = nd.c, kb = t.kb, tid = threadIdx.x, lane = tid & 63, ty = tid >> 6;
-    double* Y = zpool + nd.z * nrhs + (int64_t)n * nrhs;
+    __shared__ int is_last;
+    double* Z = zpool + nd.z * nrhs;
+    double* Y = Z + (int64_t)n * nrhs;
+    const int group = t.group;
+    if (t.row0 >= 0) {
+        typedef typename Mfma<double>::acc_t acc_t;
+        const int row = t.row0 + tid;
+        const bool live = row < n;
+        const T* Lb = pool + nd.front + (int64_t)t.k0 * n;
+        double l[64];
+#pragma unroll
+        for (int k = 0; k < 64; ++k) l[k] = (live && k < kb) ? (double)ld_off(Lb, (unsigned)(row + k * n)) : 0.0;
+        for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
+            const int nr = min(kRhsGroup, nrhs - r0);
+#pragma unroll
+            for (int r = 0; r < kRhsGroup; ++r)
+                Xs[tid][r] = (live && r < nr) ? (row < c ? Y[row + (int64_t)(r0 + r) * c] : Z[row + (int64_t)(r0 + r) * n])
+                                              : 0.0;
+            acc_t acc = {0, 0, 0, 0};
+            const int rc = lane & 15;
+            for (int sb = 0; sb < 4; ++sb) {
+                __syncthreads();
+                if (wv == sb) {
+#pragma unroll
+                    for (int k = 0; k < 64; ++k) Ls[k][lane] = l[k];
+                }
+                __syncthreads();
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    const int rr = 4 * q + (lane >> 4);
+                    const double a = Ls[16 * wv + rc][rr];
+                    const double b = rc < nr ? Xs[64 * sb + rr][rc] : 0.0;
+                    acc = Mfma<double>::mma(a, b, acc);
+                }
+            }
+            if (rc < nr) {
+                double* P = part + (int64_t)t.slot * 64 * nrhs + (int64_t)(r0 + rc) * 64;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) P[16 * wv + Mfma<double>::row(lane, v)] = acc[v];
+            }
+            __syncthreads();
+        }
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) is_last = atomicAdd(&counters[group], 1u) == (unsigned)(t.nchunk - 1);
+        __syncthreads();
+        if (!is_last) return;
+        __threadfence();
+        if (tid == 0) counters[group] = 0u;
+    }
     load_inverse<T>(pool + nd.inv + (int64_t)(t.k0 / 64) * 4096, LI);
+    const int nchunk = t.row0 >= 0 ? t.nchunk : 0;
     for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
         const int nr = min(kRhsGroup, nrhs - r0);
         for (int e = tid; e < 64 * kRhsGroup; e += kThreads) {
@@ -549,12 +732,12 @@ __global__ __launch_bounds__(kThreads) void k_bwd_diag(const T* pool, double* zp
             double z = 0.0;
             if (k < kb && r < nr) {
                 z = Y[t.k0 + k + (int64_t)(r0 + r) * c];
-                for (int ch = 0; ch < t.nchunk; ++ch) z -= part[(int64_t)(t.slot + ch) * 64 * nrhs + k + (int64_t)(r0 + r) * 64];
+                for (int ch = 0; ch < nchunk; ++ch) z -= part[(int64_t)(group + ch) * 64 * nrhs + k + (int64_t)(r0 + r) * 64];
             }
             zt[k][r] = z;
         }
         __syncthreads();
-        for (int r = ty; r < nr; r += 4) {
+        for (int r = wv; r < nr; r += 4) {
             double acc = 0.0;
             for (int m = 0; m < kb; ++m) acc += LI[lane][m] * zt[m][r];
             if (lane < kb) {
@@ -685,6 +868,11 @@ template <typename T> void launch_fill(T* p, int64_t n, hipStream_t st)
     if (n > 0) hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(n)), dim3(kThreads), 0, st, p, n);
 }
 template <typename T>
+void launch_fill_lower(T* pool, const NodeDev* nodes, const int2* tasks, int count, hipStream_t st)
+{
+    if (count > 0) hipLaunchKernelGGL(k_fill_lower<T>, dim3(count), dim3(kThreads), 0, st, pool, nodes, tasks);
+}
+template <typename T>
 void launch_scatter_a(T* pool, const double* a, const int64_t* dst, const int64_t* src, int64_t nnz, hipStream_t st)
 {
     if (nnz > 0) hipLaunchKernelGGL(k_scatter_a<T>, dim3(grid_for(nnz)), dim3(kThreads), 0, st, pool, a, dst, src, nnz);
@@ -697,17 +885,20 @@ void launch_extend_add(T* pool, const int* lidx, const NodeDev* nodes, const Ext
 template <typename T>
 void launch_small_front(T* pool, const NodeDev* nodes, const int* tasks, int count, int* info, hipStream_t st)
 {
-    if (count > 0) hipLaunchKernelGGL(k_small_front<T>, dim3(count), dim3(64), 0, st, pool, nodes, tasks, info);
+    if (count > 0) hipLaunchKernelGGL(k_small_front<T>, dim3(count), dim3(kThreads), 0, st, pool, nodes, tasks, info);
 }
 template <typename T>
-void launch_panel(T* pool, const NodeDev* nodes, const SolveTask* tasks, int count, int64_t scratch, int* info,
-                  hipStream_t st)
+void launch_diag(T* pool, const NodeDev* nodes, const DiagTask* tasks, int count, int* info, hipStream_t st)
 {
-    if (count > 0) hipLaunchKernelGGL(k_panel<T>, dim3(count), dim3(kThreads), 0, st, pool, nodes, tasks, scratch, info);
+    if (count > 0) hipLaunchKernelGGL(k_diag<T>, dim3(count), dim3(kThreads), 0, st, pool, nodes, tasks, info);
 }
 template <typename T> void launch_gemm(T* pool, const GemmTask* tasks, int count, hipStream_t st)
 {
     if (count > 0) hipLaunchKernelGGL(k_gemm<T>, dim3(count), dim3(kThreads), 0, st, pool, tasks);
+}
+template <typename T> void launch_gemm128(T* pool, const GemmTask* tasks, int count, hipStream_t st)
+{
+    if (count > 0) hipLaunchKernelGGL(k_gemm128<T>, dim3(count), dim3(kThreads), 0, st, pool, tasks);
 }
 void launch_fwd_gather(double* zpool, const double* b, int ldb, int nrhs, const NodeDev* nodes, const int* lidx,
                        const int* child_ptr, const int* child_list, const int* tasks, int count, hipStream_t st)
@@ -728,19 +919,12 @@ void launch_bwd_gather(double* zpool, int nrhs, const NodeDev* nodes, const int*
     if (count > 0) hipLaunchKernelGGL(k_bwd_gather, dim3(count), dim3(kThreads), 0, st, zpool, nrhs, nodes, lidx, tasks);
 }
 template <typename T>
-void launch_bwd_partial(const T* pool, double* zpool, double* part, int nrhs, const NodeDev* nodes,
-                        const SolveTask* tasks, int count, hipStream_t st)
-{
-    if (count > 0)
-        hipLaunchKernelGGL(k_bwd_partial<T>, dim3(count), dim3(kThreads), 0, st, pool, zpool, part, nrhs, nodes, tasks);
-}
-template <typename T>
-void launch_bwd_diag(const T* pool, double* zpool, const double* part, double* x, int ldx, int nrhs,
+void launch_bwd_step(const T* pool, double* zpool, double* part, unsigned* counters, double* x, int ldx, int nrhs,
                      const NodeDev* nodes, const SolveTask* tasks, int count, hipStream_t st)
 {
     if (count > 0)
-        hipLaunchKernelGGL(k_bwd_diag<T>, dim3(count), dim3(kThreads), 0, st, pool, zpool, part, x, ldx, nrhs, nodes,
-                           tasks);
+        hipLaunchKernelGGL(k_bwd_step<T>, dim3(count), dim3(kThreads), 0, st, pool, zpool, part, counters, x, ldx, nrhs,
+                           nodes, tasks);
 }
 void launch_gather_rows(double* out, int ld_out, const double* in, int ld_in, const int* perm, int n, int nrhs,
                         hipStream_t st)
@@ -761,15 +945,15 @@ void launch_scatter_rows(double* out, int ld_out, const double* in, int ld_in, c
 
 #define MFD_INSTANTIATE(T)                                                                                             \
     template void launch_fill<T>(T*, int64_t, hipStream_t);                                                            \
+    template void launch_fill_lower<T>(T*, const NodeDev*, const int2*, int, hipStream_t);                            \
     template void launch_scatter_a<T>(T*, const double*, const int64_t*, const int64_t*, int64_t, hipStream_t);        \
     template void launch_extend_add<T>(T*, const int*, const NodeDev*, const ExtTask*, int, hipStream_t);              \
     template void launch_small_front<T>(T*, const NodeDev*, const int*, int, int*, hipStream_t);                       \
-    template void launch_panel<T>(T*, const NodeDev*, const SolveTask*, int, int64_t, int*, hipStream_t);              \
+    template void launch_diag<T>(T*, const NodeDev*, const DiagTask*, int, int*, hipStream_t);                        \
     template void launch_gemm<T>(T*, const GemmTask*, int, hipStream_t);                                               \
+    template void launch_gemm128<T>(T*, const GemmTask*, int, hipStream_t);                                            \
     template void launch_fwd_step<T>(const T*, double*, int, const NodeDev*, const SolveTask*, int, hipStream_t);       \
-    template void launch_bwd_partial<T>(const T*, double*, double*, int, const NodeDev*, const SolveTask*, int,        \
-                                        hipStream_t);                                                                  \
-    template void launch_bwd_diag<T>(const T*, double*, const double*, double*, int, int, const NodeDev*,              \
+    template void launch_bwd_step<T>(const T*, double*, double*, unsigned*, double*, int, int, const NodeDev*,        \
                                      const SolveTask*, int, hipStream_t);
 MFD_INSTANTIATE(double)
 MFD_INSTANTIATE(float)

@@ -31,29 +31,33 @@ struct ExtTask {
     int child, ti, tj, pad;
 };
 
-// Panel step of a large front: diagonal block [k0, k0+kb) and, for the TRSM
-// kernel, the 64-row chunk starting at row0 (row0 >= k0 + kb).
-struct PanelTask {
-    int node, k0, kb, row0;
+// Diagonal block [k0, k0+kb) of a large front, to be factored (its updates
+// are done) and inverted.
+struct DiagTask {
+    int node, k0, kb, pad;
 };
 
 // One block step of the blocked supernodal solves: the pivot block
-// [k0, k0+kb) of a front and, for the row-chunk kernels, the 256 rows from
-// row0.  slot/nchunk address the backward sweep's partial-sum scratch
-// (slot s holds 64 x nrhs doubles).
+// [k0, k0+kb) of a front and, for the row-chunk tasks, the 256 rows from
+// row0 (row0 < 0: the block's own task).  Backward sweep: slot is the
+// chunk's partial-sum slot (64 x nrhs doubles), group the first slot of the
+// front's chunks (also the index of its completion counter) and nchunk their
+// number.
 struct SolveTask {
     int node, k0, kb, row0;
-    int slot, nchunk, pad0, pad1;
+    int slot, nchunk, group, pad;
 };
 
-// C (m x n tile, ldc) -= A (m x k, lda) * B (n x k, ldb)^T, offsets into the
-// front pool; tri != 0 keeps only the lower triangle (row >= col) of C.
-// k == 0 marks a copy task: the lower m x n block at a (lda) is copied to c.
+// mode 0: C (m x n tile, ldc) -= A (m x k, lda) * B (n x k, ldb)^T;
+// mode 1: C = A B^T with B stored transposed (B[j][k] at b + j * ldb + k);
+// A may be C itself: the panel TRSM X = P L^{-T}, with b the stored L^{-T}
+// (column-major), whose transpose is L^{-1}.  Offsets are into the front
+// pool; tri != 0 keeps only the lower triangle (row >= col) of C.
 struct GemmTask {
     int64_t c, a, b;
     int ldc, lda, ldb;
     int m, n, k;
-    int tri, pad;
+    int tri, mode;
 };
 
 }  // namespace mfd

@@ -33,7 +33,12 @@ struct Launch {
     int kind;       // MFD_K_*
     int64_t first;  // first task in the kind's task array
     int count;
+    int wide = 0;   // GEMM launches: 1 = 128 x 128 tiles (k_gemm128)
 };
+
+// Update and Schur GEMMs: 128 x 128 tiles (k_gemm128) or 64 x 64 (k_gemm).
+constexpr bool kUseWide = false;
+constexpr int kWide = kUseWide ? 128 : kTile;
 
 double now_ms()
 {
@@ -74,17 +79,18 @@ struct mfd_handle {
     std::vector<NodeDev> nodes;
     std::vector<ExtTask> ext;
     std::vector<int> small;
-    std::vector<SolveTask> panel;       // fused diagonal factorization + TRSM tasks
-    int64_t scratch_off = 0;             // pool offset of the factored-diagonal scratch
+    std::vector<DiagTask> diag;         // diagonal-block factorizations of large fronts
+    std::vector<int2> fill;             // (node, first column) blocks of the pool's lower triangles
     std::vector<GemmTask> gemm;
     std::vector<Launch> fseq;
     std::vector<int> level_ptr, level_nodes, child_ptr, child_list;
     // solve schedule: kinds kFG/kFS (forward) and kBG/kBP/kBD (backward)
-    std::vector<SolveTask> fstep, bpart, bdiag;
+    std::vector<SolveTask> fstep, bstep;
     std::vector<Launch> fwd_seq, bwd_seq;
     int64_t part_slots = 0;
-    SolveTask *d_fstep = nullptr, *d_bpart = nullptr, *d_bdiag = nullptr;
+    SolveTask *d_fstep = nullptr, *d_bstep = nullptr;
     double* d_part = nullptr;
+    unsigned* d_bcount = nullptr;  // backward-step completion counters (part_slots, zero between steps)
     int64_t pool_elems = 0, z_per_rhs = 0;
     int num_levels = 0;
 
@@ -93,7 +99,8 @@ struct mfd_handle {
     int* d_lidx = nullptr;
     ExtTask* d_ext = nullptr;
     int* d_small = nullptr;
-    SolveTask* d_panel = nullptr;
+    DiagTask* d_diag = nullptr;
+    int2* d_fill = nullptr;
     GemmTask* d_gemm = nullptr;
     int *d_level_nodes = nullptr, *d_child_ptr = nullptr, *d_child_list = nullptr;
     int64_t *d_dst = nullptr, *d_src = nullptr;
@@ -172,10 +179,12 @@ void free_device(mfd_handle* h)
     h->d_lidx = nullptr;
     h->d_ext = nullptr;
     h->d_small = nullptr;
-    h->d_panel = nullptr;
+    h->d_diag = nullptr;
+    h->d_fill = nullptr;
     h->d_gemm = nullptr;
     h->d_level_nodes = h->d_child_ptr = h->d_child_list = nullptr;
-    h->d_fstep = h->d_bpart = h->d_bdiag = nullptr;
+    h->d_fstep = h->d_bstep = nullptr;
+    h->d_bcount = nullptr;
     h->d_part = nullptr;
     h->d_dst = h->d_src = nullptr;
     h->d_perm = nullptr;
@@ -194,18 +203,18 @@ void free_device(mfd_handle* h)
 int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 // ------------------------------------------------------------- schedule
-enum SolveKind { kFG = 100, kFS, kBG, kBP, kBD };
+enum SolveKind { kFG = 100, kFS, kBG, kBS };
 
 // Solve schedule: per level, a gather launch, then for every 64-column pivot
 // block one forward launch (the block's writer task plus its row chunks) and
-// two backward launches (blocks in descending order: partial dot products
-// over row chunks, then their in-order sum and the diagonal block).
-// Partial-sum slots are reused from zero by every block step.
+// one backward launch (blocks in descending order: partial dot products over
+// row chunks, the last chunk of each front then sums them in order and
+// applies the diagonal block).  Partial-sum slots are reused from zero by
+// every block step.
 void build_solve_plan(mfd_handle* h)
 {
     h->fstep.clear();
-    h->bpart.clear();
-    h->bdiag.clear();
+    h->bstep.clear();
     h->fwd_seq.clear();
     h->bwd_seq.clear();
     h->part_slots = 0;
@@ -241,19 +250,23 @@ void build_solve_plan(mfd_handle* h)
         push(h->bwd_seq, kBG, b0, b1);
         for (int t = steps_of(L) - 1; t >= 0; --t) {
             const int k0 = 64 * t;
-            int64_t f0 = (int64_t)h->bpart.size(), f1 = (int64_t)h->bdiag.size();
+            int64_t f0 = (int64_t)h->bstep.size();
             int slot = 0;
             for (int k = b0; k < b1; ++k) {
                 const int s = h->level_nodes[k];
                 const NodeDev& d = h->nodes[s];
                 if (d.c <= k0) continue;
                 const int kb = std::min(64, d.c - k0), first = slot;
-                for (int r0 = k0 + kb; r0 < d.n; r0 += 256) h->bpart.push_back(SolveTask{s, k0, kb, r0, slot++, 0, 0, 0});
-                h->bdiag.push_back(SolveTask{s, k0, kb, 0, first, slot - first, 0, 0});
+                const int nchunk = (d.n - k0 - kb + 255) / 256;
+                if (nchunk == 0) {
+                    h->bstep.push_back(SolveTask{s, k0, kb, -1, first, 0, first, 0});
+                    continue;
+                }
+                for (int r0 = k0 + kb; r0 < d.n; r0 += 256)
+                    h->bstep.push_back(SolveTask{s, k0, kb, r0, slot++, nchunk, first, 0});
             }
-            h->part_slots = std::max<int64_t>(h->part_slots, slot);
-            push(h->bwd_seq, kBP, f0, (int64_t)h->bpart.size());
-            push(h->bwd_seq, kBD, f1, (int64_t)h->bdiag.size());
+            h->part_slots = std::max<int64_t>(h->part_slots, std::max(slot, 1));
+            push(h->bwd_seq, kBS, f0, (int64_t)h->bstep.size());
         }
     }
 }
@@ -279,20 +292,7 @@ void build_plan(mfd_handle* h)
         d.inv = inv_blocks;  // rebased below
         inv_blocks += (d.c + kPanel - 1) / kPanel;
     }
-    // pool layout: fronts | factored-diagonal scratch | diagonal-block inverses
-    int max_large = 0;
-    {
-        std::vector<int> lv_large;
-        std::vector<int> lev0(m, 0);
-        for (int s = 0; s < m; ++s)
-            for (int ch : S.children[s]) lev0[s] = std::max(lev0[s], lev0[ch - 1] + 1);
-        int nl = m ? *std::max_element(lev0.begin(), lev0.end()) + 1 : 0;
-        lv_large.assign(nl, 0);
-        for (int s = 0; s < m; ++s)
-            if (h->nodes[s].n > kSmallFront) max_large = std::max(max_large, ++lv_large[lev0[s]]);
-    }
-    h->scratch_off = off;
-    off += (int64_t)max_large * kPanel * kPanel;
+    // pool layout: fronts | diagonal-block inverses
     for (int s = 0; s < m; ++s) h->nodes[s].inv = off + h->nodes[s].inv * kPanel * kPanel;
     off += inv_blocks * kPanel * kPanel;
     h->pool_elems = off;
@@ -319,11 +319,14 @@ void build_plan(mfd_handle* h)
 
     h->ext.clear();
     h->small.clear();
-    h->panel.clear();
+    h->diag.clear();
+    h->fill.clear();
+    for (int s = 0; s < m; ++s)
+        for (int j0 = 0; j0 < h->nodes[s].n; j0 += 64) h->fill.push_back(make_int2(s, j0));
     h->gemm.clear();
     h->fseq.clear();
-    auto push = [&](int kind, int64_t first, int64_t end) {
-        if (end > first) h->fseq.push_back(Launch{kind, first, (int)(end - first)});
+    auto push = [&](int kind, int64_t first, int64_t end, int wide = 0) {
+        if (end > first) h->fseq.push_back(Launch{kind, first, (int)(end - first), wide});
     };
     for (int L = 0; L < h->num_levels; ++L) {
         const int* lv = h->level_nodes.data() + h->level_ptr[L];
@@ -351,80 +354,86 @@ void build_plan(mfd_handle* h)
             else large.push_back(lv[k]);
         }
         push(MFD_K_SMALL_FRONT, first, (int64_t)h->small.size());
-        // large fronts: panel steps, then one Schur update with K = c
+        // large fronts, per 64-column step t: TRSM of the rows below the
+        // diagonal block (product with its stored inverse), the in-panel
+        // update of the remaining pivot columns, then the next diagonal
+        // block (Cholesky, inverse); one Schur update with K = c closes the
+        // level
         int steps = 0;
         for (int s : large) steps = std::max(steps, (h->nodes[s].c + kPanel - 1) / kPanel);
+        {
+            int64_t f0 = (int64_t)h->diag.size();
+            for (int s : large) h->diag.push_back(DiagTask{s, 0, std::min(kPanel, h->nodes[s].c), 0});
+            push(MFD_K_POTRF, f0, (int64_t)h->diag.size());
+        }
         for (int t = 0; t < steps; ++t) {
             const int k0 = t * kPanel;
-            // panel launch: per front one writer task (slot = its index in
-            // this step) and the 256-row chunks below the diagonal block
-            int64_t f0 = (int64_t)h->panel.size();
-            int slot = 0;
-            std::vector<int> slot_of(large.size(), -1);
-            for (size_t q = 0; q < large.size(); ++q) {
-                const int s = large[q];
-                const NodeDev& d = h->nodes[s];
-                if (d.c <= k0) continue;
-                const int kb = std::min(kPanel, d.c - k0);
-                slot_of[q] = slot;
-                h->panel.push_back(SolveTask{s, k0, kb, -1, slot++, 0, 0, 0});
-                for (int r0 = k0 + kb; r0 < d.n; r0 += 256) h->panel.push_back(SolveTask{s, k0, kb, r0, 0, 0, 0, 0});
-            }
-            push(MFD_K_POTRF, f0, (int64_t)h->panel.size());
-            // update launch: copy each factored diagonal block into its front,
-            // then the trailing update of the remaining pivot columns
-            int64_t f2 = (int64_t)h->gemm.size();
-            for (size_t q = 0; q < large.size(); ++q) {
-                if (slot_of[q] < 0) continue;
-                const NodeDev& d = h->nodes[large[q]];
-                GemmTask g{};
-                g.c = d.front + k0 + (int64_t)k0 * d.n;
-                g.a = h->scratch_off + (int64_t)slot_of[q] * kPanel * kPanel;
-                g.ldc = d.n;
-                g.lda = g.ldb = kPanel;
-                g.m = g.n = std::min(kPanel, d.c - k0);
-                g.k = 0;
-                g.tri = 1;
-                h->gemm.push_back(g);
-            }
+            int64_t f1 = (int64_t)h->gemm.size();
             for (int s : large) {
                 const NodeDev& d = h->nodes[s];
                 if (d.c <= k0) continue;
-                int kb = std::min(kPanel, d.c - k0);
-                for (int c0 = k0 + kb; c0 < d.c; c0 += kTile)
-                    for (int r0 = c0; r0 < d.n; r0 += kTile) {
+                const int kb = std::min(kPanel, d.c - k0);
+                for (int r0 = k0 + kb; r0 < d.n; r0 += kTile) {
+                    GemmTask g{};
+                    g.c = g.a = d.front + r0 + (int64_t)k0 * d.n;
+                    g.b = d.inv + (int64_t)t * kPanel * kPanel;  // L^{-T}, read transposed
+                    g.ldc = g.lda = d.n;
+                    g.ldb = kPanel;
+                    g.m = std::min(kTile, d.n - r0);
+                    g.n = g.k = kb;
+                    g.mode = 1;
+                    h->gemm.push_back(g);
+                }
+            }
+            push(MFD_K_TRSM, f1, (int64_t)h->gemm.size());
+            int64_t f3 = (int64_t)h->gemm.size();
+            for (int s : large) {
+                const NodeDev& d = h->nodes[s];
+                if (d.c <= k0) continue;
+                const int kb = std::min(kPanel, d.c - k0);
+                for (int c0 = k0 + kb; c0 < d.c; c0 += kWide) {
+                    const int nc = std::min(kWide, d.c - c0);
+                    for (int r0 = c0; r0 < d.n; r0 += kWide) {
                         GemmTask g{};
                         g.c = d.front + r0 + (int64_t)c0 * d.n;
                         g.a = d.front + r0 + (int64_t)k0 * d.n;
                         g.b = d.front + c0 + (int64_t)k0 * d.n;
                         g.ldc = g.lda = g.ldb = d.n;
-                        g.m = std::min(kTile, d.n - r0);
-                        g.n = std::min(kTile, d.c - c0);
+                        g.m = std::min(kWide, d.n - r0);
+                        g.n = nc;
                         g.k = kb;
                         g.tri = r0 == c0;
                         h->gemm.push_back(g);
                     }
+                }
             }
-            push(MFD_K_UPDATE, f2, (int64_t)h->gemm.size());
+            push(MFD_K_UPDATE, f3, (int64_t)h->gemm.size(), kUseWide);
+            int64_t f2 = (int64_t)h->diag.size();
+            for (int s : large) {
+                const NodeDev& d = h->nodes[s];
+                if (d.c <= k0 + kPanel) continue;
+                h->diag.push_back(DiagTask{s, k0 + kPanel, std::min(kPanel, d.c - k0 - kPanel), 0});
+            }
+            push(MFD_K_POTRF, f2, (int64_t)h->diag.size());
         }
-        int64_t f3 = (int64_t)h->gemm.size();
+        int64_t f4 = (int64_t)h->gemm.size();
         for (int s : large) {
             const NodeDev& d = h->nodes[s];
-            for (int c0 = d.c; c0 < d.n; c0 += kTile)
-                for (int r0 = c0; r0 < d.n; r0 += kTile) {
+            for (int c0 = d.c; c0 < d.n; c0 += kWide)
+                for (int r0 = c0; r0 < d.n; r0 += kWide) {
                     GemmTask g{};
                     g.c = d.front + r0 + (int64_t)c0 * d.n;
                     g.a = d.front + r0;
                     g.b = d.front + c0;
                     g.ldc = g.lda = g.ldb = d.n;
-                    g.m = std::min(kTile, d.n - r0);
-                    g.n = std::min(kTile, d.n - c0);
+                    g.m = std::min(kWide, d.n - r0);
+                    g.n = std::min(kWide, d.n - c0);
                     g.k = d.c;
                     g.tri = r0 == c0;
                     h->gemm.push_back(g);
                 }
         }
-        push(MFD_K_SCHUR, f3, (int64_t)h->gemm.size());
+        push(MFD_K_SCHUR, f4, (int64_t)h->gemm.size(), kUseWide);
     }
     build_solve_plan(h);
 }
@@ -479,14 +488,16 @@ int setup_device(mfd_handle* h)
     MFD_TRY(upload(h, &h->d_lidx, S.lidx));
     MFD_TRY(upload(h, &h->d_ext, h->ext));
     MFD_TRY(upload(h, &h->d_small, h->small));
-    MFD_TRY(upload(h, &h->d_panel, h->panel));
+    MFD_TRY(upload(h, &h->d_diag, h->diag));
+    MFD_TRY(upload(h, &h->d_fill, h->fill));
     MFD_TRY(upload(h, &h->d_gemm, h->gemm));
     MFD_TRY(upload(h, &h->d_level_nodes, h->level_nodes));
     MFD_TRY(upload(h, &h->d_child_ptr, h->child_ptr));
     MFD_TRY(upload(h, &h->d_child_list, h->child_list));
     MFD_TRY(upload(h, &h->d_fstep, h->fstep));
-    MFD_TRY(upload(h, &h->d_bpart, h->bpart));
-    MFD_TRY(upload(h, &h->d_bdiag, h->bdiag));
+    MFD_TRY(upload(h, &h->d_bstep, h->bstep));
+    MFD_TRY(dalloc(h, &h->d_bcount, std::max<int64_t>(h->part_slots, 1)));
+    HIP_TRY(hipMemset(h->d_bcount, 0, sizeof(unsigned) * std::max<int64_t>(h->part_slots, 1)));
     // assembly map: A entry -> front position (coefficient_setter_m.f90:26-41)
     const int64_t nnz = (int64_t)S.snidx.size();
     std::vector<int64_t> dst(nnz), src(nnz);
@@ -548,11 +559,15 @@ template <typename T> void enqueue_launch(mfd_handle* h, const Launch& l)
     switch (l.kind) {
     case MFD_K_EXTEND_ADD: launch_extend_add<T>(pool, h->d_lidx, h->d_nodes, h->d_ext + l.first, l.count, st); break;
     case MFD_K_SMALL_FRONT: launch_small_front<T>(pool, h->d_nodes, h->d_small + l.first, l.count, h->d_info, st); break;
-    case MFD_K_POTRF:
-        launch_panel<T>(pool, h->d_nodes, h->d_panel + l.first, l.count, h->scratch_off, h->d_info, st);
-        break;
+    case MFD_K_POTRF: launch_diag<T>(pool, h->d_nodes, h->d_diag + l.first, l.count, h->d_info, st); break;
+    case MFD_K_TRSM:
     case MFD_K_UPDATE:
-    case MFD_K_SCHUR: launch_gemm<T>(pool, h->d_gemm + l.first, l.count, st); break;
+    case MFD_K_SCHUR:
+        if (l.wide)
+            launch_gemm128<T>(pool, h->d_gemm + l.first, l.count, st);
+        else
+            launch_gemm<T>(pool, h->d_gemm + l.first, l.count, st);
+        break;
     default: break;
     }
 }
@@ -574,7 +589,7 @@ template <typename T> void enqueue_factor(mfd_handle* h, bool profile)
     };
     launch_set_int(h->d_info, INT_MAX, st);
     mark(0);
-    launch_fill<T>(pool, h->pool_elems, st);
+    launch_fill_lower<T>(pool, h->d_nodes, h->d_fill, (int)h->fill.size(), st);
     launch_scatter_a<T>(pool, h->d_a, h->d_dst, h->d_src, (int64_t)h->S.snidx.size(), st);
     mark(0);
     for (const Launch& l : h->fseq) {
@@ -684,11 +699,9 @@ template <typename T> void enqueue_solve(mfd_handle* h, const double* bp, double
     for (const Launch& l : h->bwd_seq) {
         switch (l.kind) {
         case kBG: launch_bwd_gather(h->d_z, nrhs, h->d_nodes, h->d_lidx, h->d_level_nodes + l.first, l.count, st); break;
-        case kBP:
-            launch_bwd_partial<T>(pool, h->d_z, h->d_part, nrhs, h->d_nodes, h->d_bpart + l.first, l.count, st);
-            break;
-        case kBD:
-            launch_bwd_diag<T>(pool, h->d_z, h->d_part, xp, n, nrhs, h->d_nodes, h->d_bdiag + l.first, l.count, st);
+        case kBS:
+            launch_bwd_step<T>(pool, h->d_z, h->d_part, h->d_bcount, xp, n, nrhs, h->d_nodes, h->d_bstep + l.first, l.count,
+                               st);
             break;
         default: break;
         }
@@ -982,6 +995,9 @@ int mfd_export_front(mfd_handle* h, int node, double* out)
         HIP_TRY(hipMemcpy(tmp.data(), h->d_pool32 + d.front, cnt * 4, hipMemcpyDeviceToHost));
         for (int64_t i = 0; i < cnt; ++i) out[i] = tmp[i];
     }
+    // only the lower part is kept on the device; the rest reads as zero
+    for (int q = 1; q < d.n; ++q)
+        for (int r = 0; r < q; ++r) out[(int64_t)q * d.n + r] = 0.0;
     return MFD_OK;
 }
 
@@ -1087,15 +1103,19 @@ int mfd_get_kernel_work(mfd_handle* h, int k, int s, double* flops, double* byte
             b += 2.0 * s * tri(d.n);
         }
         break;
-    case MFD_K_POTRF:  // fused panel: one diagonal factorization per step + TRSM of the rows below
-        for (const auto& t : h->panel) {
-            if (t.row0 < 0) {
-                f += (double)t.kb * t.kb * t.kb / 3.0;
-                b += 2.0 * s * tri(t.kb);
-            } else {
-                double rows = std::min(256, h->nodes[t.node].n - t.row0);
-                f += rows * t.kb * t.kb;
-                b += s * (2.0 * rows * t.kb);
+    case MFD_K_POTRF:  // diagonal block: Cholesky and inverse
+        for (const auto& t : h->diag) {
+            f += 2.0 * t.kb * t.kb * t.kb / 3.0;
+            b += s * (2.0 * tri(t.kb) + (double)t.kb * t.kb);
+        }
+        break;
+    case MFD_K_TRSM:  // rows below a diagonal block times its inverse
+        for (const Launch& l : h->fseq) {
+            if (l.kind != k) continue;
+            for (int64_t i = l.first; i < l.first + l.count; ++i) {
+                const GemmTask& g = h->gemm[i];
+                f += (double)g.m * g.n * g.n;
+                b += s * (2.0 * g.m * g.n + (double)g.n * g.n);
             }
         }
         break;
