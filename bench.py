@@ -98,11 +98,29 @@ def roofline(p, b, max_zero, nb, precision):
         ach = r["gbs"]
         entry = {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                  "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None}
+    if (p.n, max_zero, nb) == (64 ** 3, 16384, 64) and precision == mfdag.FP64:
+        entry["traffic"] = measured_traffic(dom)
     entry["kernel"] = dom
     entry["avg_launch_ms"] = round(r["ms"] / r["launches"], 5)
     entry["launches"] = r["launches"]
     s.close()
     return entry, rows
+
+
+def measured_traffic(kernel_class):
+    """HBM bytes per launch of a kernel class from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (calibrated against a copy of known size),
+    or None when the summary does not cover the class."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_wlap3d_64cube_fp64.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch:" and parts[1] == kernel_class:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(size, max_zero, nb, seed):
@@ -217,6 +235,7 @@ def main():
                     "speedup_vs_fp64": round(wall64 / wall32, 3)},
         "analyze_s": round(analyze_s, 3),
         "roofline": roof,
+        "kernel_classes_fp64": {k: {"ms": round(r["ms"], 4), "launches": r["launches"]} for k, r in classes.items()},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -387,7 +387,9 @@ __device__ __forceinline__ void tile_product(const T* A, int lda, const T* B, in
 // panel TRSM as a product with the stored L^{-T} of the diagonal block, C
 // and A the same rows).
 // The result goes through LDS so the write of C is coalesced along columns.
-template <typename T>
+// KIND only names the instance after the kernel class that launches it
+// (TRSM, in-panel update, Schur update), so that profiles tell them apart.
+template <typename T, int KIND>
 __global__ __launch_bounds__(kThreads) void k_gemm(T* pool, const GemmTask* tasks)
 {
     const GemmTask t = tasks[blockIdx.x];
@@ -669,9 +671,13 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zp
     const SolveTask t = tasks[blockIdx.x];
     const NodeDev nd = nodes[t.node];
     const int n = nd.n, c = nd.c, kb = t.kb, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    __shared__ double Ls[64][65];  // Ls[k][i]: column k0 + k of L, row i of the current 64-row block
+    // Ls[k][i]: column k0 + k of L, row i of the current 64-row block (chunk
+    // phase); LI: the diagonal block's inverse (diagonal phase)
+    __shared__ union {
+        double Ls[64][65];
+        double LI[64][65];
+    } u;
     __shared__ double Xs[256][kRhsGroup + 1];
-    __shared__ double LI[64][65];
     __shared__ double zt[64][kRhsGroup];
     __shared__ int is_last;
     double* Z = zpool + nd.z * nrhs;
@@ -697,13 +703,13 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zp
                 __syncthreads();
                 if (wv == sb) {
 #pragma unroll
-                    for (int k = 0; k < 64; ++k) Ls[k][lane] = l[k];
+                    for (int k = 0; k < 64; ++k) u.Ls[k][lane] = l[k];
                 }
                 __syncthreads();
 #pragma unroll
                 for (int q = 0; q < 16; ++q) {
                     const int rr = 4 * q + (lane >> 4);
-                    const double a = Ls[16 * wv + rc][rr];
+                    const double a = u.Ls[16 * wv + rc][rr];
                     const double b = rc < nr ? Xs[64 * sb + rr][rc] : 0.0;
                     acc = Mfma<double>::mma(a, b, acc);
                 }
@@ -723,7 +729,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zp
         __threadfence();
         if (tid == 0) counters[group] = 0u;
     }
-    load_inverse<T>(pool + nd.inv + (int64_t)(t.k0 / 64) * 4096, LI);
+    load_inverse<T>(pool + nd.inv + (int64_t)(t.k0 / 64) * 4096, u.LI);
     const int nchunk = t.row0 >= 0 ? t.nchunk : 0;
     for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
         const int nr = min(kRhsGroup, nrhs - r0);
@@ -739,7 +745,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zp
         __syncthreads();
         for (int r = wv; r < nr; r += 4) {
             double acc = 0.0;
-            for (int m = 0; m < kb; ++m) acc += LI[lane][m] * zt[m][r];
+            for (int m = 0; m < kb; ++m) acc += u.LI[lane][m] * zt[m][r];
             if (lane < kb) {
                 Y[t.k0 + lane + (int64_t)(r0 + r) * c] = acc;
                 if (t.k0 == 0) x[(int64_t)(nd.first + lane) + (int64_t)(r0 + r) * ldx] = acc;
@@ -892,9 +898,15 @@ void launch_diag(T* pool, const NodeDev* nodes, const DiagTask* tasks, int count
 {
     if (count > 0) hipLaunchKernelGGL(k_diag<T>, dim3(count), dim3(kThreads), 0, st, pool, nodes, tasks, info);
 }
-template <typename T> void launch_gemm(T* pool, const GemmTask* tasks, int count, hipStream_t st)
+template <typename T> void launch_gemm(T* pool, const GemmTask* tasks, int count, int kind, hipStream_t st)
 {
-    if (count > 0) hipLaunchKernelGGL(k_gemm<T>, dim3(count), dim3(kThreads), 0, st, pool, tasks);
+    if (count <= 0) return;
+    if (kind == 0)
+        hipLaunchKernelGGL((k_gemm<T, 0>), dim3(count), dim3(kThreads), 0, st, pool, tasks);
+    else if (kind == 1)
+        hipLaunchKernelGGL((k_gemm<T, 1>), dim3(count), dim3(kThreads), 0, st, pool, tasks);
+    else
+        hipLaunchKernelGGL((k_gemm<T, 2>), dim3(count), dim3(kThreads), 0, st, pool, tasks);
 }
 template <typename T> void launch_gemm128(T* pool, const GemmTask* tasks, int count, hipStream_t st)
 {
@@ -950,7 +962,7 @@ void launch_scatter_rows(double* out, int ld_out, const double* in, int ld_in, c
     template void launch_extend_add<T>(T*, const int*, const NodeDev*, const ExtTask*, int, hipStream_t);              \
     template void launch_small_front<T>(T*, const NodeDev*, const int*, int, int*, hipStream_t);                       \
     template void launch_diag<T>(T*, const NodeDev*, const DiagTask*, int, int*, hipStream_t);                        \
-    template void launch_gemm<T>(T*, const GemmTask*, int, hipStream_t);                                               \
+    template void launch_gemm<T>(T*, const GemmTask*, int, int, hipStream_t);                                          \
     template void launch_gemm128<T>(T*, const GemmTask*, int, hipStream_t);                                            \
     template void launch_fwd_step<T>(const T*, double*, int, const NodeDev*, const SolveTask*, int, hipStream_t);       \
     template void launch_bwd_step<T>(const T*, double*, double*, unsigned*, double*, int, int, const NodeDev*,        \

@@ -566,7 +566,8 @@ template <typename T> void enqueue_launch(mfd_handle* h, const Launch& l)
         if (l.wide)
             launch_gemm128<T>(pool, h->d_gemm + l.first, l.count, st);
         else
-            launch_gemm<T>(pool, h->d_gemm + l.first, l.count, st);
+            launch_gemm<T>(pool, h->d_gemm + l.first, l.count,
+                           l.kind == MFD_K_TRSM ? 0 : l.kind == MFD_K_UPDATE ? 1 : 2, st);
         break;
     default: break;
     }
