@@ -31,8 +31,8 @@ void launch_fwd_step(const T* pool, double* zpool, int nrhs, const NodeDev* node
 void launch_bwd_gather(double* zpool, int nrhs, const NodeDev* nodes, const int* lidx, const int* tasks, int count,
                        hipStream_t st);
 template <typename T>
-void launch_bwd_step(const T* pool, double* zpool, double* part, unsigned* counters, double* x, int ldx, int nrhs,
-                     const NodeDev* nodes, const SolveTask* tasks, int count, hipStream_t st);
+void launch_bwd_step(const T* pool, double* zpool, double* part, double* x, int ldx, int nrhs, const NodeDev* nodes,
+                     const SolveTask* tasks, int count, hipStream_t st);
 void launch_set_int(int* p, int v, hipStream_t st);
 void launch_residual(double* r, const double* b, const double* x, int n, int nrhs, const int* ptr, const int* col,
                      const int64_t* src, const double* a, hipStream_t st);

@@ -651,43 +651,43 @@ __global__ __launch_bounds__(kThreads) void k_bwd_gather(double* zpool, int nrhs
     }
 }
 
-// Backward block step for one front (DGEMV "N" + DTRSV "U","N",
-// solve_kernel_m.f90:11-16,25-30).  A chunk task (row0 >= 0) forms the
-// partial products P[slot][k][r] = sum over its 256 rows of L[row][k0+k]
-// x[row][r] (x from Y on pivot rows, from Z on update rows) on MFMA: the
-// chunk's L columns are read once into registers, then staged 64 rows at a
-// time through LDS.  It then counts itself done on the group's counter; the
-// last chunk of the group (or the single task of a front with no rows below
-// the block, row0 < 0) forms z = Y[k0:k0+kb] - sum of the partials in chunk
-// order and x_t = inv(L_tt)^T z, written to Y and, for the front's first
-// block, to x with the rest of the front's pivot part.  The partial sums
-// are added in a fixed order, so the result does not depend on which chunk
-// finishes last; the last chunk resets the counter for the next step.
+// Backward sweep, one launch per block step t (DGEMV "N" + DTRSV "U","N",
+// solve_kernel_m.f90:11-16,25-30), pipelined one block ahead:
+//   - the block task (row0 < 0) solves block t:
+//       z = Y_t - sum of the nchunk partials at part[slot..] (rows after
+//           block t+1, formed by the previous launch) - L[B_{t+1}, B_t]^T x_{t+1}
+//       x_t = inv(L_tt)^T z, written to Y_t (and, for t = 0, the front's
+//       pivot part of x);  pad = kb of block t+1 (0: no such block);
+//   - a chunk task (row0 >= 0) forms, for block k0/64 (= t - 1), the partial
+//     products P[slot][k][r] = sum over its 256 rows of L[row][k0+k] x[row][r]
+//     (x from Y on pivot rows, from Z on update rows; every row lies after
+//     block t, so its x is final) on MFMA: the chunk's L columns are read once
+//     into registers, then staged 64 rows at a time through LDS.
+// No task of a launch depends on another task of the same launch, and the
+// partials are summed in a fixed order.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zpool, double* part, unsigned* counters,
-                                                       double* x, int ldx, int nrhs, const NodeDev* nodes,
-                                                       const SolveTask* tasks)
+__global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zpool, double* part, double* x, int ldx,
+                                                       int nrhs, const NodeDev* nodes, const SolveTask* tasks)
 {
     const SolveTask t = tasks[blockIdx.x];
     const NodeDev nd = nodes[t.node];
     const int n = nd.n, c = nd.c, kb = t.kb, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     // Ls[k][i]: column k0 + k of L, row i of the current 64-row block (chunk
-    // phase); LI: the diagonal block's inverse (diagonal phase)
+    // task); LI: the diagonal block's inverse (block task)
     __shared__ union {
         double Ls[64][65];
         double LI[64][65];
     } u;
     __shared__ double Xs[256][kRhsGroup + 1];
+    __shared__ double Ln[64][65];  // Ln[k][i] = L[k0 + 64 + i][k0 + k], block task
     __shared__ double zt[64][kRhsGroup];
-    __shared__ int is_last;
     double* Z = zpool + nd.z * nrhs;
     double* Y = Z + (int64_t)n * nrhs;
-    const int group = t.group;
+    const T* Lb = pool + nd.front + (int64_t)t.k0 * n;
     if (t.row0 >= 0) {
         typedef typename Mfma<double>::acc_t acc_t;
         const int row = t.row0 + tid;
         const bool live = row < n;
-        const T* Lb = pool + nd.front + (int64_t)t.k0 * n;
         double l[64];
 #pragma unroll
         for (int k = 0; k < 64; ++k) l[k] = (live && k < kb) ? (double)ld_off(Lb, (unsigned)(row + k * n)) : 0.0;
@@ -721,26 +721,46 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zp
             }
             __syncthreads();
         }
-        __threadfence();
-        __syncthreads();
-        if (tid == 0) is_last = atomicAdd(&counters[group], 1u) == (unsigned)(t.nchunk - 1);
-        __syncthreads();
-        if (!is_last) return;
-        __threadfence();
-        if (tid == 0) counters[group] = 0u;
+        return;
     }
-    load_inverse<T>(pool + nd.inv + (int64_t)(t.k0 / 64) * 4096, u.LI);
-    const int nchunk = t.row0 >= 0 ? t.nchunk : 0;
+    // block task: every global read first
+    const int kn = t.pad;  // rows of block t+1 (0: none)
+    const T* inv = pool + nd.inv + (int64_t)(t.k0 / 64) * 4096;
+    double li[16], ln[16];
+#pragma unroll
+    for (int p = 0; p < 16; ++p) {
+        const int e = tid + kThreads * p, i = e & 63, k = e >> 6;
+        li[p] = (double)inv[e];
+        ln[p] = (i < kn && k < kb) ? (double)ld_off(Lb, (unsigned)(t.k0 + 64 + i + k * n)) : 0.0;
+    }
+#pragma unroll
+    for (int p = 0; p < 16; ++p) {
+        const int e = tid + kThreads * p;
+        u.LI[e & 63][e >> 6] = li[p];
+        Ln[e >> 6][e & 63] = ln[p];
+    }
     for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
         const int nr = min(kRhsGroup, nrhs - r0);
         for (int e = tid; e < 64 * kRhsGroup; e += kThreads) {
-            int k = e & 63, r = e >> 6;
+            const int k = e & 63, r = e >> 6;
             double z = 0.0;
             if (k < kb && r < nr) {
                 z = Y[t.k0 + k + (int64_t)(r0 + r) * c];
-                for (int ch = 0; ch < nchunk; ++ch) z -= part[(int64_t)(group + ch) * 64 * nrhs + k + (int64_t)(r0 + r) * 64];
+                for (int ch = 0; ch < t.nchunk; ++ch) z -= part[(int64_t)(t.slot + ch) * 64 * nrhs + k + (int64_t)(r0 + r) * 64];
             }
             zt[k][r] = z;
+        }
+        // x of block t+1, staged in Xs[0..kn)
+        for (int e = tid; e < 64 * kRhsGroup; e += kThreads) {
+            const int i = e & 63, r = e >> 6;
+            Xs[i][r] = (i < kn && r < nr) ? Y[t.k0 + 64 + i + (int64_t)(r0 + r) * c] : 0.0;
+        }
+        __syncthreads();
+        // z_k -= sum_i L[k0 + 64 + i][k0 + k] x_{t+1}[i]
+        for (int r = wv; r < nr; r += 4) {
+            double acc = 0.0;
+            for (int i = 0; i < kn; ++i) acc += Ln[lane][i] * Xs[i][r];
+            if (lane < kb) zt[lane][r] -= acc;
         }
         __syncthreads();
         for (int r = wv; r < nr; r += 4) {
@@ -931,12 +951,12 @@ void launch_bwd_gather(double* zpool, int nrhs, const NodeDev* nodes, const int*
     if (count > 0) hipLaunchKernelGGL(k_bwd_gather, dim3(count), dim3(kThreads), 0, st, zpool, nrhs, nodes, lidx, tasks);
 }
 template <typename T>
-void launch_bwd_step(const T* pool, double* zpool, double* part, unsigned* counters, double* x, int ldx, int nrhs,
-                     const NodeDev* nodes, const SolveTask* tasks, int count, hipStream_t st)
+void launch_bwd_step(const T* pool, double* zpool, double* part, double* x, int ldx, int nrhs, const NodeDev* nodes,
+                     const SolveTask* tasks, int count, hipStream_t st)
 {
     if (count > 0)
-        hipLaunchKernelGGL(k_bwd_step<T>, dim3(count), dim3(kThreads), 0, st, pool, zpool, part, counters, x, ldx, nrhs,
-                           nodes, tasks);
+        hipLaunchKernelGGL(k_bwd_step<T>, dim3(count), dim3(kThreads), 0, st, pool, zpool, part, x, ldx, nrhs, nodes,
+                           tasks);
 }
 void launch_gather_rows(double* out, int ld_out, const double* in, int ld_in, const int* perm, int n, int nrhs,
                         hipStream_t st)
@@ -965,8 +985,8 @@ void launch_scatter_rows(double* out, int ld_out, const double* in, int ld_in, c
     template void launch_gemm<T>(T*, const GemmTask*, int, int, hipStream_t);                                          \
     template void launch_gemm128<T>(T*, const GemmTask*, int, hipStream_t);                                            \
     template void launch_fwd_step<T>(const T*, double*, int, const NodeDev*, const SolveTask*, int, hipStream_t);       \
-    template void launch_bwd_step<T>(const T*, double*, double*, unsigned*, double*, int, int, const NodeDev*,        \
-                                     const SolveTask*, int, hipStream_t);
+    template void launch_bwd_step<T>(const T*, double*, double*, double*, int, int, const NodeDev*, const SolveTask*,  \
+                                     int, hipStream_t);
 MFD_INSTANTIATE(double)
 MFD_INSTANTIATE(float)
 

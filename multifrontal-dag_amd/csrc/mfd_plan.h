@@ -39,13 +39,13 @@ struct DiagTask {
 
 // One block step of the blocked supernodal solves: the pivot block
 // [k0, k0+kb) of a front and, for the row-chunk tasks, the 256 rows from
-// row0 (row0 < 0: the block's own task).  Backward sweep: slot is the
-// chunk's partial-sum slot (64 x nrhs doubles), group the first slot of the
-// front's chunks (also the index of its completion counter) and nchunk their
-// number.
+// row0 (row0 < 0: the block's own task).  Backward sweep: a chunk task
+// writes partial-sum slot `slot` (64 x nrhs doubles); a block task reads the
+// nchunk slots from `slot` and pad is the order of the next pivot block (0
+// for the front's last block).
 struct SolveTask {
     int node, k0, kb, row0;
-    int slot, nchunk, group, pad;
+    int slot, nchunk, pad0, pad;
 };
 
 // mode 0: C (m x n tile, ldc) -= A (m x k, lda) * B (n x k, ldb)^T;

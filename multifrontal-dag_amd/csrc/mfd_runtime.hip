@@ -90,7 +90,6 @@ struct mfd_handle {
     int64_t part_slots = 0;
     SolveTask *d_fstep = nullptr, *d_bstep = nullptr;
     double* d_part = nullptr;
-    unsigned* d_bcount = nullptr;  // backward-step completion counters (part_slots, zero between steps)
     int64_t pool_elems = 0, z_per_rhs = 0;
     int num_levels = 0;
 
@@ -184,7 +183,6 @@ void free_device(mfd_handle* h)
     h->d_gemm = nullptr;
     h->d_level_nodes = h->d_child_ptr = h->d_child_list = nullptr;
     h->d_fstep = h->d_bstep = nullptr;
-    h->d_bcount = nullptr;
     h->d_part = nullptr;
     h->d_dst = h->d_src = nullptr;
     h->d_perm = nullptr;
@@ -206,11 +204,12 @@ int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 enum SolveKind { kFG = 100, kFS, kBG, kBS };
 
 // Solve schedule: per level, a gather launch, then for every 64-column pivot
-// block one forward launch (the block's writer task plus its row chunks) and
-// one backward launch (blocks in descending order: partial dot products over
-// row chunks, the last chunk of each front then sums them in order and
-// applies the diagonal block).  Partial-sum slots are reused from zero by
-// every block step.
+// block one forward launch (the block's writer task plus its row chunks).
+// Backward, per level: a gather launch, then launches t = steps .. 0: the
+// block task of block t (for fronts that have it) and the row chunks forming
+// the partial sums block t - 1 needs from the rows after block t (launch
+// t = steps only has chunks).  Partials of block j live in bank j % 2, so a
+// launch never overwrites the partials its block tasks read.
 void build_solve_plan(mfd_handle* h)
 {
     h->fstep.clear();
@@ -245,30 +244,53 @@ void build_solve_plan(mfd_handle* h)
             push(h->fwd_seq, kFS, f0, (int64_t)h->fstep.size());
         }
     }
+    // chunks of the partial sums of block j of front s: rows [min(64 (j + 2), c), n)
+    auto chunk_rows = [&](const NodeDev& d, int j) {
+        const int start = std::min(64 * (j + 2), d.c);
+        return std::make_pair(start, (d.n - start + 255) / 256);
+    };
+    auto blocks = [&](const NodeDev& d) { return (d.c + 63) / 64; };
+    int64_t bank = 1;  // slots per bank: the most chunks any launch has
+    for (int L = 0; L < h->num_levels; ++L) {
+        const int b0 = h->level_ptr[L], b1 = h->level_ptr[L + 1];
+        for (int t = steps_of(L); t >= 1; --t) {
+            int64_t cnt = 0;
+            for (int k = b0; k < b1; ++k) {
+                const NodeDev& d = h->nodes[h->level_nodes[k]];
+                if (t - 1 < blocks(d)) cnt += std::max(0, chunk_rows(d, t - 1).second);
+            }
+            bank = std::max(bank, cnt);
+        }
+    }
+    std::vector<int> qfirst(h->nodes.size(), 0), qcount(h->nodes.size(), 0);
     for (int L = h->num_levels - 1; L >= 0; --L) {
         const int b0 = h->level_ptr[L], b1 = h->level_ptr[L + 1];
         push(h->bwd_seq, kBG, b0, b1);
-        for (int t = steps_of(L) - 1; t >= 0; --t) {
-            const int k0 = 64 * t;
+        for (int t = steps_of(L); t >= 0; --t) {
             int64_t f0 = (int64_t)h->bstep.size();
-            int slot = 0;
+            int idx = 0;
             for (int k = b0; k < b1; ++k) {
                 const int s = h->level_nodes[k];
                 const NodeDev& d = h->nodes[s];
-                if (d.c <= k0) continue;
-                const int kb = std::min(64, d.c - k0), first = slot;
-                const int nchunk = (d.n - k0 - kb + 255) / 256;
-                if (nchunk == 0) {
-                    h->bstep.push_back(SolveTask{s, k0, kb, -1, first, 0, first, 0});
-                    continue;
+                const int T = blocks(d);
+                if (t <= T - 1) {
+                    const int k0 = 64 * t, kb = std::min(64, d.c - k0);
+                    const int kn = t + 1 <= T - 1 ? std::min(64, d.c - k0 - 64) : 0;
+                    h->bstep.push_back(SolveTask{s, k0, kb, -1, qfirst[s], qcount[s], 0, kn});
                 }
-                for (int r0 = k0 + kb; r0 < d.n; r0 += 256)
-                    h->bstep.push_back(SolveTask{s, k0, kb, r0, slot++, nchunk, first, 0});
+                if (t >= 1 && t - 1 <= T - 1) {
+                    const int j = t - 1, k0 = 64 * j, kb = std::min(64, d.c - k0);
+                    const auto rc = chunk_rows(d, j);
+                    qfirst[s] = (int)((j % 2) * bank + idx);
+                    qcount[s] = rc.second;
+                    for (int q = 0; q < rc.second; ++q)
+                        h->bstep.push_back(SolveTask{s, k0, kb, rc.first + 256 * q, (int)((j % 2) * bank + idx++), 0, 0, 0});
+                }
             }
-            h->part_slots = std::max<int64_t>(h->part_slots, std::max(slot, 1));
             push(h->bwd_seq, kBS, f0, (int64_t)h->bstep.size());
         }
     }
+    h->part_slots = 2 * bank;
 }
 
 void build_plan(mfd_handle* h)
@@ -496,8 +518,6 @@ int setup_device(mfd_handle* h)
     MFD_TRY(upload(h, &h->d_child_list, h->child_list));
     MFD_TRY(upload(h, &h->d_fstep, h->fstep));
     MFD_TRY(upload(h, &h->d_bstep, h->bstep));
-    MFD_TRY(dalloc(h, &h->d_bcount, std::max<int64_t>(h->part_slots, 1)));
-    HIP_TRY(hipMemset(h->d_bcount, 0, sizeof(unsigned) * std::max<int64_t>(h->part_slots, 1)));
     // assembly map: A entry -> front position (coefficient_setter_m.f90:26-41)
     const int64_t nnz = (int64_t)S.snidx.size();
     std::vector<int64_t> dst(nnz), src(nnz);
@@ -701,8 +721,7 @@ template <typename T> void enqueue_solve(mfd_handle* h, const double* bp, double
         switch (l.kind) {
         case kBG: launch_bwd_gather(h->d_z, nrhs, h->d_nodes, h->d_lidx, h->d_level_nodes + l.first, l.count, st); break;
         case kBS:
-            launch_bwd_step<T>(pool, h->d_z, h->d_part, h->d_bcount, xp, n, nrhs, h->d_nodes, h->d_bstep + l.first, l.count,
-                               st);
+            launch_bwd_step<T>(pool, h->d_z, h->d_part, xp, n, nrhs, h->d_nodes, h->d_bstep + l.first, l.count, st);
             break;
         default: break;
         }
