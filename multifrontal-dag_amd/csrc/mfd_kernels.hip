@@ -558,12 +558,15 @@ __device__ void load_inverse(const T* inv, double (*LI)[65])
     for (int e = threadIdx.x; e < 64 * 64; e += kThreads) LI[e & 63][e >> 6] = (double)inv[e];
 }
 
-// Forward block step: y_t = inv(L_tt) Z[k0:k0+kb] (every workgroup computes
-// it from the unmodified Z block); the writer task (row0 < 0) stores y_t in
-// Y, each chunk task subtracts L[rows, t] y_t from its 256 rows of Z
-// (DTRSV "U","T" and DGEMV "T", solve_kernel_m.f90:4-23).  Every global
-// read of the step (the inverse, the chunk's 64 columns of L, Z) is issued
-// before the first wait.
+// Forward sweep, one launch per block step t (DTRSV "U","T" and DGEMV "T",
+// solve_kernel_m.f90:4-23), pipelined one block behind:
+//   - the block task (row0 < 0) solves block t: z = Z_t - L[B_t, B_{t-1}]
+//     y_{t-1} (pad = order of block t-1; 0 for the first block), y_t =
+//     inv(L_tt) z, written to Y_t;
+//   - a chunk task (row0 >= 0) applies block k0/64 (= t - 1) to its 256
+//     rows, all after block t: Z[rows] -= L[rows, B_{t-1}] y_{t-1}.
+// No task of a launch depends on another task of the same launch, and every
+// global read of a task is issued before its first wait.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_fwd_step(const T* pool, double* zpool, int nrhs, const NodeDev* nodes,
                                                        const SolveTask* tasks)
@@ -571,61 +574,84 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(const T* pool, double* zp
     const SolveTask t = tasks[blockIdx.x];
     const NodeDev nd = nodes[t.node];
     __shared__ double LI[64][65];
+    __shared__ double Lp[64][65];  // Lp[k][i] = L[k0 + i][k0 - 64 + k], block task
     __shared__ double zt[64][kRhsGroup], yt[64][kRhsGroup];
     const int n = nd.n, c = nd.c, kb = t.kb, tid = threadIdx.x, lane = tid & 63, ty = tid >> 6;
     double* Z = zpool + nd.z * nrhs;
     double* Y = Z + (int64_t)n * nrhs;
-    const T* inv = pool + nd.inv + (int64_t)(t.k0 / 64) * 4096;
-    const int row = t.row0 + tid;
-    const bool chunk = t.row0 >= 0 && row < n;
     const T* Lb = pool + nd.front + (int64_t)t.k0 * n;
-    double li[16];
+    if (t.row0 >= 0) {
+        const int row = t.row0 + tid;
+        const bool live = row < n;
+        double l[64];
 #pragma unroll
-    for (int p = 0; p < 16; ++p) li[p] = (double)inv[tid + kThreads * p];
-    double l[64];
+        for (int k = 0; k < 64; ++k) l[k] = (live && k < kb) ? (double)ld_off(Lb, (unsigned)(row + k * n)) : 0.0;
+        for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
+            const int nr = min(kRhsGroup, nrhs - r0);
+            double zr[kRhsGroup];
 #pragma unroll
-    for (int k = 0; k < 64; ++k) l[k] = (chunk && k < kb) ? (double)ld_off(Lb, (unsigned)(row + k * n)) : 0.0;
+            for (int r = 0; r < kRhsGroup; ++r) zr[r] = (live && r < nr) ? Z[row + (int64_t)(r0 + r) * n] : 0.0;
+            for (int e = tid; e < 64 * kRhsGroup; e += kThreads) {
+                const int k = e & 63, r = e >> 6;
+                yt[k][r] = (k < kb && r < nr) ? Y[t.k0 + k + (int64_t)(r0 + r) * c] : 0.0;
+            }
+            __syncthreads();
+            if (live) {
+#pragma unroll 1
+                for (int r = 0; r < nr; ++r) {
+                    double acc0 = 0.0, acc1 = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 64; k += 2) {
+                        acc0 += l[k] * yt[k][r];
+                        acc1 += l[k + 1] * yt[k + 1][r];
+                    }
+                    double z = 0.0;
+#pragma unroll
+                    for (int q = 0; q < kRhsGroup; ++q)
+                        if (q == r) z = zr[q];
+                    Z[row + (int64_t)(r0 + r) * n] = z - (acc0 + acc1);
+                }
+            }
+            __syncthreads();
+        }
+        return;
+    }
+    // block task
+    const int kp = t.pad;
+    const T* inv = pool + nd.inv + (int64_t)(t.k0 / 64) * 4096;
+    // block (B_t, B_{t-1}): rows k0.., columns k0 - 64.. (only read when kp > 0)
+    const T* Lq = kp > 0 ? pool + nd.front + t.k0 + (int64_t)(t.k0 - 64) * n : Lb;
+    double li[16], lp[16];
+#pragma unroll
+    for (int p = 0; p < 16; ++p) {
+        const int e = tid + kThreads * p, i = e & 63, k = e >> 6;
+        li[p] = (double)inv[e];
+        lp[p] = (i < kb && k < kp) ? (double)ld_off(Lq, (unsigned)(i + k * n)) : 0.0;
+    }
 #pragma unroll
     for (int p = 0; p < 16; ++p) {
         const int e = tid + kThreads * p;
         LI[e & 63][e >> 6] = li[p];
+        Lp[e >> 6][e & 63] = lp[p];
     }
     for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
         const int nr = min(kRhsGroup, nrhs - r0);
-        double zr[kRhsGroup];
-#pragma unroll
-        for (int r = 0; r < kRhsGroup; ++r) zr[r] = (chunk && r < nr) ? Z[row + (int64_t)(r0 + r) * n] : 0.0;
         for (int e = tid; e < 64 * kRhsGroup; e += kThreads) {
-            int k = e & 63, r = e >> 6;
+            const int k = e & 63, r = e >> 6;
             zt[k][r] = (k < kb && r < nr) ? Z[t.k0 + k + (int64_t)(r0 + r) * n] : 0.0;
+            yt[k][r] = (k < kp && r < nr) ? Y[t.k0 - 64 + k + (int64_t)(r0 + r) * c] : 0.0;
         }
         __syncthreads();
         for (int r = ty; r < kRhsGroup; r += 4) {
             double acc = 0.0;
-            for (int k = 0; k < kb; ++k) acc += LI[k][lane] * zt[k][r];
-            yt[lane][r] = acc;
+            for (int k = 0; k < kp; ++k) acc += Lp[k][lane] * yt[k][r];
+            zt[lane][r] -= acc;
         }
         __syncthreads();
-        if (t.row0 < 0) {
-            for (int e = tid; e < kb * nr; e += kThreads) {
-                int i = e % kb, r = e / kb;
-                Y[t.k0 + i + (int64_t)(r0 + r) * c] = yt[i][r];
-            }
-        } else if (chunk) {
-#pragma unroll 1
-            for (int r = 0; r < nr; ++r) {
-                double acc0 = 0.0, acc1 = 0.0;
-#pragma unroll
-                for (int k = 0; k < 64; k += 2) {
-                    acc0 += l[k] * yt[k][r];
-                    acc1 += l[k + 1] * yt[k + 1][r];
-                }
-                double z = 0.0;
-#pragma unroll
-                for (int q = 0; q < kRhsGroup; ++q)
-                    if (q == r) z = zr[q];
-                Z[row + (int64_t)(r0 + r) * n] = z - (acc0 + acc1);
-            }
+        for (int r = ty; r < nr; r += 4) {
+            double acc = 0.0;
+            for (int k = 0; k < kb; ++k) acc += LI[k][lane] * zt[k][r];
+            if (lane < kb) Y[t.k0 + lane + (int64_t)(r0 + r) * c] = acc;
         }
         __syncthreads();
     }

@@ -203,8 +203,10 @@ int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 // ------------------------------------------------------------- schedule
 enum SolveKind { kFG = 100, kFS, kBG, kBS };
 
-// Solve schedule: per level, a gather launch, then for every 64-column pivot
-// block one forward launch (the block's writer task plus its row chunks).
+// Solve schedule.  Forward, per level: a gather launch, then launches
+// t = 0 .. steps: the block task of block t (which first applies block t - 1
+// to block t) and the row chunks applying block t - 1 to the rows after
+// block t.
 // Backward, per level: a gather launch, then launches t = steps .. 0: the
 // block task of block t (for fronts that have it) and the row chunks forming
 // the partial sums block t - 1 needs from the rows after block t (launch
@@ -230,16 +232,22 @@ void build_solve_plan(mfd_handle* h)
         const int b0 = h->level_ptr[L], b1 = h->level_ptr[L + 1];
         push(h->fwd_seq, kFG, b0, b1);
         const int steps = steps_of(L);
-        for (int t = 0; t < steps; ++t) {
-            const int k0 = 64 * t;
+        for (int t = 0; t <= steps; ++t) {
             int64_t f0 = (int64_t)h->fstep.size();
             for (int k = b0; k < b1; ++k) {
                 const int s = h->level_nodes[k];
                 const NodeDev& d = h->nodes[s];
-                if (d.c <= k0) continue;
-                const int kb = std::min(64, d.c - k0);
-                h->fstep.push_back(SolveTask{s, k0, kb, -1, 0, 0, 0, 0});
-                for (int r0 = k0 + kb; r0 < d.n; r0 += 256) h->fstep.push_back(SolveTask{s, k0, kb, r0, 0, 0, 0, 0});
+                const int T = (d.c + 63) / 64;
+                if (t <= T - 1) {
+                    const int k0 = 64 * t, kb = std::min(64, d.c - k0);
+                    h->fstep.push_back(SolveTask{s, k0, kb, -1, 0, 0, 0, t > 0 ? 64 : 0});
+                }
+                if (t >= 1 && t - 1 <= T - 1) {
+                    // block t - 1 applied to the rows after block t
+                    const int k0 = 64 * (t - 1), kb = std::min(64, d.c - k0);
+                    for (int r0 = std::min(64 * (t + 1), d.c); r0 < d.n; r0 += 256)
+                        h->fstep.push_back(SolveTask{s, k0, kb, r0, 0, 0, 0, 0});
+                }
             }
             push(h->fwd_seq, kFS, f0, (int64_t)h->fstep.size());
         }
