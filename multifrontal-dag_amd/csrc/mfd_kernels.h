@@ -23,12 +23,14 @@ void launch_diag(T* pool, const NodeDev* nodes, const DiagTask* tasks, int count
 // instances, same code)
 template <typename T> void launch_gemm(T* pool, const GemmTask* tasks, int count, int kind, hipStream_t st);
 template <typename T> void launch_gemm128(T* pool, const GemmTask* tasks, int count, hipStream_t st);
-void launch_fwd_gather(double* zpool, const double* b, int ldb, int nrhs, const NodeDev* nodes, const int* lidx,
-                       const int* child_ptr, const int* child_list, const int* tasks, int count, hipStream_t st);
+void launch_fwd_init(double* zpool, const double* b, int ldb, int nrhs, const NodeDev* nodes, const int2* tasks,
+                     int count, hipStream_t st);
+void launch_fwd_add(double* zpool, int nrhs, const NodeDev* nodes, const int* lidx, const int2* tasks, int count,
+                    hipStream_t st);
 template <typename T>
 void launch_fwd_step(const T* pool, double* zpool, int nrhs, const NodeDev* nodes, const SolveTask* tasks, int count,
                      hipStream_t st);
-void launch_bwd_gather(double* zpool, int nrhs, const NodeDev* nodes, const int* lidx, const int* tasks, int count,
+void launch_bwd_gather(double* zpool, int nrhs, const NodeDev* nodes, const int* lidx, const int2* tasks, int count,
                        hipStream_t st);
 template <typename T>
 void launch_bwd_step(const T* pool, double* zpool, double* part, double* x, int ldx, int nrhs, const NodeDev* nodes,

@@ -522,31 +522,36 @@ __global__ __launch_bounds__(kThreads) void k_diag(T* pool, const NodeDev* nodes
 // block, backward two (seq_forward_m.f90:14-81, seq_backward_m.f90:14-131).
 constexpr int kRhsGroup = 8;  // right-hand sides processed together
 
-// Forward, first launch of a level: pivot part of Z from b, update part
-// zero, then the children's update parts added in child order (scatter_add,
-// sparse_add_subroutines_m.f90:13-30).
-__global__ __launch_bounds__(kThreads) void k_fwd_gather(double* zpool, const double* b, int ldb, int nrhs,
-                                                         const NodeDev* nodes, const int* lidx, const int* child_ptr,
-                                                         const int* child_list, const int* tasks)
+// Forward, first launch of a level, task (front, first row): 256 rows of
+// the front's Z, pivot part from b, update part zero.
+__global__ __launch_bounds__(kThreads) void k_fwd_init(double* zpool, const double* b, int ldb, int nrhs,
+                                                       const NodeDev* nodes, const int2* tasks)
 {
-    const int s = tasks[blockIdx.x];
-    const NodeDev nd = nodes[s];
-    const int n = nd.n, c = nd.c, tid = threadIdx.x;
+    const int2 t = tasks[blockIdx.x];
+    const NodeDev nd = nodes[t.x];
+    const int i = t.y + threadIdx.x;
+    if (i >= nd.n) return;
     double* Z = zpool + nd.z * nrhs;
-    for (int e = tid; e < n * nrhs; e += kThreads) {
-        int i = e % n, r = e / n;
-        Z[i + (int64_t)r * n] = i < c ? b[(int64_t)(nd.first + i) + (int64_t)r * ldb] : 0.0;
-    }
-    __syncthreads();
-    for (int cp = child_ptr[s]; cp < child_ptr[s + 1]; ++cp) {
-        const NodeDev ch = nodes[child_list[cp]];
-        const double* Zc = zpool + ch.z * nrhs;
-        for (int e = tid; e < ch.w * nrhs; e += kThreads) {
-            int i = e % ch.w, r = e / ch.w;
-            Z[(lidx[ch.lidx + i] - 1) + (int64_t)r * n] += Zc[(ch.c + i) + (int64_t)r * ch.n];
-        }
-        __syncthreads();
-    }
+    for (int r = 0; r < nrhs; ++r)
+        Z[i + (int64_t)r * nd.n] = i < nd.c ? b[(int64_t)(nd.first + i) + (int64_t)r * ldb] : 0.0;
+}
+
+// Forward, one launch per child rank: task (child, first row) adds 256 rows
+// of the child's update part into its parent's Z (scatter_add,
+// sparse_add_subroutines_m.f90:13-30).  A launch holds at most one child of
+// each parent, and the ranks run in child order.
+__global__ __launch_bounds__(kThreads) void k_fwd_add(double* zpool, int nrhs, const NodeDev* nodes, const int* lidx,
+                                                      const int2* tasks)
+{
+    const int2 t = tasks[blockIdx.x];
+    const NodeDev ch = nodes[t.x];
+    const int i = t.y + threadIdx.x;
+    if (i >= ch.w) return;
+    const NodeDev pa = nodes[ch.parent];
+    double* Zp = zpool + pa.z * nrhs;
+    const double* Zc = zpool + ch.z * nrhs;
+    const int p = lidx[ch.lidx + i] - 1;
+    for (int r = 0; r < nrhs; ++r) Zp[p + (int64_t)r * pa.n] += Zc[(ch.c + i) + (int64_t)r * ch.n];
 }
 
 // Stage inv(L_tt) in LDS as LI[col][row] = L^{-1}[row][col] (the pool holds
@@ -657,24 +662,24 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(const T* pool, double* zp
     }
 }
 
-// Backward, first launch of a level: the update part of Z takes the
-// parent's solution at the parent-front positions -- pivot positions from
-// the parent's Y, update positions from the parent's Z (gather,
-// sparse_add_subroutines_m.f90:32-49).
+// Backward, first launch of a level, task (front, first row): 256 rows of
+// the update part of Z take the parent's solution at the parent-front
+// positions -- pivot positions from the parent's Y, update positions from
+// the parent's Z (gather, sparse_add_subroutines_m.f90:32-49).
 __global__ __launch_bounds__(kThreads) void k_bwd_gather(double* zpool, int nrhs, const NodeDev* nodes, const int* lidx,
-                                                         const int* tasks)
+                                                         const int2* tasks)
 {
-    const NodeDev nd = nodes[tasks[blockIdx.x]];
-    if (nd.parent < 0) return;
+    const int2 t = tasks[blockIdx.x];
+    const NodeDev nd = nodes[t.x];
+    const int i = t.y + threadIdx.x;
+    if (nd.parent < 0 || i >= nd.w) return;
     const NodeDev pa = nodes[nd.parent];
     double* Z = zpool + nd.z * nrhs;
     const double* Zp = zpool + pa.z * nrhs;
     const double* Yp = Zp + (int64_t)pa.n * nrhs;
-    for (int e = threadIdx.x; e < nd.w * nrhs; e += kThreads) {
-        int i = e % nd.w, r = e / nd.w;
-        int p = lidx[nd.lidx + i] - 1;
+    const int p = lidx[nd.lidx + i] - 1;
+    for (int r = 0; r < nrhs; ++r)
         Z[(nd.c + i) + (int64_t)r * nd.n] = p < pa.c ? Yp[p + (int64_t)r * pa.c] : Zp[p + (int64_t)r * pa.n];
-    }
 }
 
 // Backward sweep, one launch per block step t (DGEMV "N" + DTRSV "U","N",
@@ -958,12 +963,15 @@ template <typename T> void launch_gemm128(T* pool, const GemmTask* tasks, int co
 {
     if (count > 0) hipLaunchKernelGGL(k_gemm128<T>, dim3(count), dim3(kThreads), 0, st, pool, tasks);
 }
-void launch_fwd_gather(double* zpool, const double* b, int ldb, int nrhs, const NodeDev* nodes, const int* lidx,
-                       const int* child_ptr, const int* child_list, const int* tasks, int count, hipStream_t st)
+void launch_fwd_init(double* zpool, const double* b, int ldb, int nrhs, const NodeDev* nodes, const int2* tasks,
+                     int count, hipStream_t st)
 {
-    if (count > 0)
-        hipLaunchKernelGGL(k_fwd_gather, dim3(count), dim3(kThreads), 0, st, zpool, b, ldb, nrhs, nodes, lidx, child_ptr,
-                           child_list, tasks);
+    if (count > 0) hipLaunchKernelGGL(k_fwd_init, dim3(count), dim3(kThreads), 0, st, zpool, b, ldb, nrhs, nodes, tasks);
+}
+void launch_fwd_add(double* zpool, int nrhs, const NodeDev* nodes, const int* lidx, const int2* tasks, int count,
+                    hipStream_t st)
+{
+    if (count > 0) hipLaunchKernelGGL(k_fwd_add, dim3(count), dim3(kThreads), 0, st, zpool, nrhs, nodes, lidx, tasks);
 }
 template <typename T>
 void launch_fwd_step(const T* pool, double* zpool, int nrhs, const NodeDev* nodes, const SolveTask* tasks, int count,
@@ -971,7 +979,7 @@ void launch_fwd_step(const T* pool, double* zpool, int nrhs, const NodeDev* node
 {
     if (count > 0) hipLaunchKernelGGL(k_fwd_step<T>, dim3(count), dim3(kThreads), 0, st, pool, zpool, nrhs, nodes, tasks);
 }
-void launch_bwd_gather(double* zpool, int nrhs, const NodeDev* nodes, const int* lidx, const int* tasks, int count,
+void launch_bwd_gather(double* zpool, int nrhs, const NodeDev* nodes, const int* lidx, const int2* tasks, int count,
                        hipStream_t st)
 {
     if (count > 0) hipLaunchKernelGGL(k_bwd_gather, dim3(count), dim3(kThreads), 0, st, zpool, nrhs, nodes, lidx, tasks);

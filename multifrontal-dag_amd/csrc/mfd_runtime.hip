@@ -84,7 +84,9 @@ struct mfd_handle {
     std::vector<GemmTask> gemm;
     std::vector<Launch> fseq;
     std::vector<int> level_ptr, level_nodes, child_ptr, child_list;
-    // solve schedule: kinds kFG/kFS (forward) and kBG/kBP/kBD (backward)
+    // solve schedule: kinds kFI/kFA/kFS (forward) and kBG/kBS (backward)
+    std::vector<int2> fi, fa, bg;  // (front, first row) tasks of the gathers
+    int2 *d_fi = nullptr, *d_fa = nullptr, *d_bg = nullptr;
     std::vector<SolveTask> fstep, bstep;
     std::vector<Launch> fwd_seq, bwd_seq;
     int64_t part_slots = 0;
@@ -183,6 +185,7 @@ void free_device(mfd_handle* h)
     h->d_gemm = nullptr;
     h->d_level_nodes = h->d_child_ptr = h->d_child_list = nullptr;
     h->d_fstep = h->d_bstep = nullptr;
+    h->d_fi = h->d_fa = h->d_bg = nullptr;
     h->d_part = nullptr;
     h->d_dst = h->d_src = nullptr;
     h->d_perm = nullptr;
@@ -201,7 +204,7 @@ void free_device(mfd_handle* h)
 int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 // ------------------------------------------------------------- schedule
-enum SolveKind { kFG = 100, kFS, kBG, kBS };
+enum SolveKind { kFI = 100, kFA, kFS, kBG, kBS };
 
 // Solve schedule.  Forward, per level: a gather launch, then launches
 // t = 0 .. steps: the block task of block t (which first applies block t - 1
@@ -216,6 +219,9 @@ void build_solve_plan(mfd_handle* h)
 {
     h->fstep.clear();
     h->bstep.clear();
+    h->fi.clear();
+    h->fa.clear();
+    h->bg.clear();
     h->fwd_seq.clear();
     h->bwd_seq.clear();
     h->part_slots = 0;
@@ -230,7 +236,30 @@ void build_solve_plan(mfd_handle* h)
     };
     for (int L = 0; L < h->num_levels; ++L) {
         const int b0 = h->level_ptr[L], b1 = h->level_ptr[L + 1];
-        push(h->fwd_seq, kFG, b0, b1);
+        {
+            int64_t f = (int64_t)h->fi.size();
+            for (int k = b0; k < b1; ++k) {
+                const int s = h->level_nodes[k];
+                for (int r0 = 0; r0 < h->nodes[s].n; r0 += 256) h->fi.push_back(make_int2(s, r0));
+            }
+            push(h->fwd_seq, kFI, f, (int64_t)h->fi.size());
+            // children's update parts, one launch per child rank
+            size_t maxch = 0;
+            for (int k = b0; k < b1; ++k) {
+                const int s = h->level_nodes[k];
+                maxch = std::max(maxch, (size_t)(h->child_ptr[s + 1] - h->child_ptr[s]));
+            }
+            for (size_t r = 0; r < maxch; ++r) {
+                f = (int64_t)h->fa.size();
+                for (int k = b0; k < b1; ++k) {
+                    const int s = h->level_nodes[k];
+                    if ((size_t)(h->child_ptr[s + 1] - h->child_ptr[s]) <= r) continue;
+                    const int ch = h->child_list[h->child_ptr[s] + r];
+                    for (int r0 = 0; r0 < h->nodes[ch].w; r0 += 256) h->fa.push_back(make_int2(ch, r0));
+                }
+                push(h->fwd_seq, kFA, f, (int64_t)h->fa.size());
+            }
+        }
         const int steps = steps_of(L);
         for (int t = 0; t <= steps; ++t) {
             int64_t f0 = (int64_t)h->fstep.size();
@@ -273,7 +302,15 @@ void build_solve_plan(mfd_handle* h)
     std::vector<int> qfirst(h->nodes.size(), 0), qcount(h->nodes.size(), 0);
     for (int L = h->num_levels - 1; L >= 0; --L) {
         const int b0 = h->level_ptr[L], b1 = h->level_ptr[L + 1];
-        push(h->bwd_seq, kBG, b0, b1);
+        {
+            int64_t f = (int64_t)h->bg.size();
+            for (int k = b0; k < b1; ++k) {
+                const int s = h->level_nodes[k];
+                if (h->nodes[s].parent < 0) continue;
+                for (int r0 = 0; r0 < h->nodes[s].w; r0 += 256) h->bg.push_back(make_int2(s, r0));
+            }
+            push(h->bwd_seq, kBG, f, (int64_t)h->bg.size());
+        }
         for (int t = steps_of(L); t >= 0; --t) {
             int64_t f0 = (int64_t)h->bstep.size();
             int idx = 0;
@@ -526,6 +563,9 @@ int setup_device(mfd_handle* h)
     MFD_TRY(upload(h, &h->d_child_list, h->child_list));
     MFD_TRY(upload(h, &h->d_fstep, h->fstep));
     MFD_TRY(upload(h, &h->d_bstep, h->bstep));
+    MFD_TRY(upload(h, &h->d_fi, h->fi));
+    MFD_TRY(upload(h, &h->d_fa, h->fa));
+    MFD_TRY(upload(h, &h->d_bg, h->bg));
     // assembly map: A entry -> front position (coefficient_setter_m.f90:26-41)
     const int64_t nnz = (int64_t)S.snidx.size();
     std::vector<int64_t> dst(nnz), src(nnz);
@@ -716,10 +756,8 @@ template <typename T> void enqueue_solve(mfd_handle* h, const double* bp, double
     if (profile) (void)hipEventRecord(pe[0], st);
     for (const Launch& l : h->fwd_seq) {
         switch (l.kind) {
-        case kFG:
-            launch_fwd_gather(h->d_z, bp, n, nrhs, h->d_nodes, h->d_lidx, h->d_child_ptr, h->d_child_list,
-                              h->d_level_nodes + l.first, l.count, st);
-            break;
+        case kFI: launch_fwd_init(h->d_z, bp, n, nrhs, h->d_nodes, h->d_fi + l.first, l.count, st); break;
+        case kFA: launch_fwd_add(h->d_z, nrhs, h->d_nodes, h->d_lidx, h->d_fa + l.first, l.count, st); break;
         case kFS: launch_fwd_step<T>(pool, h->d_z, nrhs, h->d_nodes, h->d_fstep + l.first, l.count, st); break;
         default: break;
         }
@@ -727,7 +765,7 @@ template <typename T> void enqueue_solve(mfd_handle* h, const double* bp, double
     if (profile) (void)hipEventRecord(pe[1], st);
     for (const Launch& l : h->bwd_seq) {
         switch (l.kind) {
-        case kBG: launch_bwd_gather(h->d_z, nrhs, h->d_nodes, h->d_lidx, h->d_level_nodes + l.first, l.count, st); break;
+        case kBG: launch_bwd_gather(h->d_z, nrhs, h->d_nodes, h->d_lidx, h->d_bg + l.first, l.count, st); break;
         case kBS:
             launch_bwd_step<T>(pool, h->d_z, h->d_part, xp, n, nrhs, h->d_nodes, h->d_bstep + l.first, l.count, st);
             break;
