@@ -162,6 +162,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--fp64-only", action="store_true", help="skip the FP32+IR leg and the profiled pass")
+    ap.add_argument("--fp32-only", action="store_true", help="time only the FP32+IR leg (for profiling it)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device twice (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
@@ -198,6 +199,12 @@ def main():
     d_x = torch.empty_like(d_b)
     torch.cuda.synchronize()
 
+    if args.fp32_only:
+        wall32, f32, s32 = timed_steps(s, d_a, d_b, d_x, mfdag.FP32_IR, args.steps, args.warmup)
+        st32 = s.stats()
+        print(json.dumps({"fp32ir_ms_per_step": wall32 * 1e3, "factor_ms": f32, "solve_ms": s32,
+                          "refine_iters": st32["refine_iters"], "eta": st32["backward_error"]}))
+        return
     wall64, f64, s64 = timed_steps(s, d_a, d_b, d_x, mfdag.FP64, args.steps, args.warmup)
     x = d_x.cpu().numpy().T
     eta64 = P.backward_error(p, x, b)

@@ -25,7 +25,6 @@ namespace mfd {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kDLd = kPanel + 1;  // row length of the LDS copy D of a diagonal block
 
 // ------------------------------------------------------------ MFMA traits
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -588,9 +587,15 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(const T* pool, double* zp
     if (t.row0 >= 0) {
         const int row = t.row0 + tid;
         const bool live = row < n;
-        double l[64];
+        // Unconditional loads from clamped (in-bounds) positions, masked
+        // afterwards, kept in the raw element type: a conditional or
+        // converted load would wait for the previous one.
+        const int rowc = min(row, n - 1);
+        T l[64];
 #pragma unroll
-        for (int k = 0; k < 64; ++k) l[k] = (live && k < kb) ? (double)ld_off(Lb, (unsigned)(row + k * n)) : 0.0;
+        for (int k = 0; k < 64; ++k) l[k] = ld_off(Lb, (unsigned)(rowc + min(k, kb - 1) * n));
+#pragma unroll
+        for (int k = 0; k < 64; ++k) l[k] = (live && k < kb) ? l[k] : T(0);
         for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
             const int nr = min(kRhsGroup, nrhs - r0);
             double zr[kRhsGroup];
@@ -626,12 +631,17 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(const T* pool, double* zp
     const T* inv = pool + nd.inv + (int64_t)(t.k0 / 64) * 4096;
     // block (B_t, B_{t-1}): rows k0.., columns k0 - 64.. (only read when kp > 0)
     const T* Lq = kp > 0 ? pool + nd.front + t.k0 + (int64_t)(t.k0 - 64) * n : Lb;
-    double li[16], lp[16];
+    T li[16], lp[16];
 #pragma unroll
     for (int p = 0; p < 16; ++p) {
         const int e = tid + kThreads * p, i = e & 63, k = e >> 6;
-        li[p] = (double)inv[e];
-        lp[p] = (i < kb && k < kp) ? (double)ld_off(Lq, (unsigned)(i + k * n)) : 0.0;
+        li[p] = inv[e];
+        lp[p] = ld_off(Lq, (unsigned)(min(i, kb - 1) + min(k, max(kp, 1) - 1) * n));
+    }
+#pragma unroll
+    for (int p = 0; p < 16; ++p) {
+        const int e = tid + kThreads * p, i = e & 63, k = e >> 6;
+        lp[p] = (i < kb && k < kp) ? lp[p] : T(0);
     }
 #pragma unroll
     for (int p = 0; p < 16; ++p) {
@@ -719,9 +729,15 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zp
         typedef typename Mfma<double>::acc_t acc_t;
         const int row = t.row0 + tid;
         const bool live = row < n;
-        double l[64];
+        // Unconditional loads from clamped (in-bounds) positions, masked
+        // afterwards, kept in the raw element type: a conditional or
+        // converted load would wait for the previous one.
+        const int rowc = min(row, n - 1);
+        T l[64];
 #pragma unroll
-        for (int k = 0; k < 64; ++k) l[k] = (live && k < kb) ? (double)ld_off(Lb, (unsigned)(row + k * n)) : 0.0;
+        for (int k = 0; k < 64; ++k) l[k] = ld_off(Lb, (unsigned)(rowc + min(k, kb - 1) * n));
+#pragma unroll
+        for (int k = 0; k < 64; ++k) l[k] = (live && k < kb) ? l[k] : T(0);
         for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
             const int nr = min(kRhsGroup, nrhs - r0);
 #pragma unroll
@@ -757,12 +773,18 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zp
     // block task: every global read first
     const int kn = t.pad;  // rows of block t+1 (0: none)
     const T* inv = pool + nd.inv + (int64_t)(t.k0 / 64) * 4096;
-    double li[16], ln[16];
+    T li[16], ln[16];
 #pragma unroll
     for (int p = 0; p < 16; ++p) {
         const int e = tid + kThreads * p, i = e & 63, k = e >> 6;
-        li[p] = (double)inv[e];
-        ln[p] = (i < kn && k < kb) ? (double)ld_off(Lb, (unsigned)(t.k0 + 64 + i + k * n)) : 0.0;
+        li[p] = inv[e];
+        // (row k0 itself when there is no next block: always inside the front)
+        ln[p] = ld_off(Lb, (unsigned)(t.k0 + (kn > 0 ? 64 + min(i, kn - 1) : 0) + min(k, kb - 1) * n));
+    }
+#pragma unroll
+    for (int p = 0; p < 16; ++p) {
+        const int e = tid + kThreads * p, i = e & 63, k = e >> 6;
+        ln[p] = (i < kn && k < kb) ? ln[p] : T(0);
     }
 #pragma unroll
     for (int p = 0; p < 16; ++p) {
