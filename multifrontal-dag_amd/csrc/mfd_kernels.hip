@@ -861,17 +861,32 @@ __global__ __launch_bounds__(kThreads) void k_scatter_rows(double* out, int ld_o
 // r = b - A x for the full symmetric matrix in CSR (postordered numbering);
 // entry e's value is a[src[e]] (values stay in the caller's order).  One
 // wave per row, so the row sum is reduced in a fixed order.
+// One thread per row (rows of a sparse matrix are short).
 __global__ __launch_bounds__(kThreads) void k_residual(double* r, const double* b, const double* x, int n, int nrhs,
                                                        const int* ptr, const int* col, const int64_t* src,
                                                        const double* a)
 {
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int row = blockIdx.x * kThreads + threadIdx.x;
     if (row >= n) return;
+    const int e0 = ptr[row], e1 = ptr[row + 1];
     for (int k = 0; k < nrhs; ++k) {
         double acc = 0.0;
-        for (int e = ptr[row] + lane; e < ptr[row + 1]; e += 64) acc += a[src[e]] * x[col[e] + (int64_t)k * n];
-        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-        if (lane == 0) r[row + (int64_t)k * n] = b[row + (int64_t)k * n] - acc;
+        for (int e = e0; e < e1; ++e) acc += a[src[e]] * x[col[e] + (int64_t)k * n];
+        r[row + (int64_t)k * n] = b[row + (int64_t)k * n] - acc;
+    }
+}
+
+// Maximum over the workgroup, then one atomic per workgroup (as IEEE bits:
+// order-preserving for non-negative doubles, so the result is exact).
+__device__ __forceinline__ void block_max_to(unsigned long long* out, double m)
+{
+    __shared__ double wmax[kThreads / 64];
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kThreads / 64; ++w) m = fmax(m, wmax[w]);
+        atomicMax(out, (unsigned long long)__double_as_longlong(m));
     }
 }
 
@@ -884,20 +899,20 @@ __global__ __launch_bounds__(kThreads) void k_absmax(unsigned long long* out, co
     double m = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
         m = fmax(m, fabs(x[i + (int64_t)k * ld]));
-    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(out + k, (unsigned long long)__double_as_longlong(m));
+    block_max_to(out + k, m);
 }
 
 // out[0] = max over rows of sum_e |a[src[e]]|  (the infinity norm of A)
 __global__ __launch_bounds__(kThreads) void k_norm_inf(unsigned long long* out, int n, const int* ptr,
                                                        const int64_t* src, const double* a)
 {
-    const int row = blockIdx.x * kThreads + threadIdx.x;
-    double s = 0.0;
-    if (row < n)
+    double m = 0.0;
+    for (int row = blockIdx.x * kThreads + threadIdx.x; row < n; row += gridDim.x * kThreads) {
+        double s = 0.0;
         for (int e = ptr[row]; e < ptr[row + 1]; ++e) s += fabs(a[src[e]]);
-    for (int off = 32; off > 0; off >>= 1) s = fmax(s, __shfl_xor(s, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(s));
+        m = fmax(m, s);
+    }
+    block_max_to(out, m);
 }
 
 __global__ __launch_bounds__(kThreads) void k_axpy(double* x, const double* d, int64_t n)
@@ -915,16 +930,18 @@ void launch_residual(double* r, const double* b, const double* x, int n, int nrh
                      const int64_t* src, const double* a, hipStream_t st)
 {
     if (n > 0)
-        hipLaunchKernelGGL(k_residual, dim3((n + 3) / 4), dim3(kThreads), 0, st, r, b, x, n, nrhs, ptr, col, src, a);
+        hipLaunchKernelGGL(k_residual, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, st, r, b, x, n, nrhs, ptr,
+                           col, src, a);
 }
 void launch_absmax(unsigned long long* out, const double* x, int64_t n, int ld, int nrhs, hipStream_t st)
 {
-    int g = (int)std::min<int64_t>((n + kThreads - 1) / kThreads, 1024);
+    int g = (int)std::min<int64_t>((n + kThreads - 1) / kThreads, 256);
     if (n > 0) hipLaunchKernelGGL(k_absmax, dim3(std::max(g, 1), nrhs), dim3(kThreads), 0, st, out, x, n, ld, nrhs);
 }
 void launch_norm_inf(unsigned long long* out, int n, const int* ptr, const int64_t* src, const double* a, hipStream_t st)
 {
-    if (n > 0) hipLaunchKernelGGL(k_norm_inf, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, st, out, n, ptr, src, a);
+    const int g = std::min((n + kThreads - 1) / kThreads, 256);
+    if (n > 0) hipLaunchKernelGGL(k_norm_inf, dim3(g), dim3(kThreads), 0, st, out, n, ptr, src, a);
 }
 void launch_axpy(double* x, const double* d, int64_t n, hipStream_t st)
 {
