@@ -184,6 +184,47 @@ def stencil27_3d(nx, seed=0, lo=1e-1, hi=1e1, shift=1e-4, leaf=64):
     return Problem(n, cc, rr, vv, nested_dissection((nx, nx, nx), leaf), f"st27_{nx}")
 
 
+def block_stencil_3dof(nx, seed=0, leaf=64):
+    """3 unknowns per node of an nx^3 grid, 27-point block stencil: for every
+    neighbour pair (a, b), A_ab = -w (I + 0.3 R R^T) with R ~ N(0,1)^{3x3} and
+    w ~ U(0.5, 2); diagonal blocks D_a = sum_b ||A_ab||_inf I + sum_b w (I +
+    0.3 R R^T) (block diagonally dominant, hence SPD).  Unknown 3 p + d is
+    dof d of node p; the nested-dissection ordering keeps a node's three
+    unknowns together."""
+    nn = nx ** 3
+    n = 3 * nn
+    rng = np.random.Generator(np.random.PCG64(seed))
+    offs = [(dx, dy, dz) for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1)
+            if (dz, dy, dx) > (0, 0, 0)]
+    edges = grid_edges((nx, nx, nx), offs)
+    a = np.concatenate([e[0] for e in edges])
+    b = np.concatenate([e[1] for e in edges])
+    ne = len(a)
+    w = rng.uniform(0.5, 2.0, size=ne)
+    R = rng.standard_normal((ne, 3, 3))
+    M = np.eye(3)[None] + 0.3 * np.einsum("eij,ekj->eik", R, R)
+    blk = -w[:, None, None] * M                      # A_ab, symmetric 3 x 3
+    rown = np.abs(blk).sum(axis=2).max(axis=1)       # ||A_ab||_inf
+    D = np.zeros((nn, 3, 3))
+    add = rown[:, None, None] * np.eye(3)[None] + w[:, None, None] * M
+    np.add.at(D, a, add)
+    np.add.at(D, b, add)
+    # lower triangle: diagonal blocks (i >= j) and the full block at (b, a)
+    di, dj = np.tril_indices(3)
+    rows = [(3 * np.arange(nn)[:, None] + di[None]).ravel()]
+    cols = [(3 * np.arange(nn)[:, None] + dj[None]).ravel()]
+    vals = [D[:, di, dj].ravel()]
+    ii, jj = np.meshgrid(np.arange(3), np.arange(3), indexing="ij")
+    lo_, hi_ = np.minimum(a, b), np.maximum(a, b)
+    rows.append((3 * hi_[:, None] + ii.ravel()[None]).ravel())
+    cols.append((3 * lo_[:, None] + jj.ravel()[None]).ravel())
+    vals.append(blk[:, ii.ravel(), jj.ravel()].ravel())
+    cc, rr, vv = lower_ccs(n, np.concatenate(rows), np.concatenate(cols), np.concatenate(vals))
+    pn = nested_dissection((nx, nx, nx), leaf) - 1
+    perm = (3 * pn[:, None] + np.arange(3)[None]).ravel() + 1
+    return Problem(n, cc, rr, vv, perm.astype(np.int32), f"blk3_{nx}")
+
+
 def rhs(problem, nrhs=1, seed=0):
     """x_true ~ U(-1, 1) (n x nrhs) and b = A x_true."""
     rng = np.random.Generator(np.random.PCG64(seed + 12345))

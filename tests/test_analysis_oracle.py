@@ -154,3 +154,19 @@ def test_supernodal_structure_is_exact(seed, max_zero):
     # every node but the root has a parent; postorder
     assert res["parent"][-1] == 0
     assert all(p == 0 or p > i + 1 for i, p in enumerate(res["parent"]))
+
+
+
+def test_block_stencil_generator_is_spd_and_ordered():
+    """The 3-unknowns-per-node block stencil: symmetric positive definite,
+    lower CCS with the diagonal first, and an ordering that keeps a node's
+    unknowns together."""
+    from mfdag import problems as P
+    p = P.block_stencil_3dof(4, seed=1)
+    A = p.dense()
+    assert np.allclose(A, A.T)
+    assert np.linalg.eigvalsh(A).min() > 0
+    assert all(p.ccs_row[p.ccs_col[j] - 1] == j + 1 for j in range(p.n))
+    perm = p.perm - 1
+    assert sorted(perm) == list(range(p.n))
+    assert np.all(perm.reshape(-1, 3) // 3 == (perm.reshape(-1, 3) // 3)[:, :1])

@@ -103,6 +103,7 @@ CASES = [
     ("wlap3d_12x10x9", lambda: P.weighted_laplacian_3d(12, 10, 9, seed=4), 32, 5),
     ("st27_7", lambda: P.stencil27_3d(7, seed=5), 8, 7),
     ("wlap3d_16", lambda: P.weighted_laplacian_3d(16, seed=6), 64, 32),
+    ("blk3_8", lambda: P.block_stencil_3dof(8, seed=7), 16, 24),
 ]
 
 
@@ -236,3 +237,49 @@ def test_dense_front_matches_lapack(n):
     perm = s.structure()["perm"] - 1
     L = np.linalg.cholesky(Ad[np.ix_(perm, perm)])
     assert np.abs(np.tril(F) - L).max() <= 1e-12 * np.abs(L).max()
+
+
+# ---------------------------------------------------------------- configs
+# The benchmark configurations at full size, checked through properties that
+# do not depend on size (the oracle is checked on the smaller cases above).
+
+def solve_checks(p, max_zero, nb, precision, nrhs=2, seed=11):
+    s = mfdag.Solver(nb=nb, graph=True)
+    s.analyze(p.ccs_col, p.ccs_row, max_zero, p.perm)
+    assert s.factorize(p.val, precision) == 0
+    xt, b = P.rhs(p, nrhs, seed=seed)
+    x = s.solve(b)
+    eta = P.backward_error(p, x, b)
+    st = s.stats()
+    return s, xt, x, eta, st
+
+
+def test_config_lap2d_100_x_true():
+    """2D 5-point Laplacian, 100 x 100 (n = 10,000), x_true ~ U(-1,1):
+    x within 1e-12 of x_true (kappa ~ 6e3) and eta <= 1e-14."""
+    p = P.laplacian_2d(100)
+    s, xt, x, eta, st = solve_checks(p, 16, 64, mfdag.FP64, nrhs=1)
+    rel = np.linalg.norm(x - xt, np.inf) / np.linalg.norm(xt, np.inf)
+    log(f"lap2d_100: x err {rel:.2e}, eta {eta:.2e}")
+    assert rel <= 1e-12 and eta <= 1e-14
+
+
+@pytest.mark.parametrize("precision", [mfdag.FP64, mfdag.FP32_IR])
+def test_config_stencil27_48(precision):
+    """48^3 27-point stencil (n = 110,592), weights log-uniform in
+    [1e-1, 1e1], shift 1e-4: eta <= 1e-14 in both forms."""
+    p = P.stencil27_3d(48, seed=2024)
+    s, xt, x, eta, st = solve_checks(p, 16384, 64, precision)
+    log(f"st27_48 precision {precision}: eta {eta:.2e} (reported {st['backward_error']:.2e}), "
+        f"iters {st['refine_iters']}")
+    assert eta <= 1e-14
+
+
+@pytest.mark.parametrize("precision", [mfdag.FP64, mfdag.FP32_IR])
+def test_config_block_stencil_3dof_40(precision):
+    """40^3 grid with 3 unknowns per node (n = 192,000), 27-point 3 x 3 block
+    stencil: eta <= 1e-14 in both forms."""
+    p = P.block_stencil_3dof(40, seed=2024)
+    s, xt, x, eta, st = solve_checks(p, 16384, 64, precision)
+    log(f"blk3_40 precision {precision}: eta {eta:.2e}, iters {st['refine_iters']}")
+    assert eta <= 1e-14
