@@ -19,6 +19,9 @@ template <typename T>
 void launch_small_front(T* pool, const NodeDev* nodes, const int* tasks, int count, int* info, hipStream_t st);
 template <typename T>
 void launch_diag(T* pool, const NodeDev* nodes, const DiagTask* tasks, int count, int* info, hipStream_t st);
+template <typename T>
+void launch_upd_diag(T* pool, const NodeDev* nodes, const DiagTask* diags, int ndiag, const GemmTask* tiles, int ntiles,
+                     int* info, hipStream_t st);
 // kind: 0 panel TRSM, 1 in-panel update, 2 Schur update (separate kernel
 // instances, same code)
 template <typename T> void launch_gemm(T* pool, const GemmTask* tasks, int count, int kind, hipStream_t st);

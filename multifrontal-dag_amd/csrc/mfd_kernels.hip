@@ -191,17 +191,13 @@ __device__ __forceinline__ void inverse16(Front64<T>& S, int i, int j, int w)
 // stores L^{-T} of the c x c factor column-major at inv (64 x 64,
 // zero-padded: inv[r + 64 q] = L^{-1}[q][r]).  A pivot that is not positive
 // records the global column col0 + k + 1 in *info (smallest wins).
+// Same, with S.A already holding the block (lower part, zero elsewhere)
+// and S.X zeroed, both visible to the workgroup.
 template <typename T>
-__device__ __forceinline__ void factor_dense64(Front64<T>& S, T* g, int64_t ld, int n, int c, int* info, int col0,
-                                               T* inv)
+__device__ __forceinline__ void factor_dense64_loaded(Front64<T>& S, T* g, int64_t ld, int n, int c, int* info,
+                                                      int col0, T* inv)
 {
     const int tid = threadIdx.x, wv = tid >> 6;
-    for (int e = tid; e < 64 * 64; e += kThreads) {
-        const int row = e & 63, col = e >> 6;
-        S.A[col][row] = (row < n && col <= row) ? g[row + (int64_t)col * ld] : T(0);
-        S.X[col][row] = T(0);
-    }
-    __syncthreads();
     const int nblk = (c + 15) >> 4, nall = (n + 15) >> 4;
     for (int b = 0; b < nblk; ++b) {
         const int b16 = 16 * b, cb = min(16, c - b16);
@@ -222,6 +218,18 @@ __device__ __forceinline__ void factor_dense64(Front64<T>& S, T* g, int64_t ld, 
         if (row < n && col <= row) g[row + (int64_t)col * ld] = S.A[col][row];
         inv[e] = (row < c && col < c) ? S.X[row][col] : T(0);
     }
+}
+template <typename T>
+__device__ __forceinline__ void factor_dense64(Front64<T>& S, T* g, int64_t ld, int n, int c, int* info, int col0,
+                                               T* inv)
+{
+    for (int e = threadIdx.x; e < 64 * 64; e += kThreads) {
+        const int row = e & 63, col = e >> 6;
+        S.A[col][row] = (row < n && col <= row) ? g[row + (int64_t)col * ld] : T(0);
+        S.X[col][row] = T(0);
+    }
+    __syncthreads();
+    factor_dense64_loaded<T>(S, g, ld, n, c, info, col0, inv);
 }
 
 }  // namespace
@@ -311,18 +319,19 @@ template <typename T> struct TileLds {
     T Bs[16][kTile + 4];
 };
 
-// Cs[col][row] = sum_kk A[row][kk] B[col][kk] over a 64 x 64 tile (rows
-// past m and columns past n give 0), FP64/FP32 MFMA 16x16x4.  Four waves
-// each own a 32 x 32 quarter (2 x 2 MFMA blocks).  K is staged through LDS in
-// chunks of 16 with the next chunk's global loads issued before the current
-// chunk's MFMAs.  Every read of A and B happens before the function
-// returns, so the caller may overwrite A afterwards.  Ends with a barrier.
+// sum_kk A[row][kk] B[col][kk] over a 64 x 64 tile (rows past m and columns
+// past n give 0), FP64/FP32 MFMA 16x16x4.  Four waves each own a 32 x 32
+// quarter (2 x 2 MFMA blocks c[i][j]: rows wr + 16 i, columns wc + 16 j).  K
+// is staged through LDS in chunks of 16 with the next chunk's global loads
+// issued before the current chunk's MFMAs.  Every read of A and B happens
+// before the function returns (it ends with a barrier), so the caller may
+// overwrite A afterwards.
 //
 // tb != 0 takes B transposed in memory (B[col][kk] at B + col * ldb + kk);
 // its chunks are then read 16 entries at a time along kk.
 template <typename T>
-__device__ __forceinline__ void tile_product(const T* A, int lda, const T* B, int ldb, bool tb, int m, int n, int k,
-                                             TileLds<T>& L, T (*Cs)[kTile + 1])
+__device__ __forceinline__ void tile_mma(const T* A, int lda, const T* B, int ldb, bool tb, int m, int n, int k,
+                                         TileLds<T>& L, typename Mfma<T>::acc_t (&c)[2][2])
 {
     typedef typename Mfma<T>::acc_t acc_t;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -369,14 +378,27 @@ __device__ __forceinline__ void tile_product(const T* A, int lda, const T* B, in
         }
         __syncthreads();
     }
-    const int q0 = wc + (lane & 15);
+    c[0][0] = c00;
+    c[0][1] = c01;
+    c[1][0] = c10;
+    c[1][1] = c11;
+}
+// The same product staged as Cs[col][row]; ends with a barrier.
+template <typename T>
+__device__ __forceinline__ void tile_product(const T* A, int lda, const T* B, int ldb, bool tb, int m, int n, int k,
+                                             TileLds<T>& L, T (*Cs)[kTile + 1])
+{
+    typename Mfma<T>::acc_t c[2][2];
+    tile_mma<T>(A, lda, B, ldb, tb, m, n, k, L, c);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int wr = (wv & 1) * 32, wc = (wv >> 1) * 32, q0 = wc + (lane & 15);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int r = wr + Mfma<T>::row(lane, v);
-        Cs[q0][r] = c00[v];
-        Cs[q0 + 16][r] = c01[v];
-        Cs[q0][r + 16] = c10[v];
-        Cs[q0 + 16][r + 16] = c11[v];
+        Cs[q0][r] = c[0][0][v];
+        Cs[q0 + 16][r] = c[0][1][v];
+        Cs[q0][r + 16] = c[1][0][v];
+        Cs[q0 + 16][r + 16] = c[1][1][v];
     }
     __syncthreads();
 }
@@ -388,12 +410,11 @@ __device__ __forceinline__ void tile_product(const T* A, int lda, const T* B, in
 // The result goes through LDS so the write of C is coalesced along columns.
 // KIND only names the instance after the kernel class that launches it
 // (TRSM, in-panel update, Schur update), so that profiles tell them apart.
-template <typename T, int KIND>
-__global__ __launch_bounds__(kThreads) void k_gemm(T* pool, const GemmTask* tasks)
+// The result goes through LDS so the write of C is coalesced along columns
+// (written straight from the accumulators it would be 32-byte pieces).
+template <typename T>
+__device__ __forceinline__ void gemm_task(T* pool, const GemmTask& t, TileLds<T>& L, T (*Cs)[kTile + 1])
 {
-    const GemmTask t = tasks[blockIdx.x];
-    __shared__ TileLds<T> L;
-    __shared__ T Cs[kTile][kTile + 1];
     tile_product<T>(pool + t.a, t.lda, pool + t.b, t.ldb, t.mode == 1, t.m, t.n, t.k, L, Cs);
     T* C = pool + t.c;
     const int tid = threadIdx.x, row = tid & 63;
@@ -404,6 +425,47 @@ __global__ __launch_bounds__(kThreads) void k_gemm(T* pool, const GemmTask* task
     } else {
         for (int col = tid >> 6; col < t.n; col += 4) C[row + (int64_t)col * t.ldc] = Cs[col][row];
     }
+}
+
+template <typename T, int KIND>
+__global__ __launch_bounds__(kThreads) void k_gemm(T* pool, const GemmTask* tasks)
+{
+    __shared__ TileLds<T> L;
+    __shared__ T Cs[kTile][kTile + 1];
+    gemm_task<T>(pool, tasks[blockIdx.x], L, Cs);
+}
+
+// One launch holding both the in-panel update of a step (GemmTasks from
+// workgroup ndiag on) and the factorization of the next diagonal blocks
+// (workgroups below ndiag).  A diagonal workgroup first applies the step's
+// panel to its own block (C -= X X^T, X the kprev columns left of it, which
+// the GemmTasks of this launch leave out), so the two parts are independent
+// and run side by side.  Used where the update has few tiles: the update
+// then no longer adds a launch to the critical path.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_upd_diag(T* pool, const NodeDev* nodes, const DiagTask* diags, int ndiag,
+                                                       const GemmTask* tiles, int* info)
+{
+    __shared__ Front64<T> S;
+    __shared__ TileLds<T> L;
+    if ((int)blockIdx.x >= ndiag) {
+        gemm_task<T>(pool, tiles[blockIdx.x - ndiag], L, S.A);
+        return;
+    }
+    const DiagTask t = diags[blockIdx.x];
+    const NodeDev nd = nodes[t.node];
+    const int n = nd.n, kb = t.kb;
+    T* C = pool + nd.front + t.k0 + (int64_t)t.k0 * n;
+    const T* X = pool + nd.front + t.k0 + (int64_t)(t.k0 - t.kprev) * n;
+    tile_product<T>(X, n, X, n, false, kb, kb, t.kprev, L, S.A);
+    for (int e = threadIdx.x; e < 64 * 64; e += kThreads) {
+        const int row = e & 63, col = e >> 6;
+        S.A[col][row] = (row < kb && col <= row) ? C[row + (int64_t)col * n] - S.A[col][row] : T(0);
+        S.X[col][row] = T(0);
+    }
+    __syncthreads();
+    factor_dense64_loaded<T>(S, C, n, kb, kb, info, nd.first + t.k0,
+                             pool + nd.inv + (int64_t)(t.k0 / kPanel) * kPanel * kPanel);
 }
 
 // C -= A B^T on a 128 x 128 tile (GemmTask with m, n <= 128, mode 0): the
@@ -988,6 +1050,14 @@ void launch_diag(T* pool, const NodeDev* nodes, const DiagTask* tasks, int count
 {
     if (count > 0) hipLaunchKernelGGL(k_diag<T>, dim3(count), dim3(kThreads), 0, st, pool, nodes, tasks, info);
 }
+template <typename T>
+void launch_upd_diag(T* pool, const NodeDev* nodes, const DiagTask* diags, int ndiag, const GemmTask* tiles, int ntiles,
+                     int* info, hipStream_t st)
+{
+    if (ndiag + ntiles > 0)
+        hipLaunchKernelGGL(k_upd_diag<T>, dim3(ndiag + ntiles), dim3(kThreads), 0, st, pool, nodes, diags, ndiag, tiles,
+                           info);
+}
 template <typename T> void launch_gemm(T* pool, const GemmTask* tasks, int count, int kind, hipStream_t st)
 {
     if (count <= 0) return;
@@ -1055,6 +1125,7 @@ void launch_scatter_rows(double* out, int ld_out, const double* in, int ld_in, c
     template void launch_extend_add<T>(T*, const int*, const NodeDev*, const ExtTask*, int, hipStream_t);              \
     template void launch_small_front<T>(T*, const NodeDev*, const int*, int, int*, hipStream_t);                       \
     template void launch_diag<T>(T*, const NodeDev*, const DiagTask*, int, int*, hipStream_t);                        \
+    template void launch_upd_diag<T>(T*, const NodeDev*, const DiagTask*, int, const GemmTask*, int, int*, hipStream_t); \
     template void launch_gemm<T>(T*, const GemmTask*, int, int, hipStream_t);                                          \
     template void launch_gemm128<T>(T*, const GemmTask*, int, hipStream_t);                                            \
     template void launch_fwd_step<T>(const T*, double*, int, const NodeDev*, const SolveTask*, int, hipStream_t);       \

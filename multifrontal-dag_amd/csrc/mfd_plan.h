@@ -31,10 +31,12 @@ struct ExtTask {
     int child, ti, tj, pad;
 };
 
-// Diagonal block [k0, k0+kb) of a large front, to be factored (its updates
-// are done) and inverted.
+// Diagonal block [k0, k0+kb) of a large front, to be factored and
+// inverted.  kprev > 0: the block still needs the update by the kprev
+// columns to its left (the previous panel), which the factorizing workgroup
+// applies itself; kprev = 0: all its updates are done.
 struct DiagTask {
-    int node, k0, kb, pad;
+    int node, k0, kb, kprev;
 };
 
 // One block step of the blocked supernodal solves: the pivot block

@@ -34,7 +34,18 @@ struct Launch {
     int64_t first;  // first task in the kind's task array
     int count;
     int wide = 0;   // GEMM launches: 1 = 128 x 128 tiles (k_gemm128)
+    // MFD_K_POTRF launches: fused = 1 runs the step's update tiles
+    // [first, first + count) of the GEMM tasks beside the diagonal tasks
+    // [dfirst, dfirst + dcount) (k_upd_diag); otherwise [first, first +
+    // count) are diagonal tasks (k_diag).
+    int fused = 0;
+    int64_t dfirst = 0;
+    int dcount = 0;
 };
+
+// Update launches with at most this many tiles run fused with the next
+// diagonal factorizations (one workgroup per CU at the fused kernel's LDS).
+constexpr int kFuseTiles = 256;
 
 // Update and Schur GEMMs: 128 x 128 tiles (k_gemm128) or 64 x 64 (k_gemm).
 constexpr bool kUseWide = false;
@@ -453,6 +464,17 @@ void build_plan(mfd_handle* h)
                 }
             }
             push(MFD_K_TRSM, f1, (int64_t)h->gemm.size());
+            // in-panel update of step t and the diagonal blocks of step t+1:
+            // fused into one launch when the update is small (the diagonal
+            // workgroups then apply the step's panel to their own block)
+            int64_t ntiles = 0;
+            for (int s : large) {
+                const NodeDev& d = h->nodes[s];
+                if (d.c <= k0) continue;
+                const int kb = std::min(kPanel, d.c - k0);
+                for (int c0 = k0 + kb; c0 < d.c; c0 += kWide) ntiles += (d.n - c0 + kWide - 1) / kWide;
+            }
+            const bool fuse = !kUseWide && ntiles <= kFuseTiles;
             int64_t f3 = (int64_t)h->gemm.size();
             for (int s : large) {
                 const NodeDev& d = h->nodes[s];
@@ -461,27 +483,44 @@ void build_plan(mfd_handle* h)
                 for (int c0 = k0 + kb; c0 < d.c; c0 += kWide) {
                     const int nc = std::min(kWide, d.c - c0);
                     for (int r0 = c0; r0 < d.n; r0 += kWide) {
+                        // fused: the next diagonal block (rows [c0, c0 + nc)
+                        // of the first tile) belongs to its diagonal task
+                        const int rs = (fuse && c0 == k0 + kb && r0 == c0) ? c0 + nc : r0;
+                        const int m = std::min(r0 + kWide, d.n) - rs;
+                        if (m <= 0) continue;
                         GemmTask g{};
-                        g.c = d.front + r0 + (int64_t)c0 * d.n;
-                        g.a = d.front + r0 + (int64_t)k0 * d.n;
+                        g.c = d.front + rs + (int64_t)c0 * d.n;
+                        g.a = d.front + rs + (int64_t)k0 * d.n;
                         g.b = d.front + c0 + (int64_t)k0 * d.n;
                         g.ldc = g.lda = g.ldb = d.n;
-                        g.m = std::min(kWide, d.n - r0);
+                        g.m = m;
                         g.n = nc;
                         g.k = kb;
-                        g.tri = r0 == c0;
+                        g.tri = rs == c0;
                         h->gemm.push_back(g);
                     }
                 }
             }
-            push(MFD_K_UPDATE, f3, (int64_t)h->gemm.size(), kUseWide);
             int64_t f2 = (int64_t)h->diag.size();
             for (int s : large) {
                 const NodeDev& d = h->nodes[s];
                 if (d.c <= k0 + kPanel) continue;
-                h->diag.push_back(DiagTask{s, k0 + kPanel, std::min(kPanel, d.c - k0 - kPanel), 0});
+                h->diag.push_back(DiagTask{s, k0 + kPanel, std::min(kPanel, d.c - k0 - kPanel), fuse ? kPanel : 0});
             }
-            push(MFD_K_POTRF, f2, (int64_t)h->diag.size());
+            if (fuse) {
+                const int64_t ng = (int64_t)h->gemm.size() - f3, nd = (int64_t)h->diag.size() - f2;
+                if (ng + nd > 0) {
+                    push(MFD_K_POTRF, f3, f3 + std::max<int64_t>(ng, 1));  // (pushes only if non-empty)
+                    Launch& l = h->fseq.back();
+                    l.count = (int)ng;
+                    l.fused = 1;
+                    l.dfirst = f2;
+                    l.dcount = (int)nd;
+                }
+            } else {
+                push(MFD_K_UPDATE, f3, (int64_t)h->gemm.size(), kUseWide);
+                push(MFD_K_POTRF, f2, (int64_t)h->diag.size());
+            }
         }
         int64_t f4 = (int64_t)h->gemm.size();
         for (int s : large) {
@@ -627,7 +666,13 @@ template <typename T> void enqueue_launch(mfd_handle* h, const Launch& l)
     switch (l.kind) {
     case MFD_K_EXTEND_ADD: launch_extend_add<T>(pool, h->d_lidx, h->d_nodes, h->d_ext + l.first, l.count, st); break;
     case MFD_K_SMALL_FRONT: launch_small_front<T>(pool, h->d_nodes, h->d_small + l.first, l.count, h->d_info, st); break;
-    case MFD_K_POTRF: launch_diag<T>(pool, h->d_nodes, h->d_diag + l.first, l.count, h->d_info, st); break;
+    case MFD_K_POTRF:
+        if (l.fused)
+            launch_upd_diag<T>(pool, h->d_nodes, h->d_diag + l.dfirst, l.dcount, h->d_gemm + l.first, l.count, h->d_info,
+                               st);
+        else
+            launch_diag<T>(pool, h->d_nodes, h->d_diag + l.first, l.count, h->d_info, st);
+        break;
     case MFD_K_TRSM:
     case MFD_K_UPDATE:
     case MFD_K_SCHUR:
@@ -1169,10 +1214,22 @@ int mfd_get_kernel_work(mfd_handle* h, int k, int s, double* flops, double* byte
             b += 2.0 * s * tri(d.n);
         }
         break;
-    case MFD_K_POTRF:  // diagonal block: Cholesky and inverse
-        for (const auto& t : h->diag) {
-            f += 2.0 * t.kb * t.kb * t.kb / 3.0;
-            b += s * (2.0 * tri(t.kb) + (double)t.kb * t.kb);
+    case MFD_K_POTRF:  // diagonal blocks: Cholesky and inverse (and, fused, the step's update)
+        for (const Launch& l : h->fseq) {
+            if (l.kind != k) continue;
+            const int64_t d0 = l.fused ? l.dfirst : l.first, dn = l.fused ? l.dcount : l.count;
+            for (int64_t i = d0; i < d0 + dn; ++i) {
+                const DiagTask& t = h->diag[i];
+                f += 2.0 * t.kb * t.kb * t.kb / 3.0 + (double)t.kb * (t.kb + 1) * t.kprev;
+                b += s * (2.0 * tri(t.kb) + (double)t.kb * t.kb + (double)t.kb * t.kprev);
+            }
+            if (!l.fused) continue;
+            for (int64_t i = l.first; i < l.first + l.count; ++i) {
+                const GemmTask& g = h->gemm[i];
+                double e = g.tri ? (double)g.n * g.m - tri(g.n - 1) : (double)g.m * g.n;
+                f += 2.0 * g.k * e;
+                b += s * ((double)g.m * g.k + (double)g.n * g.k + 2.0 * e);
+            }
         }
         break;
     case MFD_K_TRSM:  // rows below a diagonal block times its inverse
