@@ -22,9 +22,14 @@ FLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=gfx950", "-Wall",
          "-I", os.path.join(ROOT, "include"), "-I", SRC]
 
 
+# MFMA results in ordinary VGPRs: without it the accumulators of a K loop
+# are copied to AGPRs and back around every MFMA group.
+HIP_FLAGS = ["-mllvm", "-amdgpu-mfma-vgpr-form"]
+
+
 def _compile(src):
     obj = os.path.join(OUT, os.path.splitext(src)[0] + ".o")
-    lang = ["-x", "hip"] if src.endswith(".hip") else []
+    lang = ["-x", "hip"] + HIP_FLAGS if src.endswith(".hip") else []
     cmd = [HIPCC] + FLAGS + lang + ["-c", os.path.join(SRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     noise = "unknown target feature"

@@ -403,6 +403,91 @@ __device__ __forceinline__ void tile_product(const T* A, int lda, const T* B, in
     __syncthreads();
 }
 
+// Products with operands already in LDS, same wave layout as tile_mma.
+// Am[kk][row] holds A (k columns, zero from column k up to a multiple of
+// 16), B comes from global memory like tile_mma's B (staged through L.Bs in
+// chunks of 16).  Ends with a barrier.
+template <typename T>
+__device__ __forceinline__ void tile_mma_ldsA(const T (*Am)[kTile + 1], const T* B, int ldb, bool tb, int n, int k,
+                                              TileLds<T>& L, typename Mfma<T>::acc_t (&c)[2][2])
+{
+    typedef typename Mfma<T>::acc_t acc_t;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wr = (wv & 1) * 32, wc = (wv >> 1) * 32;
+    const int li = tid & 63, lp = tid >> 6, tcol = tid >> 4, tkk = tid & 15;
+    acc_t c00 = {0, 0, 0, 0}, c01 = {0, 0, 0, 0}, c10 = {0, 0, 0, 0}, c11 = {0, 0, 0, 0};
+    for (int k0 = 0; k0 < k; k0 += 16) {
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            if (tb) {
+                const int col = tcol + 16 * p, kt = k0 + tkk;
+                L.Bs[tkk][col] = (col < n && kt < k) ? B[(int64_t)col * ldb + kt] : T(0);
+            } else {
+                const int kk = k0 + lp + 4 * p;
+                L.Bs[lp + 4 * p][li] = (li < n && kk < k) ? B[li + (int64_t)kk * ldb] : T(0);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int kr = q * 4 + (lane >> 4);
+            const T a0 = Am[k0 + kr][wr + (lane & 15)], a1 = Am[k0 + kr][wr + 16 + (lane & 15)];
+            const T b0 = L.Bs[kr][wc + (lane & 15)], b1 = L.Bs[kr][wc + 16 + (lane & 15)];
+            c00 = Mfma<T>::mma(a0, b0, c00);
+            c01 = Mfma<T>::mma(a0, b1, c01);
+            c10 = Mfma<T>::mma(a1, b0, c10);
+            c11 = Mfma<T>::mma(a1, b1, c11);
+        }
+        __syncthreads();
+    }
+    c[0][0] = c00;
+    c[0][1] = c01;
+    c[1][0] = c10;
+    c[1][1] = c11;
+}
+// sum_kk Am[kk][row] Bm[kk][col] over kk < 64 with both operands in LDS
+// (zero past the product's true K).  Same wave layout as tile_mma; fully
+// unrolled so the accumulators stay in place; no barrier.
+template <typename T>
+__device__ __forceinline__ void lds_mma(const T (*Am)[kTile + 1], const T (*Bm)[kTile + 1],
+                                        typename Mfma<T>::acc_t (&c)[2][2])
+{
+    typedef typename Mfma<T>::acc_t acc_t;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int wr = (wv & 1) * 32 + (lane & 15), wc = (wv >> 1) * 32 + (lane & 15), kl = lane >> 4;
+    acc_t c00 = {0, 0, 0, 0}, c01 = {0, 0, 0, 0}, c10 = {0, 0, 0, 0}, c11 = {0, 0, 0, 0};
+#pragma unroll
+    for (int k0 = 0; k0 < kTile; k0 += 4) {
+        const T a0 = Am[k0 + kl][wr], a1 = Am[k0 + kl][wr + 16];
+        const T b0 = Bm[k0 + kl][wc], b1 = Bm[k0 + kl][wc + 16];
+        c00 = Mfma<T>::mma(a0, b0, c00);
+        c01 = Mfma<T>::mma(a0, b1, c01);
+        c10 = Mfma<T>::mma(a1, b0, c10);
+        c11 = Mfma<T>::mma(a1, b1, c11);
+    }
+    c[0][0] = c00;
+    c[0][1] = c01;
+    c[1][0] = c10;
+    c[1][1] = c11;
+}
+// Cs[col][row] = c (sub = false) or -= c (sub = true), in the layout of
+// tile_mma's result.
+template <typename T>
+__device__ __forceinline__ void stage_acc(const typename Mfma<T>::acc_t (&c)[2][2], T (*Cs)[kTile + 1], bool sub)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int wr = (wv & 1) * 32, wc = (wv >> 1) * 32, q0 = wc + (lane & 15);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                T& dst = Cs[q0 + 16 * j][wr + 16 * i + Mfma<T>::row(lane, v)];
+                dst = sub ? dst - c[i][j][v] : c[i][j][v];
+            }
+}
+
 // One GemmTask per workgroup (see mfd_plan.h): C -= A B^T (mode 0: the
 // DSYRK/DGEMM updates) or C = A B^T with B transposed in memory (mode 1: the
 // panel TRSM as a product with the stored L^{-T} of the diagonal block, C
@@ -418,6 +503,19 @@ __device__ __forceinline__ void gemm_task(T* pool, const GemmTask& t, TileLds<T>
     tile_product<T>(pool + t.a, t.lda, pool + t.b, t.ldb, t.mode == 1, t.m, t.n, t.k, L, Cs);
     T* C = pool + t.c;
     const int tid = threadIdx.x, row = tid & 63;
+    if (t.mode == 2) {
+        // Cs = C - A B^T, then C = Cs D^T (D = L^{-1}, stored transposed)
+        for (int col = tid >> 6; col < kTile; col += 4)
+            Cs[col][row] = (row < t.m && col < t.n) ? C[row + (int64_t)col * t.ldc] - Cs[col][row] : T(0);
+        __syncthreads();
+        typename Mfma<T>::acc_t c[2][2];
+        tile_mma_ldsA<T>(Cs, pool + t.d, kPanel, true, t.n, t.kd, L, c);
+        stage_acc<T>(c, Cs, false);
+        __syncthreads();
+        if (row < t.m)
+            for (int col = tid >> 6; col < t.n; col += 4) C[row + (int64_t)col * t.ldc] = Cs[col][row];
+        return;
+    }
     if (row >= t.m) return;
     if (t.mode == 0) {
         for (int col = tid >> 6; col < t.n; col += 4)
@@ -435,36 +533,108 @@ __global__ __launch_bounds__(kThreads) void k_gemm(T* pool, const GemmTask* task
     gemm_task<T>(pool, tasks[blockIdx.x], L, Cs);
 }
 
-// One launch holding both the in-panel update of a step (GemmTasks from
-// workgroup ndiag on) and the factorization of the next diagonal blocks
-// (workgroups below ndiag).  A diagonal workgroup first applies the step's
-// panel to its own block (C -= X X^T, X the kprev columns left of it, which
-// the GemmTasks of this launch leave out), so the two parts are independent
-// and run side by side.  Used where the update has few tiles: the update
-// then no longer adds a launch to the critical path.
+// One launch per panel step s of the large fronts: GemmTasks from workgroup
+// ndiag on (the panel TRSM of the rows after block s+1, preceded by panel
+// s-1's update of those rows, and, where small, panel s-1's update of the
+// column blocks after s+1) and, below ndiag, the factorization of block s+1.
+// A diagonal workgroup applies panels s-1 and s to its own block itself,
+// forming (and storing) its rows' panel-s entries on the way, so no task of
+// the launch reads what another one writes: the chain of diagonal blocks,
+// the critical path, advances one block per launch.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_upd_diag(T* pool, const NodeDev* nodes, const DiagTask* diags, int ndiag,
                                                        const GemmTask* tiles, int* info)
 {
     __shared__ Front64<T> S;
-    __shared__ TileLds<T> L;
+    __shared__ union {
+        TileLds<T> L;         // GEMM tasks
+        T B[2][64][kTile + 1];  // diagonal task: staged operands
+    } U;
     if ((int)blockIdx.x >= ndiag) {
-        gemm_task<T>(pool, tiles[blockIdx.x - ndiag], L, S.A);
+        gemm_task<T>(pool, tiles[blockIdx.x - ndiag], U.L, S.A);
         return;
     }
     const DiagTask t = diags[blockIdx.x];
     const NodeDev nd = nodes[t.node];
-    const int n = nd.n, kb = t.kb;
-    T* C = pool + nd.front + t.k0 + (int64_t)t.k0 * n;
-    const T* X = pool + nd.front + t.k0 + (int64_t)(t.k0 - t.kprev) * n;
-    tile_product<T>(X, n, X, n, false, kb, kb, t.kprev, L, S.A);
-    for (int e = threadIdx.x; e < 64 * 64; e += kThreads) {
-        const int row = e & 63, col = e >> 6;
-        S.A[col][row] = (row < kb && col <= row) ? C[row + (int64_t)col * n] - S.A[col][row] : T(0);
-        S.X[col][row] = T(0);
+    const int n = nd.n, kb = t.kb, kp = t.kprev, kpp = t.kpp;
+    T* front = pool + nd.front;
+    const T* C = front + t.k0 + (int64_t)t.k0 * n;                         // block s+1
+    T* Pg = front + t.k0 + (int64_t)(t.k0 - kp) * n;                       // its rows in panel s
+    const T* Xd = front + t.k0 + (int64_t)(t.k0 - kp - kpp) * n;          // its rows in panel s-1
+    const T* Xs = front + (t.k0 - kp) + (int64_t)(t.k0 - kp - kpp) * n;   // block s's rows in panel s-1
+    const T* inv = pool + nd.inv + (int64_t)((t.k0 - kp) / kPanel) * kPanel * kPanel;  // L_s^{-T}
+    const int tid = threadIdx.x, row = tid & 63, c4 = tid >> 6;
+    // Stage every operand with all loads of a batch in flight: entry (row,
+    // c4 + 4 j) of each 64 x 64 block; positions past the block read entry 0
+    // and are masked after the loads.
+    {
+        T r0[16], r1[16];
+        const unsigned ra = row < kb ? row : 0u, rb = row < kp ? row : 0u;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int col = c4 + 4 * j;
+            const unsigned cd = col < kpp ? col : 0u;
+            r0[j] = ld_off(Xd, ra + cd * (unsigned)n);
+            r1[j] = ld_off(Xs, rb + cd * (unsigned)n);
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int col = c4 + 4 * j;
+            U.B[0][col][row] = (row < kb && col < kpp) ? r0[j] : T(0);
+            U.B[1][col][row] = (row < kp && col < kpp) ? r1[j] : T(0);
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int col = c4 + 4 * j;
+            const unsigned ca = col <= row && row < kb ? col : 0u, cp = col < kp ? col : 0u;
+            r0[j] = ld_off(C, ra + ca * (unsigned)n);
+            r1[j] = ld_off((const T*)Pg, ra + cp * (unsigned)n);
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int col = c4 + 4 * j;
+            S.A[col][row] = (row < kb && col <= row) ? r0[j] : T(0);
+            S.X[col][row] = (row < kb && col < kp) ? r1[j] : T(0);
+        }
+    }
+    // inverse of block s in flight during the first products:
+    // B[k][j] = L_s^{-1}[j][k] = inv[k + 64 j]
+    T rv[16];
+#pragma unroll
+    for (int p = 0; p < 16; ++p) rv[p] = ld_off(inv, (unsigned)(tid + kThreads * p));
+    __syncthreads();
+    typename Mfma<T>::acc_t c[2][2];
+    if (kpp > 0) {
+        // panel s-1: S.A -= Xd Xd^T, S.X -= Xd Xs^T
+        lds_mma<T>(U.B[0], U.B[0], c);
+        stage_acc<T>(c, S.A, true);
+        lds_mma<T>(U.B[0], U.B[1], c);
+        stage_acc<T>(c, S.X, true);
     }
     __syncthreads();
-    factor_dense64_loaded<T>(S, C, n, kb, kb, info, nd.first + t.k0,
+#pragma unroll
+    for (int p = 0; p < 16; ++p) {
+        const int e = tid + kThreads * p, k = e & 63, j = e >> 6;
+        U.B[0][k][j] = (k < kp && j < kp) ? rv[p] : T(0);
+    }
+    __syncthreads();
+    // X = S.X L_s^{-T}, stored into the front (these rows are this task's)
+    lds_mma<T>(S.X, U.B[0], c);
+    __syncthreads();
+    stage_acc<T>(c, S.X, false);
+    __syncthreads();
+    for (int col = c4; col < kp; col += 4)
+        if (row < kb) Pg[row + (int64_t)col * n] = S.X[col][row];
+    // panel s: S.A -= X X^T
+    lds_mma<T>(S.X, S.X, c);
+    stage_acc<T>(c, S.A, true);
+    __syncthreads();
+    for (int col = c4; col < kTile; col += 4) {
+        S.X[col][row] = T(0);
+        if (!(row < kb && col <= row)) S.A[col][row] = T(0);
+    }
+    __syncthreads();
+    factor_dense64_loaded<T>(S, front + t.k0 + (int64_t)t.k0 * n, n, kb, kb, info, nd.first + t.k0,
                              pool + nd.inv + (int64_t)(t.k0 / kPanel) * kPanel * kPanel);
 }
 

@@ -32,11 +32,14 @@ struct ExtTask {
 };
 
 // Diagonal block [k0, k0+kb) of a large front, to be factored and
-// inverted.  kprev > 0: the block still needs the update by the kprev
-// columns to its left (the previous panel), which the factorizing workgroup
-// applies itself; kprev = 0: all its updates are done.
+// inverted.  kprev = 0: all its updates are done.  kprev > 0: the block
+// still needs the previous panel (the kprev columns to its left, block s)
+// and, with kpp > 0, the one before (kpp columns, block s-1); the
+// factorizing workgroup also forms the panel-s entries of its own rows
+// (their TRSM with block s's inverse) and stores them.
 struct DiagTask {
     int node, k0, kb, kprev;
+    int kpp, pad0, pad1, pad2;
 };
 
 // One block step of the blocked supernodal solves: the pivot block
@@ -53,13 +56,17 @@ struct SolveTask {
 // mode 0: C (m x n tile, ldc) -= A (m x k, lda) * B (n x k, ldb)^T;
 // mode 1: C = A B^T with B stored transposed (B[j][k] at b + j * ldb + k);
 // A may be C itself: the panel TRSM X = P L^{-T}, with b the stored L^{-T}
-// (column-major), whose transpose is L^{-1}.  Offsets are into the front
-// pool; tri != 0 keeps only the lower triangle (row >= col) of C.
+// (column-major), whose transpose is L^{-1};
+// mode 2: C = (C - A B^T) D^T with D = L^{-1} stored transposed at d
+// (ld 64) and K = kd: the panel TRSM preceded by the previous panel's
+// update of the same rows.  Offsets are into the front pool; tri != 0 keeps
+// only the lower triangle (row >= col) of C.
 struct GemmTask {
-    int64_t c, a, b;
+    int64_t c, a, b, d;
     int ldc, lda, ldb;
     int m, n, k;
     int tri, mode;
+    int kd, pad;
 };
 
 }  // namespace mfd

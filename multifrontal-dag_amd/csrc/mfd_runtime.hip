@@ -43,8 +43,8 @@ struct Launch {
     int dcount = 0;
 };
 
-// Update launches with at most this many tiles run fused with the next
-// diagonal factorizations (one workgroup per CU at the fused kernel's LDS).
+// A step's update tiles join the step's launch (k_upd_diag, one workgroup
+// per CU at its LDS) up to this many; more run as their own launch.
 constexpr int kFuseTiles = 256;
 
 // Update and Schur GEMMs: 128 x 128 tiles (k_gemm128) or 64 x 64 (k_gemm).
@@ -432,16 +432,22 @@ void build_plan(mfd_handle* h)
             else large.push_back(lv[k]);
         }
         push(MFD_K_SMALL_FRONT, first, (int64_t)h->small.size());
-        // large fronts, per 64-column step t: TRSM of the rows below the
-        // diagonal block (product with its stored inverse), the in-panel
-        // update of the remaining pivot columns, then the next diagonal
-        // block (Cholesky, inverse); one Schur update with K = c closes the
-        // level
+        // large fronts: the first diagonal block, then one launch per
+        // 64-column step t (k_upd_diag) holding
+        //  * the TRSM of panel t for the rows after block t+1, each tile
+        //    first applying panel t-1 to its rows (C = (C - A B^T) L_t^{-T}),
+        //  * panel t-1's update of the column blocks after t+1 (in its own
+        //    launch when it has many tiles; it is independent of the rest),
+        //  * block t+1: panels t-1 and t applied to it (its panel-t rows
+        //    solved and stored on the way), Cholesky and inverse;
+        // no task of a launch reads what another writes, so the chain of
+        // diagonal blocks advances one block per launch.  One Schur update
+        // with K = c closes the level.
         int steps = 0;
         for (int s : large) steps = std::max(steps, (h->nodes[s].c + kPanel - 1) / kPanel);
         {
             int64_t f0 = (int64_t)h->diag.size();
-            for (int s : large) h->diag.push_back(DiagTask{s, 0, std::min(kPanel, h->nodes[s].c), 0});
+            for (int s : large) h->diag.push_back(DiagTask{s, 0, std::min(kPanel, h->nodes[s].c), 0, 0, 0, 0, 0});
             push(MFD_K_POTRF, f0, (int64_t)h->diag.size());
         }
         for (int t = 0; t < steps; ++t) {
@@ -450,77 +456,75 @@ void build_plan(mfd_handle* h)
             for (int s : large) {
                 const NodeDev& d = h->nodes[s];
                 if (d.c <= k0) continue;
-                const int kb = std::min(kPanel, d.c - k0);
-                for (int r0 = k0 + kb; r0 < d.n; r0 += kTile) {
+                const int kb = std::min(kPanel, d.c - k0), kn = std::min(kPanel, d.c - k0 - kb);
+                for (int r0 = k0 + kb + kn; r0 < d.n; r0 += kTile) {
                     GemmTask g{};
-                    g.c = g.a = d.front + r0 + (int64_t)k0 * d.n;
-                    g.b = d.inv + (int64_t)t * kPanel * kPanel;  // L^{-T}, read transposed
+                    g.c = d.front + r0 + (int64_t)k0 * d.n;
                     g.ldc = g.lda = d.n;
-                    g.ldb = kPanel;
                     g.m = std::min(kTile, d.n - r0);
-                    g.n = g.k = kb;
-                    g.mode = 1;
+                    g.n = kb;
+                    if (t == 0) {
+                        g.a = g.c;
+                        g.b = d.inv;  // L^{-T}, read transposed
+                        g.ldb = kPanel;
+                        g.k = kb;
+                        g.mode = 1;
+                    } else {
+                        g.a = d.front + r0 + (int64_t)(k0 - kPanel) * d.n;
+                        g.b = d.front + k0 + (int64_t)(k0 - kPanel) * d.n;
+                        g.ldb = d.n;
+                        g.k = kPanel;
+                        g.d = d.inv + (int64_t)t * kPanel * kPanel;
+                        g.kd = kb;
+                        g.mode = 2;
+                    }
                     h->gemm.push_back(g);
                 }
             }
-            push(MFD_K_TRSM, f1, (int64_t)h->gemm.size());
-            // in-panel update of step t and the diagonal blocks of step t+1:
-            // fused into one launch when the update is small (the diagonal
-            // workgroups then apply the step's panel to their own block)
-            int64_t ntiles = 0;
-            for (int s : large) {
-                const NodeDev& d = h->nodes[s];
-                if (d.c <= k0) continue;
-                const int kb = std::min(kPanel, d.c - k0);
-                for (int c0 = k0 + kb; c0 < d.c; c0 += kWide) ntiles += (d.n - c0 + kWide - 1) / kWide;
-            }
-            const bool fuse = !kUseWide && ntiles <= kFuseTiles;
             int64_t f3 = (int64_t)h->gemm.size();
-            for (int s : large) {
-                const NodeDev& d = h->nodes[s];
-                if (d.c <= k0) continue;
-                const int kb = std::min(kPanel, d.c - k0);
-                for (int c0 = k0 + kb; c0 < d.c; c0 += kWide) {
-                    const int nc = std::min(kWide, d.c - c0);
-                    for (int r0 = c0; r0 < d.n; r0 += kWide) {
-                        // fused: the next diagonal block (rows [c0, c0 + nc)
-                        // of the first tile) belongs to its diagonal task
-                        const int rs = (fuse && c0 == k0 + kb && r0 == c0) ? c0 + nc : r0;
-                        const int m = std::min(r0 + kWide, d.n) - rs;
-                        if (m <= 0) continue;
-                        GemmTask g{};
-                        g.c = d.front + rs + (int64_t)c0 * d.n;
-                        g.a = d.front + rs + (int64_t)k0 * d.n;
-                        g.b = d.front + c0 + (int64_t)k0 * d.n;
-                        g.ldc = g.lda = g.ldb = d.n;
-                        g.m = m;
-                        g.n = nc;
-                        g.k = kb;
-                        g.tri = rs == c0;
-                        h->gemm.push_back(g);
+            if (t >= 1) {
+                const int kq = k0 - kPanel;  // panel t-1
+                for (int s : large) {
+                    const NodeDev& d = h->nodes[s];
+                    for (int c0 = k0 + kPanel; c0 < d.c; c0 += kTile) {
+                        const int nc = std::min(kTile, d.c - c0);
+                        for (int r0 = c0; r0 < d.n; r0 += kTile) {
+                            // block t+1 itself belongs to its diagonal task
+                            const int rs = (c0 == k0 + kPanel && r0 == c0) ? c0 + nc : r0;
+                            const int m = std::min(r0 + kTile, d.n) - rs;
+                            if (m <= 0) continue;
+                            GemmTask g{};
+                            g.c = d.front + rs + (int64_t)c0 * d.n;
+                            g.a = d.front + rs + (int64_t)kq * d.n;
+                            g.b = d.front + c0 + (int64_t)kq * d.n;
+                            g.ldc = g.lda = g.ldb = d.n;
+                            g.m = m;
+                            g.n = nc;
+                            g.k = kPanel;
+                            g.tri = rs == c0;
+                            h->gemm.push_back(g);
+                        }
                     }
                 }
             }
+            const int64_t f4u = (int64_t)h->gemm.size();
+            const bool fuse = f4u - f3 <= kFuseTiles;
             int64_t f2 = (int64_t)h->diag.size();
             for (int s : large) {
                 const NodeDev& d = h->nodes[s];
                 if (d.c <= k0 + kPanel) continue;
-                h->diag.push_back(DiagTask{s, k0 + kPanel, std::min(kPanel, d.c - k0 - kPanel), fuse ? kPanel : 0});
+                h->diag.push_back(DiagTask{s, k0 + kPanel, std::min(kPanel, d.c - k0 - kPanel), kPanel,
+                                           t >= 1 ? kPanel : 0, 0, 0, 0});
             }
-            if (fuse) {
-                const int64_t ng = (int64_t)h->gemm.size() - f3, nd = (int64_t)h->diag.size() - f2;
-                if (ng + nd > 0) {
-                    push(MFD_K_POTRF, f3, f3 + std::max<int64_t>(ng, 1));  // (pushes only if non-empty)
-                    Launch& l = h->fseq.back();
-                    l.count = (int)ng;
-                    l.fused = 1;
-                    l.dfirst = f2;
-                    l.dcount = (int)nd;
-                }
-            } else {
-                push(MFD_K_UPDATE, f3, (int64_t)h->gemm.size(), kUseWide);
-                push(MFD_K_POTRF, f2, (int64_t)h->diag.size());
+            const int64_t ng = (fuse ? f4u : f3) - f1, nd = (int64_t)h->diag.size() - f2;
+            if (ng + nd > 0) {
+                Launch l{MFD_K_POTRF, f1, (int)ng};
+                l.fused = 1;
+                l.dfirst = f2;
+                l.dcount = (int)nd;
+                h->fseq.push_back(l);
             }
+            if (!fuse) push(MFD_K_UPDATE, f3, f4u);
         }
         int64_t f4 = (int64_t)h->gemm.size();
         for (int s : large) {
@@ -1214,21 +1218,31 @@ int mfd_get_kernel_work(mfd_handle* h, int k, int s, double* flops, double* byte
             b += 2.0 * s * tri(d.n);
         }
         break;
-    case MFD_K_POTRF:  // diagonal blocks: Cholesky and inverse (and, fused, the step's update)
+    case MFD_K_POTRF:  // the per-step launches: diagonal blocks (with their
+                       // panel updates and TRSM), panel TRSM tiles, small updates
         for (const Launch& l : h->fseq) {
             if (l.kind != k) continue;
             const int64_t d0 = l.fused ? l.dfirst : l.first, dn = l.fused ? l.dcount : l.count;
             for (int64_t i = d0; i < d0 + dn; ++i) {
                 const DiagTask& t = h->diag[i];
-                f += 2.0 * t.kb * t.kb * t.kb / 3.0 + (double)t.kb * (t.kb + 1) * t.kprev;
-                b += s * (2.0 * tri(t.kb) + (double)t.kb * t.kb + (double)t.kb * t.kprev);
+                const double kb = t.kb, kp = t.kprev, kpp = t.kpp;
+                f += 2.0 * kb * kb * kb / 3.0 + kb * (kb + 1) * (kp + kpp) + 2.0 * kb * kp * kpp + kb * kp * kp;
+                b += s * (2.0 * tri(t.kb) + kb * kb + 2.0 * kb * kp + kp * kp + kpp * (kb + kp));
             }
             if (!l.fused) continue;
             for (int64_t i = l.first; i < l.first + l.count; ++i) {
                 const GemmTask& g = h->gemm[i];
-                double e = g.tri ? (double)g.n * g.m - tri(g.n - 1) : (double)g.m * g.n;
-                f += 2.0 * g.k * e;
-                b += s * ((double)g.m * g.k + (double)g.n * g.k + 2.0 * e);
+                if (g.mode == 1) {  // C = C L^{-T}
+                    f += (double)g.m * g.n * g.n;
+                    b += s * (2.0 * g.m * g.n + (double)g.n * g.n);
+                } else if (g.mode == 2) {  // C = (C - A B^T) L^{-T}
+                    f += 2.0 * g.m * g.n * g.k + (double)g.m * g.n * g.n;
+                    b += s * ((double)g.m * g.k + (double)g.n * g.k + 2.0 * g.m * g.n + (double)g.n * g.n);
+                } else {
+                    double e = g.tri ? (double)g.n * g.m - tri(g.n - 1) : (double)g.m * g.n;
+                    f += 2.0 * g.k * e;
+                    b += s * ((double)g.m * g.k + (double)g.n * g.k + 2.0 * e);
+                }
             }
         }
         break;
