@@ -49,6 +49,38 @@ template <> struct Mfma<float> {
 __device__ inline double dsqrt(double x) { return sqrt(x); }
 __device__ inline float dsqrt(float x) { return sqrtf(x); }
 
+// 1 / sqrt(d): the hardware estimate (v_rsq) refined by Newton-Raphson
+// steps r <- r + (r / 2) (1 - d r^2), each squaring the relative error (two
+// for double, one for float): within a few ulp of the exact value, with a
+// dependent chain of seven steps where a correctly rounded sqrt and
+// division take about twenty.
+__device__ __forceinline__ double rsqrt_nr(double d)
+{
+    double r = __builtin_amdgcn_rsq(d);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const double e = fma(-(d * r), r, 1.0);
+        r = fma(0.5 * r, e, r);
+    }
+    return r;
+}
+__device__ __forceinline__ float rsqrt_nr(float d)
+{
+    const float r = __builtin_amdgcn_rsqf(d);
+    const float e = fmaf(-(d * r), r, 1.0f);
+    return fmaf(0.5f * r, e, r);
+}
+// a / b given rb ~ 1 / b (within a few ulp): q = a rb, then one correction
+// with the exact residual a - q b (Markstein), which returns a / b exactly
+// whenever it is representable and is within an ulp of it otherwise.  Its
+// chain is a multiply and two fmas after rb and b, against the ten
+// dependent steps of a correctly rounded division.
+template <typename T> __device__ __forceinline__ T quot(T a, T b, T rb)
+{
+    const T q = a * rb;
+    return fma(fma(-q, b, a), rb, q);
+}
+
 // p[i] for a uniform base p and a 32-bit unsigned index: the address is
 // base + zero-extended byte offset, which keeps one VGPR per address when
 // many loads are in flight.
@@ -111,24 +143,61 @@ __device__ __forceinline__ void panel16(Front64<T>& S, int b16, int cb, int n, i
         x[j] = (live && (r >= 16 || j <= r)) ? S.A[b16 + j][row] : T(0);
         y[j] = j == r ? T(1) : T(0);
     }
+    // Column k: d = x_k on lane k, L_kk = sqrt(d) (correctly rounded),
+    // l = x_k / L_kk (quot: exact whenever the quotient is representable),
+    // y_k /= L_kk as a scaling by rsqrt(d), then x_j -= l l_j, y_j -= y_k l_j
+    // for j > k.  The first non-positive pivot is recorded after the loop.
+    int bad = 16;
+    if (cb == 16) {
+        // Column k's update of column k+1 is applied at once (the next pivot
+        // needs it); its updates of columns k+2.. are issued in step k+1,
+        // beside that step's pivot chain.  Every x_j receives its updates in
+        // column order, as in the plain loop.
+        T lp = T(0), yp = T(0);
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        if (k < cb) {
+        for (int k = 0; k < 16; ++k) {
             const T d = bcast(x[k], k);
-            if (!(d > T(0)) && r == 0) atomicMin(info, col0 + b16 + k + 1);
-            const T sd = dsqrt(d);
-            const T l = r == k ? sd : x[k] / sd;
-            x[k] = l;
-            const T y0 = y[k] / sd;
-            y[k] = y0;
+            bad = (!(d > T(0)) && bad == 16) ? k : bad;
+            if (k > 0) {
 #pragma unroll
-            for (int j = k + 1; j < 16; ++j) {
-                const T lj = bcast(l, j);
-                x[j] -= l * lj;
-                y[j] -= y0 * lj;
+                for (int j = k + 1; j < 16; ++j) {
+                    const T lj = bcast(lp, j);
+                    x[j] -= lp * lj;
+                    y[j] -= yp * lj;
+                }
+            }
+            const T rs = rsqrt_nr(d), sd = dsqrt(d);
+            const T l = quot(x[k], sd, rs), y0 = y[k] * rs;
+            x[k] = r == k ? sd : l;
+            y[k] = y0;
+            if (k + 1 < 16) {
+                const T lj = bcast(l, k + 1);
+                x[k + 1] -= l * lj;
+                y[k + 1] -= y0 * lj;
+            }
+            lp = l;
+            yp = y0;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if (k < cb) {
+                const T d = bcast(x[k], k);
+                bad = (!(d > T(0)) && bad == 16) ? k : bad;
+                const T rs = rsqrt_nr(d), sd = dsqrt(d);
+                const T l = quot(x[k], sd, rs), y0 = y[k] * rs;
+                x[k] = r == k ? sd : l;
+                y[k] = y0;
+#pragma unroll
+                for (int j = k + 1; j < 16; ++j) {
+                    const T lj = bcast(l, j);
+                    x[j] -= l * lj;
+                    y[j] -= y0 * lj;
+                }
             }
         }
     }
+    if (bad < 16 && r == 0) atomicMin(info, col0 + b16 + bad + 1);
     if (live) {
 #pragma unroll
         for (int j = 0; j < 16; ++j) S.A[b16 + j][row] = x[j];
