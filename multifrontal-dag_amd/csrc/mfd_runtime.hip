@@ -41,6 +41,10 @@ struct Launch {
     int fused = 0;
     int64_t dfirst = 0;
     int dcount = 0;
+    // Large in-panel updates run on the side stream (side = 1, numbered by
+    // side_id) beside the next step's launch; wait_id >= 0: this launch
+    // first waits for side launch wait_id.  Serial when profiling.
+    int side = 0, side_id = -1, wait_id = -1;
 };
 
 // A step's update tiles join the step's launch (k_upd_diag, one workgroup
@@ -77,6 +81,8 @@ struct mfd_handle {
     int nb = 0, flags = 0;
     bool device_ok = false;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;  // large in-panel updates beside the diagonal chain
+    hipEvent_t fork = nullptr, sev[2] = {};
     hipError_t last_hip = hipSuccess;
     int64_t mem_limit = 0, device_bytes = 0, front_bytes = 0;
     std::vector<void*> allocs;
@@ -403,6 +409,7 @@ void build_plan(mfd_handle* h)
         for (int j0 = 0; j0 < h->nodes[s].n; j0 += 64) h->fill.push_back(make_int2(s, j0));
     h->gemm.clear();
     h->fseq.clear();
+    int nside = 0;
     auto push = [&](int kind, int64_t first, int64_t end, int wide = 0) {
         if (end > first) h->fseq.push_back(Launch{kind, first, (int)(end - first), wide});
     };
@@ -450,6 +457,7 @@ void build_plan(mfd_handle* h)
             for (int s : large) h->diag.push_back(DiagTask{s, 0, std::min(kPanel, h->nodes[s].c), 0, 0, 0, 0, 0});
             push(MFD_K_POTRF, f0, (int64_t)h->diag.size());
         }
+        int prev_side = -1;
         for (int t = 0; t < steps; ++t) {
             const int k0 = t * kPanel;
             int64_t f1 = (int64_t)h->gemm.size();
@@ -509,6 +517,11 @@ void build_plan(mfd_handle* h)
             }
             const int64_t f4u = (int64_t)h->gemm.size();
             const bool fuse = f4u - f3 <= kFuseTiles;
+            if (!fuse) {  // panel t-1's update, beside this step's launch
+                push(MFD_K_UPDATE, f3, f4u);
+                h->fseq.back().side = 1;
+                h->fseq.back().side_id = nside;
+            }
             int64_t f2 = (int64_t)h->diag.size();
             for (int s : large) {
                 const NodeDev& d = h->nodes[s];
@@ -522,9 +535,10 @@ void build_plan(mfd_handle* h)
                 l.fused = 1;
                 l.dfirst = f2;
                 l.dcount = (int)nd;
+                l.wait_id = prev_side;  // panel t-2's update of block column t
                 h->fseq.push_back(l);
             }
-            if (!fuse) push(MFD_K_UPDATE, f3, f4u);
+            prev_side = fuse ? -1 : nside++;
         }
         int64_t f4 = (int64_t)h->gemm.size();
         for (int s : large) {
@@ -663,10 +677,9 @@ template <typename T> T* pool_of(mfd_handle* h);
 template <> double* pool_of<double>(mfd_handle* h) { return h->d_pool64; }
 template <> float* pool_of<float>(mfd_handle* h) { return h->d_pool32; }
 
-template <typename T> void enqueue_launch(mfd_handle* h, const Launch& l)
+template <typename T> void enqueue_launch(mfd_handle* h, const Launch& l, hipStream_t st)
 {
     T* pool = pool_of<T>(h);
-    hipStream_t st = h->stream;
     switch (l.kind) {
     case MFD_K_EXTEND_ADD: launch_extend_add<T>(pool, h->d_lidx, h->d_nodes, h->d_ext + l.first, l.count, st); break;
     case MFD_K_SMALL_FRONT: launch_small_front<T>(pool, h->d_nodes, h->d_small + l.first, l.count, h->d_info, st); break;
@@ -710,10 +723,31 @@ template <typename T> void enqueue_factor(mfd_handle* h, bool profile)
     launch_fill_lower<T>(pool, h->d_nodes, h->d_fill, (int)h->fill.size(), st);
     launch_scatter_a<T>(pool, h->d_a, h->d_dst, h->d_src, (int64_t)h->S.snidx.size(), st);
     mark(0);
+    // Side launches fork from the main stream's current position and run
+    // beside it; a step's launch joins the side launch it depends on, and
+    // any other launch joins every side launch before it.
+    const bool two = !profile && h->side;
+    int last_side = -1, joined = -1;
     for (const Launch& l : h->fseq) {
-        enqueue_launch<T>(h, l);
+        if (two && l.side) {
+            (void)hipEventRecord(h->fork, st);
+            (void)hipStreamWaitEvent(h->side, h->fork, 0);
+            enqueue_launch<T>(h, l, h->side);
+            (void)hipEventRecord(h->sev[l.side_id & 1], h->side);
+            last_side = l.side_id;
+            continue;
+        }
+        if (two && l.kind != MFD_K_POTRF && last_side > joined) {
+            (void)hipStreamWaitEvent(st, h->sev[last_side & 1], 0);
+            joined = last_side;
+        } else if (two && l.wait_id > joined) {
+            (void)hipStreamWaitEvent(st, h->sev[l.wait_id & 1], 0);
+            joined = l.wait_id;
+        }
+        enqueue_launch<T>(h, l, st);
         mark(l.kind);
     }
+    if (two && last_side > joined) (void)hipStreamWaitEvent(st, h->sev[last_side & 1], 0);
 }
 
 void collect_profile(mfd_handle* h, const std::vector<int>& kinds)
@@ -1004,6 +1038,9 @@ int mfd_create(mfd_handle** out, int nb, int flags)
             hipSetDevice(0) == hipSuccess && hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) == hipSuccess) {
             h->device_ok = true;
             for (auto& e : h->ev) (void)hipEventCreate(&e);
+            if (hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess) h->side = nullptr;
+            (void)hipEventCreateWithFlags(&h->fork, hipEventDisableTiming);
+            for (auto& e : h->sev) (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
         }
     }
     (void)hipGetLastError();
@@ -1019,6 +1056,10 @@ void mfd_destroy(mfd_handle* h)
         free_device(h);
         for (auto e : h->ev)
             if (e) (void)hipEventDestroy(e);
+        if (h->fork) (void)hipEventDestroy(h->fork);
+        for (auto e : h->sev)
+            if (e) (void)hipEventDestroy(e);
+        if (h->side) (void)hipStreamDestroy(h->side);
         (void)hipStreamDestroy(h->stream);
     }
     delete h->layout;
