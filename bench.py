@@ -163,6 +163,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--fp64-only", action="store_true", help="skip the FP32+IR leg and the profiled pass")
     ap.add_argument("--fp32-only", action="store_true", help="time only the FP32+IR leg (for profiling it)")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the profiled FP64 pass the roofline entry comes from (for rocprofv3)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device twice (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
@@ -186,6 +188,11 @@ def main():
     from mfdag import problems as P
 
     p, b = build_problem(args.size, args.seed)
+    if args.roofline_only:
+        roof, classes = roofline(p, b, args.max_zero, args.nb, mfdag.FP64)
+        print(json.dumps({"roofline": roof, "kernel_classes_fp64": {
+            k: {"ms": round(r["ms"], 4), "launches": r["launches"]} for k, r in classes.items()}}), flush=True)
+        return
     s = mfdag.Solver(nb=args.nb, graph=not args.no_graph)
     t0 = time.perf_counter()
     s.analyze(p.ccs_col, p.ccs_row, args.max_zero, p.perm)
