@@ -939,6 +939,18 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(const T* pool, double* zp
         li[p] = inv[e];
         lp[p] = ld_off(Lq, (unsigned)(min(i, kb - 1) + min(k, max(kp, 1) - 1) * n));
     }
+    // the first right-hand-side group's z and y, in flight with the above
+    double zv[2], yv[2];
+    auto load_zy = [&](int r0) {
+        const int nr = min(kRhsGroup, nrhs - r0);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int e = tid + kThreads * h, k = e & 63, r = e >> 6;
+            zv[h] = (k < kb && r < nr) ? Z[t.k0 + k + (int64_t)(r0 + r) * n] : 0.0;
+            yv[h] = (k < kp && r < nr) ? Y[t.k0 - 64 + k + (int64_t)(r0 + r) * c] : 0.0;
+        }
+    };
+    load_zy(0);
 #pragma unroll
     for (int p = 0; p < 16; ++p) {
         const int e = tid + kThreads * p, i = e & 63, k = e >> 6;
@@ -950,24 +962,30 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(const T* pool, double* zp
         LI[e & 63][e >> 6] = li[p];
         Lp[e >> 6][e & 63] = lp[p];
     }
+    // Lp, LI, zt and yt are zero past kp / kb, so the products run over all
+    // 64 entries, unrolled, with four partial sums.
     for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
         const int nr = min(kRhsGroup, nrhs - r0);
-        for (int e = tid; e < 64 * kRhsGroup; e += kThreads) {
-            const int k = e & 63, r = e >> 6;
-            zt[k][r] = (k < kb && r < nr) ? Z[t.k0 + k + (int64_t)(r0 + r) * n] : 0.0;
-            yt[k][r] = (k < kp && r < nr) ? Y[t.k0 - 64 + k + (int64_t)(r0 + r) * c] : 0.0;
+        if (r0 > 0) load_zy(r0);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int e = tid + kThreads * h;
+            zt[e & 63][e >> 6] = zv[h];
+            yt[e & 63][e >> 6] = yv[h];
         }
         __syncthreads();
         for (int r = ty; r < kRhsGroup; r += 4) {
-            double acc = 0.0;
-            for (int k = 0; k < kp; ++k) acc += Lp[k][lane] * yt[k][r];
-            zt[lane][r] -= acc;
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int k = 0; k < 64; ++k) acc[k & 3] += Lp[k][lane] * yt[k][r];
+            zt[lane][r] -= (acc[0] + acc[1]) + (acc[2] + acc[3]);
         }
         __syncthreads();
         for (int r = ty; r < nr; r += 4) {
-            double acc = 0.0;
-            for (int k = 0; k < kb; ++k) acc += LI[k][lane] * zt[k][r];
-            if (lane < kb) Y[t.k0 + lane + (int64_t)(r0 + r) * c] = acc;
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int k = 0; k < 64; ++k) acc[k & 3] += LI[k][lane] * zt[k][r];
+            if (lane < kb) Y[t.k0 + lane + (int64_t)(r0 + r) * c] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
         }
         __syncthreads();
     }
@@ -1110,19 +1128,24 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zp
             Xs[i][r] = (i < kn && r < nr) ? Y[t.k0 + 64 + i + (int64_t)(r0 + r) * c] : 0.0;
         }
         __syncthreads();
-        // z_k -= sum_i L[k0 + 64 + i][k0 + k] x_{t+1}[i]
+        // z_k -= sum_i L[k0 + 64 + i][k0 + k] x_{t+1}[i]; Ln, LI, Xs and zt
+        // are zero past kn / kb, so the sums run over all 64 entries,
+        // unrolled, with four partial sums
         for (int r = wv; r < nr; r += 4) {
-            double acc = 0.0;
-            for (int i = 0; i < kn; ++i) acc += Ln[lane][i] * Xs[i][r];
-            if (lane < kb) zt[lane][r] -= acc;
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int i = 0; i < 64; ++i) acc[i & 3] += Ln[lane][i] * Xs[i][r];
+            if (lane < kb) zt[lane][r] -= (acc[0] + acc[1]) + (acc[2] + acc[3]);
         }
         __syncthreads();
         for (int r = wv; r < nr; r += 4) {
-            double acc = 0.0;
-            for (int m = 0; m < kb; ++m) acc += u.LI[lane][m] * zt[m][r];
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int m = 0; m < 64; ++m) acc[m & 3] += u.LI[lane][m] * zt[m][r];
+            const double v = (acc[0] + acc[1]) + (acc[2] + acc[3]);
             if (lane < kb) {
-                Y[t.k0 + lane + (int64_t)(r0 + r) * c] = acc;
-                if (t.k0 == 0) x[(int64_t)(nd.first + lane) + (int64_t)(r0 + r) * ldx] = acc;
+                Y[t.k0 + lane + (int64_t)(r0 + r) * c] = v;
+                if (t.k0 == 0) x[(int64_t)(nd.first + lane) + (int64_t)(r0 + r) * ldx] = v;
             }
         }
         __syncthreads();

@@ -928,12 +928,12 @@ int backward_error(mfd_handle* h, const double* bp, const double* xp, double* r,
 }
 
 constexpr int kMaxRefine = 10;
-constexpr double kRefineTarget = 1e-15;
+constexpr double kRefineTarget = 1e-14;  // the backward error the FP64 form is held to
 
 // x for b, both postordered (ld n) in device memory.  FP32 factors are
 // refined in FP64 (residual with the FP64 matrix, correction with the FP32
 // factor) until the backward error reaches the target or stops improving;
-// if it then still exceeds 1e-14, the matrix is refactored in FP64.
+// if it then still exceeds the target, the matrix is refactored in FP64.
 int solve_permuted_device(mfd_handle* h, int nrhs)
 {
     const int64_t n = h->S.n;
@@ -957,7 +957,7 @@ int solve_permuted_device(mfd_handle* h, int nrhs)
     }
     h->st.refine_iters = it;
     h->st.backward_error = eta;
-    if (!(eta <= 1e-14)) {
+    if (!(eta <= kRefineTarget)) {
         // refinement stalled: FP64 refactorization fallback
         int rc = factorize_common(h, MFD_FP64);
         if (rc != MFD_OK) return rc;
