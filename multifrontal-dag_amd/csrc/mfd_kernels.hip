@@ -888,15 +888,15 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(const T* pool, double* zp
     if (t.row0 >= 0) {
         const int row = t.row0 + tid;
         const bool live = row < n;
-        // Unconditional loads from clamped (in-bounds) positions, masked
-        // afterwards, kept in the raw element type: a conditional or
-        // converted load would wait for the previous one.
+        // Unconditional loads from clamped (in-bounds) positions, kept in the
+        // raw element type (a conditional or converted load would wait for
+        // the previous one) and not masked: past column kb they multiply the
+        // zeros of yt, and rows past n are not stored, so the z and y loads
+        // below go out without waiting for these.
         const int rowc = min(row, n - 1);
         T l[64];
 #pragma unroll
         for (int k = 0; k < 64; ++k) l[k] = ld_off(Lb, (unsigned)(rowc + min(k, kb - 1) * n));
-#pragma unroll
-        for (int k = 0; k < 64; ++k) l[k] = (live && k < kb) ? l[k] : T(0);
         for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
             const int nr = min(kRhsGroup, nrhs - r0);
             double zr[kRhsGroup];
@@ -910,17 +910,14 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(const T* pool, double* zp
             if (live) {
 #pragma unroll 1
                 for (int r = 0; r < nr; ++r) {
-                    double acc0 = 0.0, acc1 = 0.0;
+                    double acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                    for (int k = 0; k < 64; k += 2) {
-                        acc0 += l[k] * yt[k][r];
-                        acc1 += l[k + 1] * yt[k + 1][r];
-                    }
+                    for (int k = 0; k < 64; ++k) acc[k & 3] += l[k] * yt[k][r];
                     double z = 0.0;
 #pragma unroll
                     for (int q = 0; q < kRhsGroup; ++q)
                         if (q == r) z = zr[q];
-                    Z[row + (int64_t)(r0 + r) * n] = z - (acc0 + acc1);
+                    Z[row + (int64_t)(r0 + r) * n] = z - ((acc[0] + acc[1]) + (acc[2] + acc[3]));
                 }
             }
             __syncthreads();
@@ -1048,15 +1045,15 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zp
         typedef typename Mfma<double>::acc_t acc_t;
         const int row = t.row0 + tid;
         const bool live = row < n;
-        // Unconditional loads from clamped (in-bounds) positions, masked
-        // afterwards, kept in the raw element type: a conditional or
-        // converted load would wait for the previous one.
+        // Unconditional loads from clamped (in-bounds) positions, kept in the
+        // raw element type (a conditional or converted load would wait for
+        // the previous one) and not masked: rows past n meet the zero x of
+        // Xs, and partials past column kb are never read, so the x loads
+        // below go out without waiting for these.
         const int rowc = min(row, n - 1);
         T l[64];
 #pragma unroll
         for (int k = 0; k < 64; ++k) l[k] = ld_off(Lb, (unsigned)(rowc + min(k, kb - 1) * n));
-#pragma unroll
-        for (int k = 0; k < 64; ++k) l[k] = (live && k < kb) ? l[k] : T(0);
         for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
             const int nr = min(kRhsGroup, nrhs - r0);
 #pragma unroll
