@@ -868,8 +868,9 @@ __device__ void load_inverse(const T* inv, double (*LI)[65])
 //   - the block task (row0 < 0) solves block t: z = Z_t - L[B_t, B_{t-1}]
 //     y_{t-1} (pad = order of block t-1; 0 for the first block), y_t =
 //     inv(L_tt) z, written to Y_t;
-//   - a chunk task (row0 >= 0) applies block k0/64 (= t - 1) to its 256
-//     rows, all after block t: Z[rows] -= L[rows, B_{t-1}] y_{t-1}.
+//   - a chunk task (row0 >= 0) applies block k0/64 (= t - 1) to its 64
+//     rows, all after block t: Z[rows] -= L[rows, B_{t-1}] y_{t-1} (four
+//     threads per row, 16 columns each, summed through LDS).
 // No task of a launch depends on another task of the same launch, and every
 // global read of a task is issued before its first wait.
 template <typename T>
@@ -878,15 +879,23 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(const T* pool, double* zp
 {
     const SolveTask t = tasks[blockIdx.x];
     const NodeDev nd = nodes[t.node];
-    __shared__ double LI[64][65];
-    __shared__ double Lp[64][65];  // Lp[k][i] = L[k0 + i][k0 - 64 + k], block task
+    __shared__ union {
+        struct {
+            double LI[64][65];
+            double Lp[64][65];  // Lp[k][i] = L[k0 + i][k0 - 64 + k], block task
+        } b;
+        double red[4][64][kRhsGroup + 1];  // chunk task: partial sums per column quarter
+    } u;
     __shared__ double zt[64][kRhsGroup], yt[64][kRhsGroup];
+    double(*LI)[65] = u.b.LI;
+    double(*Lp)[65] = u.b.Lp;
     const int n = nd.n, c = nd.c, kb = t.kb, tid = threadIdx.x, lane = tid & 63, ty = tid >> 6;
     double* Z = zpool + nd.z * nrhs;
     double* Y = Z + (int64_t)n * nrhs;
     const T* Lb = pool + nd.front + (int64_t)t.k0 * n;
     if (t.row0 >= 0) {
-        const int row = t.row0 + tid;
+        // 64 rows; thread (row lane, quarter ty) takes columns 16 ty .. 16 ty + 15
+        const int row = t.row0 + lane;
         const bool live = row < n;
         // Unconditional loads from clamped (in-bounds) positions, kept in the
         // raw element type (a conditional or converted load would wait for
@@ -894,31 +903,39 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(const T* pool, double* zp
         // zeros of yt, and rows past n are not stored, so the z and y loads
         // below go out without waiting for these.
         const int rowc = min(row, n - 1);
-        T l[64];
+        T l[16];
 #pragma unroll
-        for (int k = 0; k < 64; ++k) l[k] = ld_off(Lb, (unsigned)(rowc + min(k, kb - 1) * n));
+        for (int j = 0; j < 16; ++j) l[j] = ld_off(Lb, (unsigned)(rowc + min(16 * ty + j, kb - 1) * n));
         for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
             const int nr = min(kRhsGroup, nrhs - r0);
-            double zr[kRhsGroup];
-#pragma unroll
-            for (int r = 0; r < kRhsGroup; ++r) zr[r] = (live && r < nr) ? Z[row + (int64_t)(r0 + r) * n] : 0.0;
+            // this thread stores right-hand sides r0 + ty and r0 + ty + 4
+            const double z0 = (live && ty < nr) ? Z[row + (int64_t)(r0 + ty) * n] : 0.0;
+            const double z1 = (live && ty + 4 < nr) ? Z[row + (int64_t)(r0 + ty + 4) * n] : 0.0;
             for (int e = tid; e < 64 * kRhsGroup; e += kThreads) {
                 const int k = e & 63, r = e >> 6;
                 yt[k][r] = (k < kb && r < nr) ? Y[t.k0 + k + (int64_t)(r0 + r) * c] : 0.0;
             }
             __syncthreads();
+            double acc[kRhsGroup];
+#pragma unroll
+            for (int r = 0; r < kRhsGroup; ++r) acc[r] = 0.0;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const double lk = (double)l[j];
+#pragma unroll
+                for (int r = 0; r < kRhsGroup; ++r) acc[r] += lk * yt[16 * ty + j][r];
+            }
+#pragma unroll
+            for (int r = 0; r < kRhsGroup; ++r) u.red[ty][lane][r] = acc[r];
+            __syncthreads();
             if (live) {
-#pragma unroll 1
-                for (int r = 0; r < nr; ++r) {
-                    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                    for (int k = 0; k < 64; ++k) acc[k & 3] += l[k] * yt[k][r];
-                    double z = 0.0;
-#pragma unroll
-                    for (int q = 0; q < kRhsGroup; ++q)
-                        if (q == r) z = zr[q];
-                    Z[row + (int64_t)(r0 + r) * n] = z - ((acc[0] + acc[1]) + (acc[2] + acc[3]));
-                }
+                if (ty < nr)
+                    Z[row + (int64_t)(r0 + ty) * n] =
+                        z0 - ((u.red[0][lane][ty] + u.red[1][lane][ty]) + (u.red[2][lane][ty] + u.red[3][lane][ty]));
+                if (ty + 4 < nr)
+                    Z[row + (int64_t)(r0 + ty + 4) * n] =
+                        z1 - ((u.red[0][lane][ty + 4] + u.red[1][lane][ty + 4]) +
+                              (u.red[2][lane][ty + 4] + u.red[3][lane][ty + 4]));
             }
             __syncthreads();
         }

@@ -291,7 +291,7 @@ void build_solve_plan(mfd_handle* h)
                 if (t >= 1 && t - 1 <= T - 1) {
                     // block t - 1 applied to the rows after block t
                     const int k0 = 64 * (t - 1), kb = std::min(64, d.c - k0);
-                    for (int r0 = std::min(64 * (t + 1), d.c); r0 < d.n; r0 += 256)
+                    for (int r0 = std::min(64 * (t + 1), d.c); r0 < d.n; r0 += 64)
                         h->fstep.push_back(SolveTask{s, k0, kb, r0, 0, 0, 0, 0});
                 }
             }
