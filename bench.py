@@ -67,6 +67,48 @@ def timed_steps(s, d_a, d_b, d_x, precision, steps, warmup):
     return wall, float(np.median(fms)), float(np.median(sms))
 
 
+def batch_leg(args):
+    """The batching workload: 512 anisotropic 2D Laplacians (L ~ U{16..160},
+    y-coupling log-uniform in [1e-3, 1], A_ii = 2 + 2 c_y), one right-hand
+    side each, analysed into one forest (mfd_analyze_batch) and factored and
+    solved together; FP64 and FP32 + refinement, timed like the main leg."""
+    import torch
+    import mfdag
+    from mfdag import problems as P
+    t0 = time.perf_counter()
+    systems = P.batch_anisotropic(args.batch, seed=args.seed)
+    bp = P.batch_problem(systems)
+    xs, bs = zip(*[P.rhs(p, 1, seed=args.seed + k) for k, p in enumerate(systems)])
+    b = np.concatenate(bs)
+    s = mfdag.Solver(nb=args.nb, graph=not args.no_graph)
+    t1 = time.perf_counter()
+    s.analyze_batch([(p.ccs_col, p.ccs_row) for p in systems], args.batch_max_zero, [p.perm for p in systems])
+    analyze_s = time.perf_counter() - t1
+    log(f"batch: {len(systems)} systems, n={bp.n}, fronts={s.num_node}, cost={s.cost():.3e} "
+        f"(setup {t1 - t0:.1f}s, analysis {analyze_s:.2f}s)")
+    dev = torch.device("cuda:0")
+    d_a = torch.from_numpy(np.ascontiguousarray(bp.val)).to(dev)
+    d_b = torch.from_numpy(np.ascontiguousarray(b.T)).to(dev)
+    d_x = torch.empty_like(d_b)
+    torch.cuda.synchronize()
+    out = {"workload": f"{len(systems)} anisotropic 2D 5-point Laplacians, L~U{{16..160}}, c_y~logU[1e-3,1], "
+                       f"A_ii=2+2c_y, ND per system, max_zero={args.batch_max_zero}, 1 RHS each",
+           "n": bp.n, "fronts": s.num_node, "cost_flops": s.cost(), "analyze_s": round(analyze_s, 3)}
+    for name, prec in (("fp64", mfdag.FP64), ("fp32_ir", mfdag.FP32_IR)):
+        wall, f, so = timed_steps(s, d_a, d_b, d_x, prec, args.steps, args.warmup)
+        x = d_x.cpu().numpy().T
+        st = s.stats()
+        out[name] = {"ms_per_step": round(wall * 1e3, 3), "factor_ms": round(f, 3), "solve_ms": round(so, 3),
+                     "factor_gflops": round(s.cost() / (f * 1e6), 2),
+                     "backward_error": P.batch_backward_error(systems, x, b)}
+        if prec == mfdag.FP32_IR:
+            out[name]["refine_iters"] = st["refine_iters"]
+        log(f"batch {name}: step {wall * 1e3:.2f} ms (factor {f:.2f} ms, solve {so:.2f} ms), "
+            f"eta {out[name]['backward_error']:.2e}")
+    s.close()
+    return out
+
+
 def roofline(p, b, max_zero, nb, precision):
     """One profiled factorization + solve: per-class time (HIP events on the
     library's stream, one pair per launch; the solve sweeps as a whole)
@@ -163,6 +205,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--fp64-only", action="store_true", help="skip the FP32+IR leg and the profiled pass")
     ap.add_argument("--fp32-only", action="store_true", help="time only the FP32+IR leg (for profiling it)")
+    ap.add_argument("--batch", type=int, default=512, help="systems in the batching leg (0: skip it)")
+    ap.add_argument("--batch-max-zero", type=int, default=4096)
+    ap.add_argument("--batch-only", action="store_true", help="run only the batching leg")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the profiled FP64 pass the roofline entry comes from (for rocprofv3)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -187,6 +232,9 @@ def main():
     import mfdag
     from mfdag import problems as P
 
+    if args.batch_only:
+        print(json.dumps({"batch": batch_leg(args)}), flush=True)
+        return
     p, b = build_problem(args.size, args.seed)
     if args.roofline_only:
         roof, classes = roofline(p, b, args.max_zero, args.nb, mfdag.FP64)
@@ -231,6 +279,7 @@ def main():
     roof, classes = roofline(p, b, args.max_zero, args.nb, mfdag.FP64)
     log("kernel classes (FP64):", json.dumps({k: {kk: round(v, 4) for kk, v in r.items()} for k, r in classes.items()}))
 
+    batch = batch_leg(args) if args.batch > 0 else None
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_size, args.max_zero, args.nb, args.seed)
 
     value = cost / (wall64 * 1e9)
@@ -251,6 +300,7 @@ def main():
         "roofline": roof,
         "kernel_classes_fp64": {k: {"ms": round(r["ms"], 4), "launches": r["launches"]} for k, r in classes.items()},
         "cpu_baseline": cpu,
+        "batch": batch,
     }
     print(json.dumps(out), flush=True)
 

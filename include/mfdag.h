@@ -84,6 +84,19 @@ void mfd_destroy(mfd_handle* h);
    storage.  Unlike the reference, ccs_row is copied, not retained. */
 int mfd_analyze(mfd_handle* h, int n, const int* ccs_col, const int* ccs_row, int max_zero, const int* perm);
 
+/* The batching form: nsys independent systems, system k given like
+   mfd_analyze's arguments (n[k], ccs_col[k], ccs_row[k], perm[k]; perm and
+   each perm[k] nullable), each analysed exactly as mfd_analyze would and
+   joined into one elimination forest that the level schedules factor and
+   solve together.  Afterwards the handle holds one block-diagonal system:
+   mfd_factorize takes the systems' values concatenated (system 0's CCS
+   entries first) and mfd_solve the systems' right-hand sides stacked (rows
+   of system 0 first).  No counterpart in the reference, whose analysis takes
+   one matrix with one elimination tree (a reducible matrix returns
+   MFD_ERR_REDUCIBLE from mfd_analyze). */
+int mfd_analyze_batch(mfd_handle* h, int nsys, const int* n, const int* const* ccs_col, const int* const* ccs_row,
+                      int max_zero, const int* const* perm);
+
 /* The structure given directly, as the reference's driver tests build it
    (seq_factorize_test.f90:23-35): per node supernode_size, work_size and
    parent (1-based, 0 for the root, postordered), local_index (sum of

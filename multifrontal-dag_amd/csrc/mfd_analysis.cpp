@@ -10,6 +10,7 @@
 #include "mfd_analysis.h"
 
 #include <algorithm>
+#include <climits>
 #include <numeric>
 
 #include "mfdag.h"
@@ -561,6 +562,49 @@ int structure_from_parts(int num_node, const int* c, const int* w, const int* pa
                 }
         }
     }
+    return MFD_OK;
+}
+
+int analyze_batch(int nsys, const int* n, const int* const* ccs_col, const int* const* ccs_row, int max_zero,
+                  const int* const* perm, Structure& out)
+{
+    if (nsys < 1 || !n || !ccs_col || !ccs_row) return MFD_ERR_ARG;
+    out = Structure();
+    int64_t entries = 0;  // caller entries of the systems before this one
+    for (int k = 0; k < nsys; ++k) {
+        if (!ccs_col[k] || !ccs_row[k]) return MFD_ERR_ARG;
+        Structure S;
+        int rc = analyze(n[k], ccs_col[k], ccs_row[k], max_zero, perm ? perm[k] : nullptr, S);
+        if (rc != MFD_OK) return rc;
+        if ((int64_t)out.n + S.n > INT32_MAX - 1) return MFD_ERR_ARG;
+        const int col_off = out.n, node_off = out.num_node;
+        const int64_t a_off = (int64_t)out.snidx.size(), l_off = (int64_t)out.l_rows.size(),
+                      li_off = (int64_t)out.lidx.size();
+        for (int v : S.perm) out.perm.push_back(v + col_off);
+        for (int64_t v : S.ccs_perm) out.ccs_perm.push_back(v + entries);
+        for (int j = 0; j < S.n; ++j) out.a_colptr.push_back(S.a_colptr[j] + a_off);
+        out.snidx.insert(out.snidx.end(), S.snidx.begin(), S.snidx.end());
+        out.c.insert(out.c.end(), S.c.begin(), S.c.end());
+        out.w.insert(out.w.end(), S.w.begin(), S.w.end());
+        for (int v : S.parent) out.parent.push_back(v ? v + node_off : 0);
+        for (int s = 0; s < S.num_node; ++s) {
+            out.first.push_back(S.first[s] + col_off);
+            out.l_ptr.push_back(S.l_ptr[s] + l_off);
+            out.li_ptr.push_back(S.li_ptr[s] + li_off);
+            std::vector<int> ch(S.children[s]);
+            for (int& v : ch) v += node_off;
+            out.children.push_back(std::move(ch));
+        }
+        for (int v : S.l_rows) out.l_rows.push_back(v + col_off);
+        out.lidx.insert(out.lidx.end(), S.lidx.begin(), S.lidx.end());
+        out.n += S.n;
+        out.num_node += S.num_node;
+        entries += (int64_t)ccs_col[k][n[k]] - 1;
+    }
+    out.a_colptr.push_back((int64_t)out.snidx.size());
+    out.first.push_back(out.n + 1);
+    out.l_ptr.push_back((int64_t)out.l_rows.size());
+    out.li_ptr.push_back((int64_t)out.lidx.size());
     return MFD_OK;
 }
 

@@ -31,6 +31,14 @@ struct Structure {
 // error code.
 int analyze(int n, const int* ccs_col, const int* ccs_row, int max_zero, const int* perm, Structure& out);
 
+// Several independent systems analysed one by one (each exactly as
+// analyze would) and joined into one elimination forest: system k's
+// columns, nodes and entries follow those of systems 0..k-1, so the
+// caller's values and right-hand sides are the systems' concatenated.
+// perm may be null, and so may each perm[k].
+int analyze_batch(int nsys, const int* n, const int* const* ccs_col, const int* const* ccs_row, int max_zero,
+                  const int* const* perm, Structure& out);
+
 // Structure given directly (node sizes, parents, local index and the
 // supernodal CCS of A), as the reference's driver tests build it.
 int structure_from_parts(int num_node, const int* c, const int* w, const int* parent, const int* lidx,

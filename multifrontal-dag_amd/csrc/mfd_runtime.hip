@@ -1080,6 +1080,21 @@ int mfd_analyze(mfd_handle* h, int n, const int* ccs_col, const int* ccs_row, in
     return rc;
 }
 
+int mfd_analyze_batch(mfd_handle* h, int nsys, const int* n, const int* const* ccs_col, const int* const* ccs_row,
+                      int max_zero, const int* const* perm)
+{
+    if (!h) return MFD_ERR_ARG;
+    double t0 = now_ms();
+    h->analyzed = h->factored = false;
+    h->from_parts = false;
+    int rc = analyze_batch(nsys, n, ccs_col, ccs_row, max_zero, perm, h->S);
+    if (rc != MFD_OK) return rc;
+    rc = finish_analysis(h);
+    h->times[MFD_TIME_ANALYZE] = (now_ms() - t0) / 1e3;
+    h->st.analyze_ms = now_ms() - t0;
+    return rc;
+}
+
 int mfd_set_structure(mfd_handle* h, int num_node, const int* c, const int* w, const int* parent, const int* lidx,
                       const int64_t* a_colptr, const int* snidx)
 {

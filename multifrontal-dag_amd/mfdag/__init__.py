@@ -62,6 +62,7 @@ def lib():
             "mfd_create": (i, [C.POINTER(P), i, i]),
             "mfd_destroy": (None, [P]),
             "mfd_analyze": (i, [P, i, ip, ip, i, ip]),
+            "mfd_analyze_batch": (i, [P, i, ip, C.POINTER(ip), C.POINTER(ip), i, C.POINTER(ip)]),
             "mfd_set_structure": (i, [P, i, ip, ip, ip, ip, lp, ip]),
             "mfd_factorize": (i, [P, dp, i]),
             "mfd_factorize_device": (i, [P, C.c_void_p, i]),
@@ -141,6 +142,30 @@ class Solver:
         p = _i32(perm) if perm is not None else None
         self._check(lib().mfd_analyze(self.h, n, _ptr(ccs_col, C.c_int), _ptr(ccs_row, C.c_int), int(max_zero),
                                       _ptr(p, C.c_int)), "mfd_analyze")
+        self._dims()
+
+    def analyze_batch(self, systems, max_zero=0, perms=None):
+        """The batching form: systems = [(ccs_col, ccs_row), ...] (each like
+        analyze's), perms = None or a list with one perm (or None) per system.
+        Afterwards the solver holds the block-diagonal system: values and
+        right-hand sides are the systems' concatenated
+        (problems.batch_problem builds that system for checks)."""
+        k = len(systems)
+        cols = [_i32(c) for c, _ in systems]
+        rows = [_i32(r) for _, r in systems]
+        ns = np.array([len(c) - 1 for c in cols], np.int32)
+        ip = C.POINTER(C.c_int)
+        col_arr = (ip * k)(*[_ptr(c, C.c_int) for c in cols])
+        row_arr = (ip * k)(*[_ptr(r, C.c_int) for r in rows])
+        perm_arr = None
+        keep = []
+        if perms is not None:
+            ps = [_i32(p) if p is not None else None for p in perms]
+            keep = ps
+            perm_arr = (ip * k)(*[_ptr(p, C.c_int) if p is not None else ip() for p in ps])
+        self._check(lib().mfd_analyze_batch(self.h, k, _ptr(ns, C.c_int), col_arr, row_arr, int(max_zero), perm_arr),
+                    "mfd_analyze_batch")
+        del keep
         self._dims()
 
     def set_structure(self, supernode_size, work_size, parent, local_index, a_colptr, supernodal_index):

@@ -225,6 +225,57 @@ def block_stencil_3dof(nx, seed=0, leaf=64):
     return Problem(n, cc, rr, vv, perm.astype(np.int32), f"blk3_{nx}")
 
 
+def anisotropic_laplacian_2d(L, cy, leaf=64):
+    """2D 5-point anisotropic Laplacian on an L x L grid, Dirichlet:
+    x-coupling 1, y-coupling cy, A_ii = 2 + 2 cy."""
+    n = L * L
+    (a1, b1), (a2, b2) = grid_edges((L, L), [(1, 0), (0, 1)])
+    r = np.concatenate([np.arange(n), b1, b2])
+    c = np.concatenate([np.arange(n), a1, a2])
+    v = np.concatenate([np.full(n, 2.0 + 2.0 * cy), np.full(len(a1), -1.0), np.full(len(a2), -cy)])
+    cc, rr, vv = lower_ccs(n, r, c, v)
+    return Problem(n, cc, rr, vv, nested_dissection((L, L), leaf), f"alap2d_{L}_{cy:.3g}")
+
+
+def batch_anisotropic(nsys=512, seed=0, lo=16, hi=160, leaf=64):
+    """The batching workload: nsys independent anisotropic Laplacians, system
+    k on an L_k x L_k grid with L_k ~ U{lo..hi} and y-coupling c_y
+    log-uniform in [1e-3, 1]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    Ls = rng.integers(lo, hi + 1, size=nsys)
+    cys = np.exp(rng.uniform(np.log(1e-3), 0.0, size=nsys))
+    return [anisotropic_laplacian_2d(int(L), float(cy), leaf) for L, cy in zip(Ls, cys)]
+
+
+def batch_problem(systems):
+    """The block-diagonal system of a batch, as one Problem (for checks):
+    values and right-hand sides are the systems' concatenated, in the
+    order mfd_analyze_batch joins them."""
+    n = sum(p.n for p in systems)
+    cols, rows, vals = [np.zeros(1, np.int64)], [], []
+    off = 0
+    eoff = 0
+    for p in systems:
+        cols.append(np.asarray(p.ccs_col[1:], np.int64) - 1 + eoff)
+        rows.append(np.asarray(p.ccs_row, np.int64) + off)
+        vals.append(np.asarray(p.val))
+        off += p.n
+        eoff += len(p.ccs_row)
+    cc = (np.concatenate(cols) + 1).astype(np.int32)
+    return Problem(n, cc, np.concatenate(rows).astype(np.int32), np.concatenate(vals), None,
+                   f"batch_{len(systems)}")
+
+
+def batch_backward_error(systems, x, b):
+    """max over the systems of a batch of each system's backward error
+    (x, b: the systems' solutions and right-hand sides stacked)."""
+    worst, off = 0.0, 0
+    for p in systems:
+        worst = max(worst, backward_error(p, x[off:off + p.n], b[off:off + p.n]))
+        off += p.n
+    return worst
+
+
 def rhs(problem, nrhs=1, seed=0):
     """x_true ~ U(-1, 1) (n x nrhs) and b = A x_true."""
     rng = np.random.Generator(np.random.PCG64(seed + 12345))
