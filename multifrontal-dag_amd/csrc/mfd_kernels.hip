@@ -597,9 +597,14 @@ __device__ __forceinline__ void gemm_task(T* pool, const GemmTask& t, TileLds<T>
 template <typename T, int KIND>
 __global__ __launch_bounds__(kThreads) void k_gemm(T* pool, const GemmTask* tasks)
 {
-    __shared__ TileLds<T> L;
-    __shared__ T Cs[kTile][kTile + 1];
-    gemm_task<T>(pool, tasks[blockIdx.x], L, Cs);
+    // modes 0 and 1 only: the staged result Cs is written after the last
+    // read of the operand staging L, so the two share LDS (four workgroups
+    // per CU instead of three)
+    __shared__ union {
+        TileLds<T> L;
+        T Cs[kTile][kTile + 1];
+    } u;
+    gemm_task<T>(pool, tasks[blockIdx.x], u.L, u.Cs);
 }
 
 // One launch per panel step s of the large fronts: GemmTasks from workgroup
