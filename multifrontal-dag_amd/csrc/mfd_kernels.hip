@@ -320,9 +320,19 @@ __global__ __launch_bounds__(kThreads) void k_fill_lower(T* pool, const NodeDev*
     const int2 t = tasks[blockIdx.x];
     const NodeDev nd = nodes[t.x];
     const int n = nd.n, jend = min(t.y + 64, n);
+    constexpr int V = 16 / sizeof(T);  // elements per 16-byte store
     T* g = pool + nd.front;
-    for (int j = t.y; j < jend; ++j)
-        for (int i = j + threadIdx.x; i < n; i += kThreads) g[i + (int64_t)j * n] = T(0);
+    for (int j = t.y; j < jend; ++j) {
+        // rows [j, n) of column j: scalar head up to a 16-byte boundary,
+        // 16-byte stores, scalar tail
+        T* col = g + (int64_t)j * n;
+        const int head = min((int)((V - ((reinterpret_cast<uintptr_t>(col + j) / sizeof(T)) % V)) % V), n - j);
+        const int i0 = j + head, nv = (n - i0) / V, tail = i0 + nv * V;
+        if ((int)threadIdx.x < head) col[j + threadIdx.x] = T(0);
+        uint4* v = reinterpret_cast<uint4*>(col + i0);
+        for (int p = threadIdx.x; p < nv; p += kThreads) v[p] = make_uint4(0u, 0u, 0u, 0u);
+        if ((int)threadIdx.x < n - tail) col[tail + threadIdx.x] = T(0);
+    }
 }
 
 // pool[dst[k]] = a[src[k]]: each stored A entry lands on exactly one front
