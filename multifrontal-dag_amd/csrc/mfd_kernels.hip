@@ -1003,7 +1003,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(const T* pool, double* zp
             yt[e & 63][e >> 6] = yv[h];
         }
         __syncthreads();
-        for (int r = ty; r < kRhsGroup; r += 4) {
+        for (int r = ty; r < nr; r += 4) {
             double acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int k = 0; k < 64; ++k) acc[k & 3] += Lp[k][lane] * yt[k][r];
