@@ -1129,6 +1129,27 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zp
         // (row k0 itself when there is no next block: always inside the front)
         ln[p] = ld_off(Lb, (unsigned)(t.k0 + (kn > 0 ? 64 + min(i, kn - 1) : 0) + min(k, kb - 1) * n));
     }
+    // z = y_t - the partial sums, and x of block t+1, for one group of
+    // right-hand sides (two entries of each per thread); the first group's
+    // loads go out with the factor loads above
+    double zv[2], xv[2];
+    auto load_zx = [&](int r0) {
+        const int nr = min(kRhsGroup, nrhs - r0);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int e = tid + kThreads * h, k = e & 63, r = e >> 6;
+            double z = 0.0;
+            if (k < kb && r < nr) {
+                z = Y[t.k0 + k + (int64_t)(r0 + r) * c];
+                const double* pp = part + (int64_t)t.slot * 64 * nrhs + k + (int64_t)(r0 + r) * 64;
+#pragma unroll 4
+                for (int ch = 0; ch < t.nchunk; ++ch) z -= pp[(int64_t)ch * 64 * nrhs];
+            }
+            zv[h] = z;
+            xv[h] = (k < kn && r < nr) ? Y[t.k0 + 64 + k + (int64_t)(r0 + r) * c] : 0.0;
+        }
+    };
+    load_zx(0);
 #pragma unroll
     for (int p = 0; p < 16; ++p) {
         const int e = tid + kThreads * p, i = e & 63, k = e >> 6;
@@ -1142,19 +1163,13 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zp
     }
     for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
         const int nr = min(kRhsGroup, nrhs - r0);
-        for (int e = tid; e < 64 * kRhsGroup; e += kThreads) {
-            const int k = e & 63, r = e >> 6;
-            double z = 0.0;
-            if (k < kb && r < nr) {
-                z = Y[t.k0 + k + (int64_t)(r0 + r) * c];
-                for (int ch = 0; ch < t.nchunk; ++ch) z -= part[(int64_t)(t.slot + ch) * 64 * nrhs + k + (int64_t)(r0 + r) * 64];
-            }
-            zt[k][r] = z;
-        }
-        // x of block t+1, staged in Xs[0..kn)
-        for (int e = tid; e < 64 * kRhsGroup; e += kThreads) {
-            const int i = e & 63, r = e >> 6;
-            Xs[i][r] = (i < kn && r < nr) ? Y[t.k0 + 64 + i + (int64_t)(r0 + r) * c] : 0.0;
+        if (r0 > 0) load_zx(r0);
+        // zt, and x of block t+1 staged in Xs[0..kn)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int e = tid + kThreads * h;
+            zt[e & 63][e >> 6] = zv[h];
+            Xs[e & 63][e >> 6] = xv[h];
         }
         __syncthreads();
         // z_k -= sum_i L[k0 + 64 + i][k0 + k] x_{t+1}[i]; Ln, LI, Xs and zt
