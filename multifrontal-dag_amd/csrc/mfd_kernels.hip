@@ -524,9 +524,9 @@ __device__ __forceinline__ void tile_mma_ldsA(const T (*Am)[kTile + 1], const T*
     c[1][0] = c10;
     c[1][1] = c11;
 }
-// sum_kk Am[kk][row] Bm[kk][col] over kk < 64 with both operands in LDS
-// (zero past the product's true K).  Same wave layout as tile_mma; fully
-// unrolled so the accumulators stay in place; no barrier.
+// c += sum_kk Am[kk][row] Bm[kk][col] over kk < 64 with both operands in
+// LDS (zero past the product's true K).  Same wave layout as tile_mma;
+// fully unrolled so the accumulators stay in place; no barrier.
 template <typename T>
 __device__ __forceinline__ void lds_mma(const T (*Am)[kTile + 1], const T (*Bm)[kTile + 1],
                                         typename Mfma<T>::acc_t (&c)[2][2])
@@ -534,7 +534,7 @@ __device__ __forceinline__ void lds_mma(const T (*Am)[kTile + 1], const T (*Bm)[
     typedef typename Mfma<T>::acc_t acc_t;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int wr = (wv & 1) * 32 + (lane & 15), wc = (wv >> 1) * 32 + (lane & 15), kl = lane >> 4;
-    acc_t c00 = {0, 0, 0, 0}, c01 = {0, 0, 0, 0}, c10 = {0, 0, 0, 0}, c11 = {0, 0, 0, 0};
+    acc_t c00 = c[0][0], c01 = c[0][1], c10 = c[1][0], c11 = c[1][1];
 #pragma unroll
     for (int k0 = 0; k0 < kTile; k0 += 4) {
         const T a0 = Am[k0 + kl][wr], a1 = Am[k0 + kl][wr + 16];
@@ -618,13 +618,14 @@ __global__ __launch_bounds__(kThreads) void k_gemm(T* pool, const GemmTask* task
 }
 
 // One launch per panel step s of the large fronts: GemmTasks from workgroup
-// ndiag on (the panel TRSM of the rows after block s+1, preceded by panel
-// s-1's update of those rows, and, where small, panel s-1's update of the
-// column blocks after s+1) and, below ndiag, the factorization of block s+1.
-// A diagonal workgroup applies panels s-1 and s to its own block itself,
-// forming (and storing) its rows' panel-s entries on the way, so no task of
-// the launch reads what another one writes: the chain of diagonal blocks,
-// the critical path, advances one block per launch.
+// ndiag on (the panel TRSM of the rows after block s+1, preceded by the
+// update of those rows by the panels still pending for block column s, and,
+// where small, the periodic in-panel update of the column blocks after s+1)
+// and, below ndiag, the factorization of block s+1.  A diagonal workgroup
+// applies the pending panels and panel s to its own block itself, forming
+// (and storing) its rows' panel-s entries on the way, so no task of the
+// launch reads what another one writes: the chain of diagonal blocks, the
+// critical path, advances one block per launch.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_upd_diag(T* pool, const NodeDev* nodes, const DiagTask* diags, int ndiag,
                                                        const GemmTask* tiles, int* info)
@@ -644,8 +645,8 @@ __global__ __launch_bounds__(kThreads) void k_upd_diag(T* pool, const NodeDev* n
     T* front = pool + nd.front;
     const T* C = front + t.k0 + (int64_t)t.k0 * n;                         // block s+1
     T* Pg = front + t.k0 + (int64_t)(t.k0 - kp) * n;                       // its rows in panel s
-    const T* Xd = front + t.k0 + (int64_t)(t.k0 - kp - kpp) * n;          // its rows in panel s-1
-    const T* Xs = front + (t.k0 - kp) + (int64_t)(t.k0 - kp - kpp) * n;   // block s's rows in panel s-1
+    const T* Xd = front + t.k0 + (int64_t)(t.k0 - kp - kpp) * n;          // its rows in the pending panels
+    const T* Xs = front + (t.k0 - kp) + (int64_t)(t.k0 - kp - kpp) * n;   // block s's rows in them
     const T* inv = pool + nd.inv + (int64_t)((t.k0 - kp) / kPanel) * kPanel * kPanel;  // L_s^{-T}
     const int tid = threadIdx.x, row = tid & 63, c4 = tid >> 6;
     // Stage every operand with all loads of a batch in flight: entry (row,
@@ -687,13 +688,37 @@ __global__ __launch_bounds__(kThreads) void k_upd_diag(T* pool, const NodeDev* n
 #pragma unroll
     for (int p = 0; p < 16; ++p) rv[p] = ld_off(inv, (unsigned)(tid + kThreads * p));
     __syncthreads();
-    typename Mfma<T>::acc_t c[2][2];
+    typedef typename Mfma<T>::acc_t acc_t;
+    acc_t c[2][2] = {{{0, 0, 0, 0}, {0, 0, 0, 0}}, {{0, 0, 0, 0}, {0, 0, 0, 0}}};
     if (kpp > 0) {
-        // panel s-1: S.A -= Xd Xd^T, S.X -= Xd Xs^T
-        lds_mma<T>(U.B[0], U.B[0], c);
+        // the pending panels (64 columns per round): S.A -= Xd Xd^T,
+        // S.X -= Xd Xs^T
+        acc_t c2[2][2] = {{{0, 0, 0, 0}, {0, 0, 0, 0}}, {{0, 0, 0, 0}, {0, 0, 0, 0}}};
+        for (int q = 0; q < kpp; q += 64) {
+            if (q > 0) {
+                T r0[16], r1[16];
+                const unsigned ra = row < kb ? row : 0u, rb = row < kp ? row : 0u;
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const int col = q + c4 + 4 * j;
+                    const unsigned cd = col < kpp ? col : 0u;
+                    r0[j] = ld_off(Xd, ra + cd * (unsigned)n);
+                    r1[j] = ld_off(Xs, rb + cd * (unsigned)n);
+                }
+                __syncthreads();
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const int col = c4 + 4 * j;
+                    U.B[0][col][row] = (row < kb && q + col < kpp) ? r0[j] : T(0);
+                    U.B[1][col][row] = (row < kp && q + col < kpp) ? r1[j] : T(0);
+                }
+                __syncthreads();
+            }
+            lds_mma<T>(U.B[0], U.B[0], c);
+            lds_mma<T>(U.B[0], U.B[1], c2);
+        }
         stage_acc<T>(c, S.A, true);
-        lds_mma<T>(U.B[0], U.B[1], c);
-        stage_acc<T>(c, S.X, true);
+        stage_acc<T>(c2, S.X, true);
     }
     __syncthreads();
 #pragma unroll
@@ -703,6 +728,7 @@ __global__ __launch_bounds__(kThreads) void k_upd_diag(T* pool, const NodeDev* n
     }
     __syncthreads();
     // X = S.X L_s^{-T}, stored into the front (these rows are this task's)
+    c[0][0] = c[0][1] = c[1][0] = c[1][1] = acc_t{0, 0, 0, 0};
     lds_mma<T>(S.X, U.B[0], c);
     __syncthreads();
     stage_acc<T>(c, S.X, false);
@@ -710,6 +736,7 @@ __global__ __launch_bounds__(kThreads) void k_upd_diag(T* pool, const NodeDev* n
     for (int col = c4; col < kp; col += 4)
         if (row < kb) Pg[row + (int64_t)col * n] = S.X[col][row];
     // panel s: S.A -= X X^T
+    c[0][0] = c[0][1] = c[1][0] = c[1][1] = acc_t{0, 0, 0, 0};
     lds_mma<T>(S.X, S.X, c);
     stage_acc<T>(c, S.A, true);
     __syncthreads();

@@ -34,9 +34,10 @@ struct ExtTask {
 // Diagonal block [k0, k0+kb) of a large front, to be factored and
 // inverted.  kprev = 0: all its updates are done.  kprev > 0: the block
 // still needs the previous panel (the kprev columns to its left, block s)
-// and, with kpp > 0, the one before (kpp columns, block s-1); the
-// factorizing workgroup also forms the panel-s entries of its own rows
-// (their TRSM with block s's inverse) and stores them.
+// and, with kpp > 0, the kpp columns before those (panels no in-panel
+// update has applied yet); the factorizing workgroup also forms the
+// panel-s entries of its own rows (their TRSM with block s's inverse) and
+// stores them.
 struct DiagTask {
     int node, k0, kb, kprev;
     int kpp, pad0, pad1, pad2;

@@ -51,6 +51,12 @@ struct Launch {
 // per CU at its LDS) up to this many; more run as their own launch.
 constexpr int kFuseTiles = 256;
 
+// The in-panel update of the large fronts runs every kUpdPeriod steps with
+// K = kUpdPeriod panels (fewer passes over the trailing columns); the TRSM
+// tiles and the diagonal task of a step apply the panels still pending for
+// their columns themselves (at most kUpdPeriod, 64 columns each).
+constexpr int kUpdPeriod = 2;
+
 // Update and Schur GEMMs: 128 x 128 tiles (k_gemm128) or 64 x 64 (k_gemm).
 constexpr bool kUseWide = false;
 constexpr int kWide = kUseWide ? 128 : kTile;
@@ -460,6 +466,11 @@ void build_plan(mfd_handle* h)
         int prev_side = -1;
         for (int t = 0; t < steps; ++t) {
             const int k0 = t * kPanel;
+            // panels not yet applied to block column t (and to block t+1):
+            // the in-panel update runs every kUpdPeriod steps, on the
+            // kUpdPeriod panels before it
+            const int pend = t == 0 ? 0 : ((t % kUpdPeriod) == 0 ? kUpdPeriod : t % kUpdPeriod) * kPanel;
+            const bool upd = t >= kUpdPeriod && t % kUpdPeriod == 0;
             int64_t f1 = (int64_t)h->gemm.size();
             for (int s : large) {
                 const NodeDev& d = h->nodes[s];
@@ -478,10 +489,10 @@ void build_plan(mfd_handle* h)
                         g.k = kb;
                         g.mode = 1;
                     } else {
-                        g.a = d.front + r0 + (int64_t)(k0 - kPanel) * d.n;
-                        g.b = d.front + k0 + (int64_t)(k0 - kPanel) * d.n;
+                        g.a = d.front + r0 + (int64_t)(k0 - pend) * d.n;
+                        g.b = d.front + k0 + (int64_t)(k0 - pend) * d.n;
                         g.ldb = d.n;
-                        g.k = kPanel;
+                        g.k = pend;
                         g.d = d.inv + (int64_t)t * kPanel * kPanel;
                         g.kd = kb;
                         g.mode = 2;
@@ -490,8 +501,8 @@ void build_plan(mfd_handle* h)
                 }
             }
             int64_t f3 = (int64_t)h->gemm.size();
-            if (t >= 1) {
-                const int kq = k0 - kPanel;  // panel t-1
+            if (upd) {
+                const int kq = k0 - kUpdPeriod * kPanel;  // panels t-kUpdPeriod .. t-1
                 for (int s : large) {
                     const NodeDev& d = h->nodes[s];
                     for (int c0 = k0 + kPanel; c0 < d.c; c0 += kTile) {
@@ -508,7 +519,7 @@ void build_plan(mfd_handle* h)
                             g.ldc = g.lda = g.ldb = d.n;
                             g.m = m;
                             g.n = nc;
-                            g.k = kPanel;
+                            g.k = kUpdPeriod * kPanel;
                             g.tri = rs == c0;
                             h->gemm.push_back(g);
                         }
@@ -526,8 +537,7 @@ void build_plan(mfd_handle* h)
             for (int s : large) {
                 const NodeDev& d = h->nodes[s];
                 if (d.c <= k0 + kPanel) continue;
-                h->diag.push_back(DiagTask{s, k0 + kPanel, std::min(kPanel, d.c - k0 - kPanel), kPanel,
-                                           t >= 1 ? kPanel : 0, 0, 0, 0});
+                h->diag.push_back(DiagTask{s, k0 + kPanel, std::min(kPanel, d.c - k0 - kPanel), kPanel, pend, 0, 0, 0});
             }
             const int64_t ng = (fuse ? f4u : f3) - f1, nd = (int64_t)h->diag.size() - f2;
             if (ng + nd > 0) {
