@@ -1076,9 +1076,10 @@ __global__ __launch_bounds__(kThreads) void k_bwd_gather(double* zpool, int nrhs
 //       pivot part of x);  pad = kb of block t+1 (0: no such block);
 //   - a chunk task (row0 >= 0) forms, for block k0/64 (= t - 1), the partial
 //     products P[slot][k][r] = sum over its 256 rows of L[row][k0+k] x[row][r]
-//     (x from Y on pivot rows, from Z on update rows; every row lies after
-//     block t, so its x is final) on MFMA: the chunk's L columns are read once
-//     into registers, then staged 64 rows at a time through LDS.
+//     for its 16 of the block's columns (pad0 = which 16; four tasks per
+//     chunk) or, with pad0 < 0, for all 64 (launches with many chunks) on
+//     MFMA (x from Y on pivot rows, from Z on update rows; every row lies
+//     after block t, so its x is final).
 // No task of a launch depends on another task of the same launch, and the
 // partials are summed in a fixed order.
 template <typename T>
@@ -1100,7 +1101,9 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zp
     double* Z = zpool + nd.z * nrhs;
     double* Y = Z + (int64_t)n * nrhs;
     const T* Lb = pool + nd.front + (int64_t)t.k0 * n;
-    if (t.row0 >= 0) {
+    if (t.row0 >= 0 && t.pad0 < 0) {
+        // all 64 columns in one task: wave w takes columns 16 w .. 16 w + 15,
+        // the 256 rows staged 64 at a time through LDS
         typedef typename Mfma<double>::acc_t acc_t;
         const int row = t.row0 + tid;
         const bool live = row < n;
@@ -1140,6 +1143,55 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zp
                 double* P = part + (int64_t)t.slot * 64 * nrhs + (int64_t)(r0 + rc) * 64;
 #pragma unroll
                 for (int v = 0; v < 4; ++v) P[16 * wv + Mfma<double>::row(lane, v)] = acc[v];
+            }
+            __syncthreads();
+        }
+        return;
+    }
+    if (t.row0 >= 0) {
+        // columns 16 pad0 .. 16 pad0 + 15 of block k0/64 over the chunk's 256
+        // rows: one row per thread, a quarter of the rows per wave, the four
+        // waves' 16 x 16 products added through LDS
+        typedef typename Mfma<double>::acc_t acc_t;
+        const int col0 = 16 * t.pad0, row = t.row0 + tid;
+        const bool live = row < n;
+        // Unconditional loads from clamped (in-bounds) positions, kept in the
+        // raw element type (a conditional or converted load would wait for
+        // the previous one) and not masked: rows past n meet the zero x of
+        // Xs, and partials past column kb are never read, so the x loads
+        // below go out without waiting for these.
+        const int rowc = min(row, n - 1);
+        T l[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) l[j] = ld_off(Lb, (unsigned)(rowc + min(col0 + j, kb - 1) * n));
+        double(*Lc)[kThreads + 4] = reinterpret_cast<double(*)[kThreads + 4]>(&u.Ls[0][0]);   // 16 x 260
+        double(*red)[16][17] = reinterpret_cast<double(*)[16][17]>(&u.Ls[0][0]);              // 4 x 16 x 17
+        const int rc = lane & 15;
+        for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
+            const int nr = min(kRhsGroup, nrhs - r0);
+#pragma unroll
+            for (int r = 0; r < kRhsGroup; ++r)
+                Xs[tid][r] = (live && r < nr) ? (row < c ? Y[row + (int64_t)(r0 + r) * c] : Z[row + (int64_t)(r0 + r) * n])
+                                              : 0.0;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) Lc[j][tid] = (double)l[j];
+            __syncthreads();
+            acc_t acc = {0, 0, 0, 0};
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int rr = 64 * wv + 4 * q + (lane >> 4);
+                const double a = Lc[rc][rr];
+                const double b = rc < nr ? Xs[rr][rc] : 0.0;
+                acc = Mfma<double>::mma(a, b, acc);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int v = 0; v < 4; ++v) red[wv][Mfma<double>::row(lane, v)][rc] = acc[v];
+            __syncthreads();
+            if (tid < 16 * nr) {
+                const int m = tid & 15, r = tid >> 4;
+                part[(int64_t)t.slot * 64 * nrhs + (int64_t)(r0 + r) * 64 + col0 + m] =
+                    (red[0][m][r] + red[1][m][r]) + (red[2][m][r] + red[3][m][r]);
             }
             __syncthreads();
         }

@@ -51,6 +51,10 @@ struct Launch {
 // per CU at its LDS) up to this many; more run as their own launch.
 constexpr int kFuseTiles = 256;
 
+// Backward solve steps with at most this many row chunks split each chunk
+// into four 16-column tasks (shorter tasks where the step is latency-bound).
+constexpr int kSplitChunks = 256;
+
 // The in-panel update of the large fronts runs every kUpdPeriod steps with
 // K = kUpdPeriod panels (fewer passes over the trailing columns); the TRSM
 // tiles and the diagonal task of a step apply the panels still pending for
@@ -337,6 +341,14 @@ void build_solve_plan(mfd_handle* h)
         for (int t = steps_of(L); t >= 0; --t) {
             int64_t f0 = (int64_t)h->bstep.size();
             int idx = 0;
+            // few chunks (latency-bound steps): four 16-column tasks per
+            // chunk; many: one task per chunk
+            int64_t nch = 0;
+            for (int k = b0; k < b1; ++k) {
+                const NodeDev& d = h->nodes[h->level_nodes[k]];
+                if (t >= 1 && t - 1 <= blocks(d) - 1) nch += chunk_rows(d, t - 1).second;
+            }
+            const bool split = nch <= kSplitChunks;
             for (int k = b0; k < b1; ++k) {
                 const int s = h->level_nodes[k];
                 const NodeDev& d = h->nodes[s];
@@ -351,8 +363,10 @@ void build_solve_plan(mfd_handle* h)
                     const auto rc = chunk_rows(d, j);
                     qfirst[s] = (int)((j % 2) * bank + idx);
                     qcount[s] = rc.second;
-                    for (int q = 0; q < rc.second; ++q)
-                        h->bstep.push_back(SolveTask{s, k0, kb, rc.first + 256 * q, (int)((j % 2) * bank + idx++), 0, 0, 0});
+                    for (int q = 0; q < rc.second; ++q, ++idx)
+                        for (int cq = 0; cq < (split ? 4 : 1); ++cq)
+                            h->bstep.push_back(
+                                SolveTask{s, k0, kb, rc.first + 256 * q, (int)((j % 2) * bank + idx), 0, split ? cq : -1, 0});
                 }
             }
             push(h->bwd_seq, kBS, f0, (int64_t)h->bstep.size());
