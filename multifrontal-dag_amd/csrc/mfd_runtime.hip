@@ -49,7 +49,7 @@ struct Launch {
 
 // A step's update tiles join the step's launch (k_upd_diag, one workgroup
 // per CU at its LDS) up to this many; more run as their own launch.
-constexpr int kFuseTiles = 256;
+constexpr int kFuseTiles = 1024;
 
 // Backward solve steps with at most this many row chunks split each chunk
 // into four 16-column tasks (shorter tasks where the step is latency-bound).
