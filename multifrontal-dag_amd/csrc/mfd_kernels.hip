@@ -607,14 +607,22 @@ __device__ __forceinline__ void gemm_task(T* pool, const GemmTask& t, TileLds<T>
 template <typename T, int KIND>
 __global__ __launch_bounds__(kThreads) void k_gemm(T* pool, const GemmTask* tasks)
 {
-    // modes 0 and 1 only: the staged result Cs is written after the last
-    // read of the operand staging L, so the two share LDS (four workgroups
-    // per CU instead of three)
-    __shared__ union {
-        TileLds<T> L;
-        T Cs[kTile][kTile + 1];
-    } u;
-    gemm_task<T>(pool, tasks[blockIdx.x], u.L, u.Cs);
+    if constexpr (KIND == 0) {
+        // panel TRSM tiles (modes 1 and 2): mode 2 keeps its first product
+        // in Cs while staging the inverse through L, so the two are separate
+        __shared__ TileLds<T> L;
+        __shared__ T Cs[kTile][kTile + 1];
+        gemm_task<T>(pool, tasks[blockIdx.x], L, Cs);
+    } else {
+        // modes 0 and 1: the staged result Cs is written after the last read
+        // of the operand staging L, so the two share LDS (four workgroups per
+        // CU instead of three)
+        __shared__ union {
+            TileLds<T> L;
+            T Cs[kTile][kTile + 1];
+        } u;
+        gemm_task<T>(pool, tasks[blockIdx.x], u.L, u.Cs);
+    }
 }
 
 // One launch per panel step s of the large fronts: GemmTasks from workgroup

@@ -51,6 +51,10 @@ struct Launch {
 // per CU at its LDS) up to this many; more run as their own launch.
 constexpr int kFuseTiles = 1024;
 
+// Step launches with at least this many diagonal tasks run their GEMM
+// tiles (TRSM and fused update tiles) as a second launch (k_gemm).
+constexpr int kApartDiag = 512;
+
 // Backward solve steps with at most this many row chunks split each chunk
 // into four 16-column tasks (shorter tasks where the step is latency-bound).
 constexpr int kSplitChunks = 256;
@@ -554,13 +558,22 @@ void build_plan(mfd_handle* h)
                 h->diag.push_back(DiagTask{s, k0 + kPanel, std::min(kPanel, d.c - k0 - kPanel), kPanel, pend, 0, 0, 0});
             }
             const int64_t ng = (fuse ? f4u : f3) - f1, nd = (int64_t)h->diag.size() - f2;
+            // many diagonal tasks (many fronts at this level): the GEMM tiles
+            // go to a launch of their own, where they are not held to the
+            // diagonal tasks' one workgroup per CU
+            const bool apart = nd >= kApartDiag && ng > 0;
             if (ng + nd > 0) {
-                Launch l{MFD_K_POTRF, f1, (int)ng};
+                Launch l{MFD_K_POTRF, f1, apart ? 0 : (int)ng};
                 l.fused = 1;
                 l.dfirst = f2;
                 l.dcount = (int)nd;
                 l.wait_id = prev_side;  // panel t-2's update of block column t
                 h->fseq.push_back(l);
+                if (apart) {
+                    Launch g{MFD_K_TRSM, f1, (int)ng};
+                    g.wait_id = prev_side;
+                    h->fseq.push_back(g);
+                }
             }
             prev_side = fuse ? -1 : nside++;
         }
@@ -748,7 +761,7 @@ template <typename T> void enqueue_factor(mfd_handle* h, bool profile)
     launch_scatter_a<T>(pool, h->d_a, h->d_dst, h->d_src, (int64_t)h->S.snidx.size(), st);
     mark(0);
     // Side launches fork from the main stream's current position and run
-    // beside it; a step's launch joins the side launch it depends on, and
+    // beside it; a step's launches join the side launch they depend on, and
     // any other launch joins every side launch before it.
     const bool two = !profile && h->side;
     int last_side = -1, joined = -1;
@@ -761,7 +774,7 @@ template <typename T> void enqueue_factor(mfd_handle* h, bool profile)
             last_side = l.side_id;
             continue;
         }
-        if (two && l.kind != MFD_K_POTRF && last_side > joined) {
+        if (two && l.kind != MFD_K_POTRF && l.kind != MFD_K_TRSM && last_side > joined) {
             (void)hipStreamWaitEvent(st, h->sev[last_side & 1], 0);
             joined = last_side;
         } else if (two && l.wait_id > joined) {
@@ -1277,6 +1290,20 @@ int mfd_get_kernel_work(mfd_handle* h, int k, int s, double* flops, double* byte
     if (!h->analyzed) return MFD_ERR_STATE;
     double f = 0, b = 0;
     auto tri = [](double x) { return x * (x + 1) / 2; };
+    // a GEMM tile of a step launch
+    auto step_tile = [&](const GemmTask& g) {
+        if (g.mode == 1) {  // C = C L^{-T}
+            f += (double)g.m * g.n * g.n;
+            b += s * (2.0 * g.m * g.n + (double)g.n * g.n);
+        } else if (g.mode == 2) {  // C = (C - A B^T) L^{-T}
+            f += 2.0 * g.m * g.n * g.k + (double)g.m * g.n * g.n;
+            b += s * ((double)g.m * g.k + (double)g.n * g.k + 2.0 * g.m * g.n + (double)g.n * g.n);
+        } else {
+            double e = g.tri ? (double)g.n * g.m - tri(g.n - 1) : (double)g.m * g.n;
+            f += 2.0 * g.k * e;
+            b += s * ((double)g.m * g.k + (double)g.n * g.k + 2.0 * e);
+        }
+    };
     switch (k) {
     case MFD_K_ASSEMBLE: {
         double nnz = (double)h->S.snidx.size();
@@ -1310,30 +1337,13 @@ int mfd_get_kernel_work(mfd_handle* h, int k, int s, double* flops, double* byte
                 b += s * (2.0 * tri(t.kb) + kb * kb + 2.0 * kb * kp + kp * kp + kpp * (kb + kp));
             }
             if (!l.fused) continue;
-            for (int64_t i = l.first; i < l.first + l.count; ++i) {
-                const GemmTask& g = h->gemm[i];
-                if (g.mode == 1) {  // C = C L^{-T}
-                    f += (double)g.m * g.n * g.n;
-                    b += s * (2.0 * g.m * g.n + (double)g.n * g.n);
-                } else if (g.mode == 2) {  // C = (C - A B^T) L^{-T}
-                    f += 2.0 * g.m * g.n * g.k + (double)g.m * g.n * g.n;
-                    b += s * ((double)g.m * g.k + (double)g.n * g.k + 2.0 * g.m * g.n + (double)g.n * g.n);
-                } else {
-                    double e = g.tri ? (double)g.n * g.m - tri(g.n - 1) : (double)g.m * g.n;
-                    f += 2.0 * g.k * e;
-                    b += s * ((double)g.m * g.k + (double)g.n * g.k + 2.0 * e);
-                }
-            }
+            for (int64_t i = l.first; i < l.first + l.count; ++i) step_tile(h->gemm[i]);
         }
         break;
-    case MFD_K_TRSM:  // rows below a diagonal block times its inverse
+    case MFD_K_TRSM:  // a step's GEMM tiles launched apart from its diagonal tasks
         for (const Launch& l : h->fseq) {
             if (l.kind != k) continue;
-            for (int64_t i = l.first; i < l.first + l.count; ++i) {
-                const GemmTask& g = h->gemm[i];
-                f += (double)g.m * g.n * g.n;
-                b += s * (2.0 * g.m * g.n + (double)g.n * g.n);
-            }
+            for (int64_t i = l.first; i < l.first + l.count; ++i) step_tile(h->gemm[i]);
         }
         break;
     case MFD_K_UPDATE:
