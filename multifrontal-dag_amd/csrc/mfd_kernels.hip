@@ -1097,15 +1097,20 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zp
     const SolveTask t = tasks[blockIdx.x];
     const NodeDev nd = nodes[t.node];
     const int n = nd.n, c = nd.c, kb = t.kb, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    // Ls[k][i]: column k0 + k of L, row i of the current 64-row block (chunk
-    // task); LI: the diagonal block's inverse (block task)
+    // block and chunk tasks use disjoint LDS layouts in one union (two
+    // workgroups per CU)
     __shared__ union {
-        double Ls[64][65];
-        double LI[64][65];
+        struct {
+            double LI[64][65];             // the diagonal block's inverse
+            double Ln[64][65];             // Ln[k][i] = L[k0 + 64 + i][k0 + k]
+            double zt[64][kRhsGroup];
+            double Xn[64][kRhsGroup + 1];  // x of block t+1
+        } b;
+        struct {
+            double Ls[64][65];             // L columns, staged rows
+            double Xs[256][kRhsGroup + 1]; // x of the chunk's rows
+        } c;
     } u;
-    __shared__ double Xs[256][kRhsGroup + 1];
-    __shared__ double Ln[64][65];  // Ln[k][i] = L[k0 + 64 + i][k0 + k], block task
-    __shared__ double zt[64][kRhsGroup];
     double* Z = zpool + nd.z * nrhs;
     double* Y = Z + (int64_t)n * nrhs;
     const T* Lb = pool + nd.front + (int64_t)t.k0 * n;
@@ -1128,7 +1133,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zp
             const int nr = min(kRhsGroup, nrhs - r0);
 #pragma unroll
             for (int r = 0; r < kRhsGroup; ++r)
-                Xs[tid][r] = (live && r < nr) ? (row < c ? Y[row + (int64_t)(r0 + r) * c] : Z[row + (int64_t)(r0 + r) * n])
+                u.c.Xs[tid][r] = (live && r < nr) ? (row < c ? Y[row + (int64_t)(r0 + r) * c] : Z[row + (int64_t)(r0 + r) * n])
                                               : 0.0;
             acc_t acc = {0, 0, 0, 0};
             const int rc = lane & 15;
@@ -1136,14 +1141,14 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zp
                 __syncthreads();
                 if (wv == sb) {
 #pragma unroll
-                    for (int k = 0; k < 64; ++k) u.Ls[k][lane] = l[k];
+                    for (int k = 0; k < 64; ++k) u.c.Ls[k][lane] = l[k];
                 }
                 __syncthreads();
 #pragma unroll
                 for (int q = 0; q < 16; ++q) {
                     const int rr = 4 * q + (lane >> 4);
-                    const double a = u.Ls[16 * wv + rc][rr];
-                    const double b = rc < nr ? Xs[64 * sb + rr][rc] : 0.0;
+                    const double a = u.c.Ls[16 * wv + rc][rr];
+                    const double b = rc < nr ? u.c.Xs[64 * sb + rr][rc] : 0.0;
                     acc = Mfma<double>::mma(a, b, acc);
                 }
             }
@@ -1172,14 +1177,14 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zp
         T l[16];
 #pragma unroll
         for (int j = 0; j < 16; ++j) l[j] = ld_off(Lb, (unsigned)(rowc + min(col0 + j, kb - 1) * n));
-        double(*Lc)[kThreads + 4] = reinterpret_cast<double(*)[kThreads + 4]>(&u.Ls[0][0]);   // 16 x 260
-        double(*red)[16][17] = reinterpret_cast<double(*)[16][17]>(&u.Ls[0][0]);              // 4 x 16 x 17
+        double(*Lc)[kThreads + 4] = reinterpret_cast<double(*)[kThreads + 4]>(&u.c.Ls[0][0]);   // 16 x 260
+        double(*red)[16][17] = reinterpret_cast<double(*)[16][17]>(&u.c.Ls[0][0]);              // 4 x 16 x 17
         const int rc = lane & 15;
         for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
             const int nr = min(kRhsGroup, nrhs - r0);
 #pragma unroll
             for (int r = 0; r < kRhsGroup; ++r)
-                Xs[tid][r] = (live && r < nr) ? (row < c ? Y[row + (int64_t)(r0 + r) * c] : Z[row + (int64_t)(r0 + r) * n])
+                u.c.Xs[tid][r] = (live && r < nr) ? (row < c ? Y[row + (int64_t)(r0 + r) * c] : Z[row + (int64_t)(r0 + r) * n])
                                               : 0.0;
 #pragma unroll
             for (int j = 0; j < 16; ++j) Lc[j][tid] = (double)l[j];
@@ -1189,7 +1194,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zp
             for (int q = 0; q < 16; ++q) {
                 const int rr = 64 * wv + 4 * q + (lane >> 4);
                 const double a = Lc[rc][rr];
-                const double b = rc < nr ? Xs[rr][rc] : 0.0;
+                const double b = rc < nr ? u.c.Xs[rr][rc] : 0.0;
                 acc = Mfma<double>::mma(a, b, acc);
             }
             __syncthreads();
@@ -1245,34 +1250,34 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zp
 #pragma unroll
     for (int p = 0; p < 16; ++p) {
         const int e = tid + kThreads * p;
-        u.LI[e & 63][e >> 6] = li[p];
-        Ln[e >> 6][e & 63] = ln[p];
+        u.b.LI[e & 63][e >> 6] = li[p];
+        u.b.Ln[e >> 6][e & 63] = ln[p];
     }
     for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
         const int nr = min(kRhsGroup, nrhs - r0);
         if (r0 > 0) load_zx(r0);
-        // zt, and x of block t+1 staged in Xs[0..kn)
+        // zt, and x of block t+1 staged in u.b.Xn[0..kn)
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int e = tid + kThreads * h;
-            zt[e & 63][e >> 6] = zv[h];
-            Xs[e & 63][e >> 6] = xv[h];
+            u.b.zt[e & 63][e >> 6] = zv[h];
+            u.b.Xn[e & 63][e >> 6] = xv[h];
         }
         __syncthreads();
-        // z_k -= sum_i L[k0 + 64 + i][k0 + k] x_{t+1}[i]; Ln, LI, Xs and zt
+        // z_k -= sum_i L[k0 + 64 + i][k0 + k] x_{t+1}[i]; Ln, LI, Xn and zt
         // are zero past kn / kb, so the sums run over all 64 entries,
         // unrolled, with four partial sums
         for (int r = wv; r < nr; r += 4) {
             double acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int i = 0; i < 64; ++i) acc[i & 3] += Ln[lane][i] * Xs[i][r];
-            if (lane < kb) zt[lane][r] -= (acc[0] + acc[1]) + (acc[2] + acc[3]);
+            for (int i = 0; i < 64; ++i) acc[i & 3] += u.b.Ln[lane][i] * u.b.Xn[i][r];
+            if (lane < kb) u.b.zt[lane][r] -= (acc[0] + acc[1]) + (acc[2] + acc[3]);
         }
         __syncthreads();
         for (int r = wv; r < nr; r += 4) {
             double acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int m = 0; m < 64; ++m) acc[m & 3] += u.LI[lane][m] * zt[m][r];
+            for (int m = 0; m < 64; ++m) acc[m & 3] += u.b.LI[lane][m] * u.b.zt[m][r];
             const double v = (acc[0] + acc[1]) + (acc[2] + acc[3]);
             if (lane < kb) {
                 Y[t.k0 + lane + (int64_t)(r0 + r) * c] = v;
