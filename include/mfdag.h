@@ -60,12 +60,12 @@ typedef struct mfd_handle mfd_handle;
 
 /* kernel classes reported by mfd_get_kernel_time */
 #define MFD_K_ASSEMBLE 0
-#define MFD_K_EXTEND_ADD 1
+#define MFD_K_EXTEND_ADD 1  /* extend-add of small children (n <= 64) */
 #define MFD_K_SMALL_FRONT 2
 #define MFD_K_POTRF 3       /* per panel step: next diagonal block (its panel updates, Cholesky, inverse), panel TRSM tiles, small updates */
 #define MFD_K_TRSM 4        /* a step's GEMM tiles (panel TRSM, small updates) launched apart from its diagonal tasks (levels with many fronts) */
 #define MFD_K_UPDATE 5      /* in-panel trailing update (GEMM/SYRK, K = panel width) */
-#define MFD_K_SCHUR 6       /* Schur-complement update (SYRK, K = front's pivot count) */
+#define MFD_K_SCHUR 6       /* Schur-complement update (SYRK, K = front's pivot count) and its extend-add into the parent */
 #define MFD_K_SOLVE_FWD 7
 #define MFD_K_SOLVE_BWD 8
 #define MFD_K_REFINE 9

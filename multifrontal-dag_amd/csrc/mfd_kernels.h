@@ -24,7 +24,10 @@ void launch_upd_diag(T* pool, const NodeDev* nodes, const DiagTask* diags, int n
                      int* info, hipStream_t st);
 // kind: 0 panel TRSM, 1 in-panel update, 2 Schur update (separate kernel
 // instances, same code)
-template <typename T> void launch_gemm(T* pool, const GemmTask* tasks, int count, int kind, hipStream_t st);
+// kind 0: panel TRSM tiles; 1: in-panel update; 2: Schur update (mode 3
+// tiles read local_index from lidx)
+template <typename T>
+void launch_gemm(T* pool, const GemmTask* tasks, int count, int kind, const int* lidx, hipStream_t st);
 template <typename T> void launch_gemm128(T* pool, const GemmTask* tasks, int count, hipStream_t st);
 void launch_fwd_init(double* zpool, const double* b, int ldb, int nrhs, const NodeDev* nodes, const int2* tasks,
                      int count, hipStream_t st);

@@ -577,7 +577,8 @@ __device__ __forceinline__ void stage_acc(const typename Mfma<T>::acc_t (&c)[2][
 // The result goes through LDS so the write of C is coalesced along columns
 // (written straight from the accumulators it would be 32-byte pieces).
 template <typename T>
-__device__ __forceinline__ void gemm_task(T* pool, const GemmTask& t, TileLds<T>& L, T (*Cs)[kTile + 1])
+__device__ __forceinline__ void gemm_task(T* pool, const GemmTask& t, TileLds<T>& L, T (*Cs)[kTile + 1],
+                                          const int* lidx = nullptr)
 {
     tile_product<T>(pool + t.a, t.lda, pool + t.b, t.ldb, t.mode == 1, t.m, t.n, t.k, L, Cs);
     T* C = pool + t.c;
@@ -596,7 +597,18 @@ __device__ __forceinline__ void gemm_task(T* pool, const GemmTask& t, TileLds<T>
         return;
     }
     if (row >= t.m) return;
-    if (t.mode == 0) {
+    if (t.mode == 3) {
+        // the child's update entry, and the same value added into the parent
+        // (one launch holds at most one child of each parent, and local_index
+        // is injective within a child: no two lanes add into one entry)
+        T* P = pool + t.p + (lidx[t.li + row] - 1);
+        for (int col = tid >> 6; col < t.n; col += 4)
+            if (!t.tri || row >= col) {
+                const T v = C[row + (int64_t)col * t.ldc] - Cs[col][row];
+                C[row + (int64_t)col * t.ldc] = v;
+                P[(int64_t)(lidx[t.lj + col] - 1) * t.np] += v;
+            }
+    } else if (t.mode == 0) {
         for (int col = tid >> 6; col < t.n; col += 4)
             if (!t.tri || row >= col) C[row + (int64_t)col * t.ldc] -= Cs[col][row];
     } else {
@@ -605,7 +617,7 @@ __device__ __forceinline__ void gemm_task(T* pool, const GemmTask& t, TileLds<T>
 }
 
 template <typename T, int KIND>
-__global__ __launch_bounds__(kThreads) void k_gemm(T* pool, const GemmTask* tasks)
+__global__ __launch_bounds__(kThreads) void k_gemm(T* pool, const GemmTask* tasks, const int* lidx)
 {
     if constexpr (KIND == 0) {
         // panel TRSM tiles (modes 1 and 2): mode 2 keeps its first product
@@ -621,7 +633,7 @@ __global__ __launch_bounds__(kThreads) void k_gemm(T* pool, const GemmTask* task
             TileLds<T> L;
             T Cs[kTile][kTile + 1];
         } u;
-        gemm_task<T>(pool, tasks[blockIdx.x], u.L, u.Cs);
+        gemm_task<T>(pool, tasks[blockIdx.x], u.L, u.Cs, lidx);
     }
 }
 
@@ -1456,15 +1468,16 @@ void launch_upd_diag(T* pool, const NodeDev* nodes, const DiagTask* diags, int n
         hipLaunchKernelGGL(k_upd_diag<T>, dim3(ndiag + ntiles), dim3(kThreads), 0, st, pool, nodes, diags, ndiag, tiles,
                            info);
 }
-template <typename T> void launch_gemm(T* pool, const GemmTask* tasks, int count, int kind, hipStream_t st)
+template <typename T>
+void launch_gemm(T* pool, const GemmTask* tasks, int count, int kind, const int* lidx, hipStream_t st)
 {
     if (count <= 0) return;
     if (kind == 0)
-        hipLaunchKernelGGL((k_gemm<T, 0>), dim3(count), dim3(kThreads), 0, st, pool, tasks);
+        hipLaunchKernelGGL((k_gemm<T, 0>), dim3(count), dim3(kThreads), 0, st, pool, tasks, lidx);
     else if (kind == 1)
-        hipLaunchKernelGGL((k_gemm<T, 1>), dim3(count), dim3(kThreads), 0, st, pool, tasks);
+        hipLaunchKernelGGL((k_gemm<T, 1>), dim3(count), dim3(kThreads), 0, st, pool, tasks, lidx);
     else
-        hipLaunchKernelGGL((k_gemm<T, 2>), dim3(count), dim3(kThreads), 0, st, pool, tasks);
+        hipLaunchKernelGGL((k_gemm<T, 2>), dim3(count), dim3(kThreads), 0, st, pool, tasks, lidx);
 }
 template <typename T> void launch_gemm128(T* pool, const GemmTask* tasks, int count, hipStream_t st)
 {
@@ -1524,7 +1537,7 @@ void launch_scatter_rows(double* out, int ld_out, const double* in, int ld_in, c
     template void launch_small_front<T>(T*, const NodeDev*, const int*, int, int*, hipStream_t);                       \
     template void launch_diag<T>(T*, const NodeDev*, const DiagTask*, int, int*, hipStream_t);                        \
     template void launch_upd_diag<T>(T*, const NodeDev*, const DiagTask*, int, const GemmTask*, int, int*, hipStream_t); \
-    template void launch_gemm<T>(T*, const GemmTask*, int, int, hipStream_t);                                          \
+    template void launch_gemm<T>(T*, const GemmTask*, int, int, const int*, hipStream_t);                                          \
     template void launch_gemm128<T>(T*, const GemmTask*, int, hipStream_t);                                            \
     template void launch_fwd_step<T>(const T*, double*, int, const NodeDev*, const SolveTask*, int, hipStream_t);       \
     template void launch_bwd_step<T>(const T*, double*, double*, double*, int, int, const NodeDev*, const SolveTask*,  \

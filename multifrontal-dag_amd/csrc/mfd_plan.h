@@ -62,14 +62,18 @@ struct SolveTask {
 // (column-major), whose transpose is L^{-1};
 // mode 2: C = (C - A B^T) D^T with D = L^{-1} stored transposed at d
 // (ld 64) and K = kd: the panel TRSM preceded by the previous panel's
-// update of the same rows.  Offsets are into the front pool; tri != 0 keeps
-// only the lower triangle (row >= col) of C.
+// update of the same rows;
+// mode 3: mode 0, and each updated entry also added into the parent front
+// at p (order np) at the rows and columns local_index gives (lidx offsets
+// li for the tile's rows, lj for its columns): the Schur update with the
+// extend-add of its result.  Offsets are into the front pool; tri != 0
+// keeps only the lower triangle (row >= col) of C.
 struct GemmTask {
-    int64_t c, a, b, d;
+    int64_t c, a, b, d, p;
     int ldc, lda, ldb;
     int m, n, k;
     int tri, mode;
-    int kd, pad;
+    int kd, np, li, lj;
 };
 
 }  // namespace mfd

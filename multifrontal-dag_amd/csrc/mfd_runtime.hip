@@ -418,6 +418,10 @@ void build_plan(mfd_handle* h)
         std::vector<int> pos(h->level_ptr.begin(), h->level_ptr.end() - 1);
         for (int s = 0; s < m; ++s) h->level_nodes[pos[lev[s]]++] = s;
     }
+    // rank of each node among its parent's children (0 for roots)
+    std::vector<int> rank(m, 0);
+    for (int s = 0; s < m; ++s)
+        for (size_t r = 0; r < S.children[s].size(); ++r) rank[S.children[s][r] - 1] = (int)r;
     h->child_ptr.assign(m + 1, 0);
     h->child_list.clear();
     for (int s = 0; s < m; ++s) {
@@ -440,7 +444,9 @@ void build_plan(mfd_handle* h)
     for (int L = 0; L < h->num_levels; ++L) {
         const int* lv = h->level_nodes.data() + h->level_ptr[L];
         const int cnt = h->level_ptr[L + 1] - h->level_ptr[L];
-        // extend-add: the r-th child of every parent of this level in one launch
+        // extend-add of the small children (a large child adds its update
+        // into the parent in its own Schur launch): the r-th child of every
+        // parent of this level in one launch
         size_t maxch = 0;
         for (int k = 0; k < cnt; ++k) maxch = std::max(maxch, S.children[lv[k]].size());
         for (size_t r = 0; r < maxch; ++r) {
@@ -449,6 +455,7 @@ void build_plan(mfd_handle* h)
                 const auto& chs = S.children[lv[k]];
                 if (chs.size() <= r) continue;
                 int ch = chs[r] - 1, nt = (S.w[ch] + kTile - 1) / kTile;
+                if (h->nodes[ch].n > kSmallFront) continue;
                 for (int ti = 0; ti < nt; ++ti)
                     for (int tj = 0; tj <= ti; ++tj) h->ext.push_back(ExtTask{ch, ti, tj, 0});
             }
@@ -577,24 +584,41 @@ void build_plan(mfd_handle* h)
             }
             prev_side = fuse ? -1 : nside++;
         }
-        int64_t f4 = (int64_t)h->gemm.size();
-        for (int s : large) {
-            const NodeDev& d = h->nodes[s];
-            for (int c0 = d.c; c0 < d.n; c0 += kWide)
-                for (int r0 = c0; r0 < d.n; r0 += kWide) {
-                    GemmTask g{};
-                    g.c = d.front + r0 + (int64_t)c0 * d.n;
-                    g.a = d.front + r0;
-                    g.b = d.front + c0;
-                    g.ldc = g.lda = g.ldb = d.n;
-                    g.m = std::min(kWide, d.n - r0);
-                    g.n = std::min(kWide, d.n - c0);
-                    g.k = d.c;
-                    g.tri = r0 == c0;
-                    h->gemm.push_back(g);
-                }
+        // Schur update with the extend-add of its result into the parent,
+        // one launch per child rank (the r-th children of their parents), so
+        // no two tiles of a launch add into one parent entry
+        int maxrank = 0;
+        for (int s : large) maxrank = std::max(maxrank, rank[s]);
+        for (int r = 0; r <= maxrank; ++r) {
+            int64_t f4 = (int64_t)h->gemm.size();
+            for (int s : large) {
+                if (rank[s] != r) continue;
+                const NodeDev& d = h->nodes[s];
+                const NodeDev* pa = d.parent >= 0 ? &h->nodes[d.parent] : nullptr;
+                for (int c0 = d.c; c0 < d.n; c0 += kWide)
+                    for (int r0 = c0; r0 < d.n; r0 += kWide) {
+                        GemmTask g{};
+                        g.c = d.front + r0 + (int64_t)c0 * d.n;
+                        g.a = d.front + r0;
+                        g.b = d.front + c0;
+                        g.ldc = g.lda = g.ldb = d.n;
+                        g.m = std::min(kWide, d.n - r0);
+                        g.n = std::min(kWide, d.n - c0);
+                        g.k = d.c;
+                        g.tri = r0 == c0;
+                        if (pa) {
+                            g.mode = 3;
+                            g.p = pa->front;
+                            g.np = pa->n;
+                            g.li = d.lidx + (r0 - d.c);
+                            g.lj = d.lidx + (c0 - d.c);
+                        }
+                        h->gemm.push_back(g);
+                    }
+            }
+            push(MFD_K_SCHUR, f4, (int64_t)h->gemm.size(), kUseWide);
         }
-        push(MFD_K_SCHUR, f4, (int64_t)h->gemm.size(), kUseWide);
+
     }
     build_solve_plan(h);
 }
@@ -734,7 +758,7 @@ template <typename T> void enqueue_launch(mfd_handle* h, const Launch& l, hipStr
             launch_gemm128<T>(pool, h->d_gemm + l.first, l.count, st);
         else
             launch_gemm<T>(pool, h->d_gemm + l.first, l.count,
-                           l.kind == MFD_K_TRSM ? 0 : l.kind == MFD_K_UPDATE ? 1 : 2, st);
+                           l.kind == MFD_K_TRSM ? 0 : l.kind == MFD_K_UPDATE ? 1 : 2, h->d_lidx, st);
         break;
     default: break;
     }
@@ -1310,9 +1334,9 @@ int mfd_get_kernel_work(mfd_handle* h, int k, int s, double* flops, double* byte
         b = s * (double)h->pool_elems + nnz * (8 + 8 + 8 + s);
         break;
     }
-    case MFD_K_EXTEND_ADD:
+    case MFD_K_EXTEND_ADD:  // small children (large ones add in their Schur launch)
         for (const auto& d : h->nodes)
-            if (d.parent >= 0) {
+            if (d.parent >= 0 && d.n <= kSmallFront) {
                 double e = tri(d.w);
                 f += e;
                 b += e * 3 * s + 4.0 * d.w;
@@ -1355,6 +1379,10 @@ int mfd_get_kernel_work(mfd_handle* h, int k, int s, double* flops, double* byte
                 double e = g.tri ? (double)g.n * g.m - tri(g.n - 1) : (double)g.m * g.n;
                 f += 2.0 * g.k * e;
                 b += s * ((double)g.m * g.k + (double)g.n * g.k + 2.0 * e);
+                if (g.mode == 3) {  // the extend-add: e adds into the parent, plus the index maps
+                    f += e;
+                    b += 2.0 * s * e + 4.0 * (g.m + g.n);
+                }
             }
         }
         break;
