@@ -597,20 +597,36 @@ __device__ __forceinline__ void gemm_task(T* pool, const GemmTask& t, TileLds<T>
         return;
     }
     if (row >= t.m) return;
-    if (t.mode == 3) {
-        // the child's update entry, and the same value added into the parent
+    if (t.mode == 3 || t.mode == 0) {
+        // C -= the product; mode 3 also adds the new entry into the parent
         // (one launch holds at most one child of each parent, and local_index
-        // is injective within a child: no two lanes add into one entry)
-        T* P = pool + t.p + (lidx[t.li + row] - 1);
-        for (int col = tid >> 6; col < t.n; col += 4)
-            if (!t.tri || row >= col) {
-                const T v = C[row + (int64_t)col * t.ldc] - Cs[col][row];
-                C[row + (int64_t)col * t.ldc] = v;
-                P[(int64_t)(lidx[t.lj + col] - 1) * t.np] += v;
+        // is injective within a child: no two lanes add into one entry).
+        // Four columns at a time, their loads (from clamped columns) all
+        // issued before the stores, which may alias them as far as the
+        // compiler knows.
+        const bool ea = t.mode == 3;
+        T* P = ea ? pool + t.p + (lidx[t.li + row] - 1) : C;
+        for (int c0 = tid >> 6; c0 < t.n; c0 += 16) {
+            T cv[4], pv[4];
+            int64_t po[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int col = min(c0 + 4 * q, t.n - 1);
+                cv[q] = C[row + (int64_t)col * t.ldc];
+                po[q] = ea ? (int64_t)(lidx[t.lj + col] - 1) * t.np : 0;
             }
-    } else if (t.mode == 0) {
-        for (int col = tid >> 6; col < t.n; col += 4)
-            if (!t.tri || row >= col) C[row + (int64_t)col * t.ldc] -= Cs[col][row];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) pv[q] = ea ? P[po[q]] : T(0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int col = c0 + 4 * q;
+                if (col < t.n && (!t.tri || row >= col)) {
+                    const T v = cv[q] - Cs[col][row];
+                    C[row + (int64_t)col * t.ldc] = v;
+                    if (ea) P[po[q]] = pv[q] + v;
+                }
+            }
+        }
     } else {
         for (int col = tid >> 6; col < t.n; col += 4) C[row + (int64_t)col * t.ldc] = Cs[col][row];
     }
