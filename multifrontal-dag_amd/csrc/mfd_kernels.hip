@@ -7,13 +7,15 @@
 //
 // Reference operations these kernels replace (paths relative to the
 // reference root, src/main/):
-//   k_scatter_a     set_zero + set_coefficient        zero_setter_m.f90:12-31, coefficient_setter_m.f90:15-50
+//   k_fill_lower,   set_zero + set_coefficient        zero_setter_m.f90:12-31, coefficient_setter_m.f90:15-50
+//   k_scatter_a
 //   k_extend_add    extend_add_tri / extend_add_rect  seq_factorize/kernel/extend_add_kernel_m.f90:6-34
+//                   (small children; k_gemm mode 3 for the others)
 //   k_small_front   the per-front tile loop           seq_factorize_m.f90:31-79
-//   k_diag          DPOTRF("U") of a diagonal block   seq_factorize/kernel/factorize_kernel_m.f90:4-9,
-//                                                     border_kernel_m.f90:9-19
-//   k_gemm          DTRSM (mode 1), DSYRK / DGEMM     factorize_kernel_m.f90:11-33, border_kernel_m.f90:21-30,
-//                   updates (mode 0)                  border_subroutines_m.f90:41-73
+//   k_diag,         DPOTRF("U") of a diagonal block,  seq_factorize/kernel/factorize_kernel_m.f90:4-16,
+//   k_upd_diag      with its panel's DTRSM tiles      border_kernel_m.f90:9-30
+//   k_gemm          DTRSM (modes 1, 2), DSYRK / DGEMM factorize_kernel_m.f90:11-33, border_kernel_m.f90:21-30,
+//                   updates (modes 0, 3)              border_subroutines_m.f90:41-73
 //   k_fwd_*         seq_forward (scatter_add, TRSV, GEMV)   seq_solve/seq_forward_m.f90:14-81
 //   k_bwd_*         seq_backward (gather, GEMV, TRSV)       seq_solve/seq_backward_m.f90:14-131
 #include <hip/hip_runtime.h>
@@ -348,10 +350,11 @@ __global__ __launch_bounds__(kThreads) void k_scatter_a(T* pool, const double* a
 
 // ------------------------------------------------------------- extend-add
 // One 64 x 64 tile (ti >= tj) of the child's update matrix, lower part only,
-// added into the parent front at the rows/columns local_index gives.  The
-// map is injective within a child and one launch carries at most one child
-// per parent, so the read-modify-write is race-free and the parent receives
-// its children in ascending order, as in seq_factorize_m.f90:21-27.
+// added into the parent front at the rows/columns local_index gives (used
+// for the children of order <= 64; the Schur tiles of larger children add
+// their results themselves).  The map is injective within a child and one
+// launch carries at most one child per parent, so the read-modify-write is
+// race-free; a launch per child rank keeps the order of the additions fixed.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_extend_add(T* pool, const int* lidx, const NodeDev* nodes,
                                                          const ExtTask* tasks)
