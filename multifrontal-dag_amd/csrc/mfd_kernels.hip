@@ -270,9 +270,15 @@ __device__ __forceinline__ void factor_dense64_loaded(Front64<T>& S, T* g, int64
 {
     const int tid = threadIdx.x, wv = tid >> 6;
     const int nblk = (c + 15) >> 4, nall = (n + 15) >> 4;
+    // Row i of the inverse's 16 x 16 blocks (X_ij, j < i) needs only the
+    // panels up to i and the rows above it, so row b - 1 is formed by waves
+    // 1.. while wave 0 runs panel b's chain; the last row follows the loop.
     for (int b = 0; b < nblk; ++b) {
         const int b16 = 16 * b, cb = min(16, c - b16);
-        if (wv == 0) panel16<T>(S, b16, cb, n, info, col0);
+        if (wv == 0)
+            panel16<T>(S, b16, cb, n, info, col0);
+        else if (b >= 2 && wv <= b - 1)
+            inverse16<T>(S, b - 1, wv - 1, wv);
         __syncthreads();
         int idx = 0;
         for (int J = b + 1; J < nall; ++J)
@@ -280,10 +286,8 @@ __device__ __forceinline__ void factor_dense64_loaded(Front64<T>& S, T* g, int64
                 if ((idx & 3) == wv) trailing16<T>(S, b16, cb, I, J);
         __syncthreads();
     }
-    for (int d = 1; d < nblk; ++d) {
-        for (int j = wv; j + d < nblk; j += 4) inverse16<T>(S, j + d, j, wv);
-        __syncthreads();
-    }
+    if (nblk >= 2 && wv < nblk - 1) inverse16<T>(S, nblk - 1, wv, wv);
+    __syncthreads();
     for (int e = tid; e < 64 * 64; e += kThreads) {
         const int row = e & 63, col = e >> 6;
         if (row < n && col <= row) g[row + (int64_t)col * ld] = S.A[col][row];
