@@ -155,9 +155,11 @@ typedef struct mfd_stats {
     double analyze_ms, assemble_ms, factor_ms, solve_ms;  /* last calls, wall (HIP events) */
     double factor_gflops;        /* cost / factor time (cost_calculator_m.f90:9-20) */
     int precision;               /* of the current factor */
-    int refine_iters;            /* last solve */
-    double backward_error;       /* last solve: max over rhs of ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf) */
-    double refine_residual;      /* last solve: max over rhs of ||b - A x||_2 / ||x||_2 (example.f90:147-180) */
+    int refine_iters;            /* last solve (< 0: -1 - iterations before the FP64 fallback) */
+    /* last FP32_IR solve, for its final x (-1 after an FP64 solve, which
+       computes no residual): */
+    double backward_error;       /* max over rhs of ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf) */
+    double refine_residual;      /* max over rhs of ||b - A x||_2 / ||x||_2 (example.f90:147-180) */
     int64_t front_bytes;         /* device bytes of the front pool */
     int64_t device_bytes;        /* all device bytes held by the handle */
     int num_levels, num_launches_factor, num_launches_solve;
@@ -173,6 +175,21 @@ int mfd_get_kernel_time(mfd_handle* h, int k, double* total_ms, int* launches);
    bytes the class must move at minimum (each operand read once, each result
    written once) for an element size of elem_bytes. */
 int mfd_get_kernel_work(mfd_handle* h, int k, int elem_bytes, double* flops, double* bytes);
+
+/* MatrixMarket input, as the reference's driver reads its matrices
+   (example.f90:63-145; no counterpart in the hppllt module): a symmetric
+   matrix stored as its lower triangle, "row col value" lines in coordinate
+   form.  An optional banner must name a real (or integer) symmetric
+   coordinate matrix; '%' lines before the size line "n n nnz" are skipped.
+   mfd_mm_size returns n and nnz.  mfd_mm_read fills the 1-based lower CCS
+   that mfd_analyze / mfd_factorize take: ccs_col (n + 1), ccs_row and
+   ccs_val (nnz), rows ascending within each column (so the diagonal comes
+   first), an entry given twice kept twice with the later one first -- the
+   column order of example.f90:104-143.  Entries above the diagonal, indices
+   out of range, n != m, a short file or unreadable numbers are MFD_ERR_ARG;
+   a banner naming another kind of matrix is MFD_ERR_UNSUPPORTED. */
+int mfd_mm_size(const char* path, int* n, int64_t* nnz);
+int mfd_mm_read(const char* path, int n, int64_t nnz, int* ccs_col, int* ccs_row, double* ccs_val);
 
 const char* mfd_error_string(int code);
 

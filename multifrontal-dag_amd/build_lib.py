@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "lib", "libmfdag.so")
-SOURCES = ["mfd_analysis.cpp", "mfd_layout.cpp", "mfd_kernels.hip", "mfd_runtime.hip"]
+SOURCES = ["mfd_analysis.cpp", "mfd_layout.cpp", "mfd_mmio.cpp", "mfd_kernels.hip", "mfd_runtime.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 instructions are disabled for all device code.
 FLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=gfx950", "-Wall",
@@ -52,8 +52,21 @@ def build(verbose=False):
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs
     subprocess.run(cmd, check=True, capture_output=True, text=True)
     os.replace(tmp, LIB)
+    build_cli()
     build_fortran()
     return LIB
+
+
+CLI_SRC = os.path.join(HERE, "cli", "mfdag_example.cpp")
+CLI = os.path.join(HERE, "lib", "mfdag_example")
+
+
+def build_cli():
+    """The command-line driver (reads MatrixMarket, writes the one-line
+    record), linked to libmfdag.so through an $ORIGIN run path."""
+    libdir = os.path.dirname(LIB)
+    subprocess.run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), CLI_SRC, "-o", CLI,
+                    "-L", libdir, "-lmfdag", "-Wl,-rpath,$ORIGIN"], check=True, capture_output=True, text=True)
 
 
 FLANG = os.environ.get("FLANG", "/opt/rocm/llvm/bin/flang")

@@ -22,8 +22,6 @@ void launch_diag(T* pool, const NodeDev* nodes, const DiagTask* tasks, int count
 template <typename T>
 void launch_upd_diag(T* pool, const NodeDev* nodes, const DiagTask* diags, int ndiag, const GemmTask* tiles, int ntiles,
                      int* info, hipStream_t st);
-// kind: 0 panel TRSM, 1 in-panel update, 2 Schur update (separate kernel
-// instances, same code)
 // kind 0: panel TRSM tiles; 1: in-panel update; 2: Schur update (mode 3
 // tiles read local_index from lidx)
 template <typename T>
@@ -45,6 +43,11 @@ void launch_set_int(int* p, int v, hipStream_t st);
 void launch_residual(double* r, const double* b, const double* x, int n, int nrhs, const int* ptr, const int* col,
                      const int64_t* src, const double* a, hipStream_t st);
 void launch_absmax(unsigned long long* out, const double* x, int64_t n, int ld, int nrhs, hipStream_t st);
+// Sums of squares of the columns of x, as sumsq_parts(n) (<= kSumParts)
+// partials per column at part + k * sumsq_parts(n).
+constexpr int kSumParts = 256;
+int sumsq_parts(int64_t n);
+void launch_sumsq(double* part, const double* x, int64_t n, int ld, int nrhs, hipStream_t st);
 void launch_norm_inf(unsigned long long* out, int n, const int* ptr, const int64_t* src, const double* a,
                      hipStream_t st);
 void launch_axpy(double* x, const double* d, int64_t n, hipStream_t st);

@@ -1397,6 +1397,27 @@ __global__ __launch_bounds__(kThreads) void k_absmax(unsigned long long* out, co
     block_max_to(out + k, m);
 }
 
+// part[k * gridDim.x + b] = sum over block b's share of x[:, k] of x^2.
+// The caller adds the gridDim.x partials of each column in order, so the sum
+// does not depend on the timing of the blocks.
+__global__ __launch_bounds__(kThreads) void k_sumsq(double* part, const double* x, int64_t n, int ld)
+{
+    __shared__ double wsum[kThreads / 64];
+    const int k = blockIdx.y;
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        const double v = x[i + (int64_t)k * ld];
+        s += v * v;
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kThreads / 64; ++w) s += wsum[w];
+        part[(int64_t)k * gridDim.x + blockIdx.x] = s;
+    }
+}
+
 // out[0] = max over rows of sum_e |a[src[e]]|  (the infinity norm of A)
 __global__ __launch_bounds__(kThreads) void k_norm_inf(unsigned long long* out, int n, const int* ptr,
                                                        const int64_t* src, const double* a)
@@ -1432,6 +1453,11 @@ void launch_absmax(unsigned long long* out, const double* x, int64_t n, int ld, 
 {
     int g = (int)std::min<int64_t>((n + kThreads - 1) / kThreads, 256);
     if (n > 0) hipLaunchKernelGGL(k_absmax, dim3(std::max(g, 1), nrhs), dim3(kThreads), 0, st, out, x, n, ld, nrhs);
+}
+int sumsq_parts(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + kThreads - 1) / kThreads, kSumParts)); }
+void launch_sumsq(double* part, const double* x, int64_t n, int ld, int nrhs, hipStream_t st)
+{
+    if (n > 0) hipLaunchKernelGGL(k_sumsq, dim3(sumsq_parts(n), nrhs), dim3(kThreads), 0, st, part, x, n, ld);
 }
 void launch_norm_inf(unsigned long long* out, int n, const int* ptr, const int64_t* src, const double* a, hipStream_t st)
 {

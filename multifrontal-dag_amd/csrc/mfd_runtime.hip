@@ -726,7 +726,7 @@ int ensure_rhs(mfd_handle* h, int nrhs)
     MFD_TRY(dalloc(h, &h->d_xp, n * nrhs));
     MFD_TRY(dalloc(h, &h->d_r, n * nrhs));
     MFD_TRY(dalloc(h, &h->d_d, n * nrhs));
-    MFD_TRY(dalloc(h, &h->d_norms, 3 * nrhs + 1));
+    MFD_TRY(dalloc(h, &h->d_norms, 3 * nrhs + 1 + 2 * kSumParts * nrhs));
     MFD_TRY(dalloc(h, &h->d_part, h->part_slots * 64 * nrhs));
     drop_solve_graphs(h);  // captured solves refer to the old buffers
     h->nrhs_cap = nrhs;
@@ -958,19 +958,25 @@ template <typename T> int run_solve(mfd_handle* h, const double* bp, double* xp,
 }
 
 // Normwise backward error per right-hand side, max over them:
-// ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf), and the reference's
-// ||b - A x||_2 / ||x||_2 is not computed here (see the host tests).
-int backward_error(mfd_handle* h, const double* bp, const double* xp, double* r, int nrhs, double* eta)
+// eta = ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf), and res2 = the
+// max of ||b - A x||_2 / ||x||_2 (the measure of example.f90:147-180; both
+// norms are unchanged by the analysis permutation).  The sums of squares
+// come back as per-block partials added here in a fixed order.
+int backward_error(mfd_handle* h, const double* bp, const double* xp, double* r, int nrhs, double* eta, double* res2)
 {
     const int n = h->S.n;
     hipStream_t st = h->stream;
+    const int nm = 3 * nrhs + 1, parts = sumsq_parts(n);
+    double* sq = reinterpret_cast<double*>(h->d_norms + nm);
     launch_residual(r, bp, xp, n, nrhs, h->d_csr_ptr, h->d_csr_col, h->d_csr_src, h->d_a, st);
-    launch_zero_u64(h->d_norms, 3 * nrhs + 1, st);
+    launch_zero_u64(h->d_norms, nm, st);
     launch_absmax(h->d_norms, r, n, n, nrhs, st);
     launch_absmax(h->d_norms + nrhs, xp, n, n, nrhs, st);
     launch_absmax(h->d_norms + 2 * nrhs, bp, n, n, nrhs, st);
     launch_norm_inf(h->d_norms + 3 * nrhs, n, h->d_csr_ptr, h->d_csr_src, h->d_a, st);
-    std::vector<unsigned long long> v(3 * nrhs + 1);
+    launch_sumsq(sq, r, n, n, nrhs, st);
+    launch_sumsq(sq + (int64_t)parts * nrhs, xp, n, n, nrhs, st);
+    std::vector<unsigned long long> v(nm + 2 * parts * nrhs);
     HIP_TRY(hipMemcpyAsync(v.data(), h->d_norms, v.size() * 8, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     auto d = [&](int i) {
@@ -978,13 +984,22 @@ int backward_error(mfd_handle* h, const double* bp, const double* xp, double* r,
         memcpy(&x, &v[i], 8);
         return x;
     };
-    double an = d(3 * nrhs), worst = 0;
+    auto sum = [&](int first) {
+        double s = 0;
+        for (int p = 0; p < parts; ++p) s += d(first + p);
+        return s;
+    };
+    double an = d(3 * nrhs), worst = 0, worst2 = 0;
     for (int k = 0; k < nrhs; ++k) {
         double den = an * d(nrhs + k) + d(2 * nrhs + k);
         double e = den > 0 ? d(k) / den : (d(k) > 0 ? INFINITY : 0.0);
         if (!(e <= worst)) worst = e;  // NaN propagates
+        const double rr = sum(nm + k * parts), xx = sum(nm + (nrhs + k) * parts);
+        double e2 = xx > 0 ? std::sqrt(rr) / std::sqrt(xx) : (rr > 0 ? INFINITY : 0.0);
+        if (!(e2 <= worst2)) worst2 = e2;
     }
     *eta = worst;
+    *res2 = worst2;
     return MFD_OK;
 }
 
@@ -1001,15 +1016,16 @@ int solve_permuted_device(mfd_handle* h, int nrhs)
     hipStream_t st = h->stream;
     h->st.refine_iters = 0;
     h->st.backward_error = -1;
+    h->st.refine_residual = -1;
     if (h->precision == MFD_FP64) {
         MFD_TRY(run_solve<double>(h, h->d_bp, h->d_xp, nrhs));
         return MFD_OK;
     }
     MFD_TRY(run_solve<float>(h, h->d_bp, h->d_xp, nrhs));
-    double eta = 0, prev = INFINITY;
+    double eta = 0, res2 = 0, prev = INFINITY;
     int it = 0;
     for (;;) {
-        MFD_TRY(backward_error(h, h->d_bp, h->d_xp, h->d_r, nrhs, &eta));
+        MFD_TRY(backward_error(h, h->d_bp, h->d_xp, h->d_r, nrhs, &eta, &res2));
         if (!(eta > kRefineTarget) || it >= kMaxRefine || !(eta < 0.5 * prev)) break;
         prev = eta;
         MFD_TRY(run_solve<float>(h, h->d_r, h->d_d, nrhs));
@@ -1018,13 +1034,15 @@ int solve_permuted_device(mfd_handle* h, int nrhs)
     }
     h->st.refine_iters = it;
     h->st.backward_error = eta;
+    h->st.refine_residual = res2;
     if (!(eta <= kRefineTarget)) {
         // refinement stalled: FP64 refactorization fallback
         int rc = factorize_common(h, MFD_FP64);
         if (rc != MFD_OK) return rc;
         MFD_TRY(run_solve<double>(h, h->d_bp, h->d_xp, nrhs));
-        MFD_TRY(backward_error(h, h->d_bp, h->d_xp, h->d_r, nrhs, &eta));
+        MFD_TRY(backward_error(h, h->d_bp, h->d_xp, h->d_r, nrhs, &eta, &res2));
         h->st.backward_error = eta;
+        h->st.refine_residual = res2;
         h->st.refine_iters = -1 - it;  // negative: fell back to FP64
     }
     return MFD_OK;

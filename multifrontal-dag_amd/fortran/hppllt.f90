@@ -256,7 +256,8 @@ contains
     rc = status
   end function
 
-  ! Normwise backward error of the last solve (max over right-hand sides).
+  ! Normwise backward error of the last solve (max over right-hand sides);
+  ! -1 after an FP64 solve (only FP32 + refinement solves compute it).
   double precision function hppllt_get_backward_error() result(eta)
     type(mfd_stats) :: st
     integer(c_int) :: rc

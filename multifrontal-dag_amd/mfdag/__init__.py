@@ -17,7 +17,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libmfdag.so")
 
 MFD_OK = 0
+MFD_ERR_ARG = -1
 MFD_ERR_NODEVICE = -7
+MFD_ERR_UNSUPPORTED = -8
 FP64 = 0
 FP32_IR = 1
 FLAG_GRAPH = 1
@@ -80,6 +82,8 @@ def lib():
             "mfd_get_stats": (i, [P, C.POINTER(MfdStats)]),
             "mfd_get_kernel_time": (i, [P, i, dp, ip]),
             "mfd_get_kernel_work": (i, [P, i, i, dp, dp]),
+            "mfd_mm_size": (i, [C.c_char_p, ip, lp]),
+            "mfd_mm_read": (i, [C.c_char_p, i, i64, ip, ip, dp]),
             "mfd_error_string": (C.c_char_p, [i]),
         }
         for name, (res, args) in sig.items():
@@ -96,6 +100,26 @@ def _ptr(a, ct):
 
 def _i32(a):
     return np.ascontiguousarray(np.asarray(a), dtype=np.int32)
+
+
+def read_matrix_market(path):
+    """(n, ccs_col, ccs_row, ccs_val) of a MatrixMarket file holding the lower
+    triangle of a symmetric matrix: the 1-based lower CCS, diagonal first in
+    each column, that Solver.analyze / factorize take (the reference driver's
+    reading, example.f90:63-145)."""
+    L = lib()
+    name = os.fsencode(path)
+    n, nnz = C.c_int(), C.c_int64()
+    rc = L.mfd_mm_size(name, C.byref(n), C.byref(nnz))
+    if rc != MFD_OK:
+        raise MfdError(rc, f"mfd_mm_size({path})")
+    col = np.empty(n.value + 1, np.int32)
+    row = np.empty(nnz.value, np.int32)
+    val = np.empty(nnz.value, np.float64)
+    rc = L.mfd_mm_read(name, n.value, nnz.value, _ptr(col, C.c_int), _ptr(row, C.c_int), _ptr(val, C.c_double))
+    if rc != MFD_OK:
+        raise MfdError(rc, f"mfd_mm_read({path})")
+    return n.value, col, row, val
 
 
 class Solver:

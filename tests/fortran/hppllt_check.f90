@@ -65,7 +65,6 @@ program hppllt_check
     if (hppllt_get_status() /= 0) stop 1
     err = maxval(abs(b - xt)) / maxval(abs(xt))
     print '(a,es10.3)', "fp64_x_err ", err
-    print '(a,es10.3)', "fp64_backward_error ", hppllt_get_backward_error()
     if (err > 1d-10) stop 1
 
     allocate(bm(n, 3), xm(n, 3))
