@@ -48,6 +48,7 @@ typedef struct mfd_handle mfd_handle;
 /* mfd_create flags */
 #define MFD_FLAG_GRAPH 1    /* replay each schedule from a captured hipGraph */
 #define MFD_FLAG_PROFILE 2  /* time every kernel class with HIP events */
+#define MFD_FLAG_ORDER_ND 4 /* use_metis = 1: mfd_analyze(_batch) orders with mfd_order_nd when no perm is given */
 
 /* timer ids: time_list_m.f90:3-10 */
 #define MFD_TIME_ORDERING 1
@@ -80,8 +81,11 @@ void mfd_destroy(mfd_handle* h);
    ccs_col (n+1) / ccs_row: 1-based CCS of the lower triangle with the
    diagonal first in each column.  perm (nullable, new -> old, 1-based) takes
    the place of the METIS ordering (use_metis = 1); NULL analyses the matrix
-   as given (use_metis = 0).  Builds the GPU schedule and allocates all device
-   storage.  Unlike the reference, ccs_row is copied, not retained. */
+   as given (use_metis = 0), or, on a handle created with MFD_FLAG_ORDER_ND,
+   in the order mfd_order_nd gives (use_metis = 1 with the library's own
+   ordering; timed as MFD_TIME_ORDERING).  Builds the GPU schedule and
+   allocates all device storage.  Unlike the reference, ccs_row is copied,
+   not retained. */
 int mfd_analyze(mfd_handle* h, int n, const int* ccs_col, const int* ccs_row, int max_zero, const int* perm);
 
 /* The batching form: nsys independent systems, system k given like
@@ -175,6 +179,12 @@ int mfd_get_kernel_time(mfd_handle* h, int k, double* total_ms, int* launches);
    bytes the class must move at minimum (each operand read once, each result
    written once) for an element size of elem_bytes. */
 int mfd_get_kernel_work(mfd_handle* h, int k, int elem_bytes, double* flops, double* bytes);
+
+/* Fill-reducing ordering in place of METIS_NodeND (hppllt.f90:47-51,
+   ordering_m.f90:10-38): graph nested dissection of the pattern of the
+   1-based lower CCS (as mfd_analyze takes it).  perm (n) receives new -> old,
+   1-based, the orientation mfd_analyze's perm argument takes.  Host only. */
+int mfd_order_nd(int n, const int* ccs_col, const int* ccs_row, int* perm);
 
 /* MatrixMarket input, as the reference's driver reads its matrices
    (example.f90:63-145; no counterpart in the hppllt module): a symmetric

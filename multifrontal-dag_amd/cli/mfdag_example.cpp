@@ -4,10 +4,13 @@
 // record the reference's experiment scripts read (example.f90:47-60).
 //
 // usage: mfdag_example matrix.mtx max_zero nb record_file
-//                      [--perm FILE] [--fp32] [--analyze-only]
-//   --perm FILE     ordering (new -> old, 1-based, n integers) in place of
-//                   the reference's METIS ordering; without it the matrix is
-//                   analysed in its own order (use_metis = 0)
+//                      [--natural | --perm FILE] [--fp32] [--analyze-only]
+//   The matrix is ordered by the library's graph nested dissection
+//   (mfd_order_nd) where the reference's driver has METIS order it
+//   (options(4) = use_metis = 1, example.f90:33), unless
+//   --natural       the matrix is analysed in its own order (use_metis = 0)
+//   --perm FILE     the ordering is given (new -> old, 1-based, n integers)
+// and
 //   --fp32          FP32 fronts + FP64 iterative refinement
 //   --analyze-only  analysis and its getters only (no GPU needed); the
 //                   record's factorize / solve times are 0 and its error NaN
@@ -121,10 +124,12 @@ int main(int argc, char** argv)
 {
     std::vector<const char*> pos;
     const char* perm_file = nullptr;
-    bool fp32 = false, analyze_only = false;
+    bool fp32 = false, analyze_only = false, natural = false;
     for (int a = 1; a < argc; ++a) {
         if (!std::strcmp(argv[a], "--perm") && a + 1 < argc)
             perm_file = argv[++a];
+        else if (!std::strcmp(argv[a], "--natural"))
+            natural = true;
         else if (!std::strcmp(argv[a], "--fp32"))
             fp32 = true;
         else if (!std::strcmp(argv[a], "--analyze-only"))
@@ -132,7 +137,7 @@ int main(int argc, char** argv)
         else
             pos.push_back(argv[a]);
     }
-    if (pos.size() != 4) {
+    if (pos.size() != 4 || (natural && perm_file)) {
         std::printf(" command line argument error\n");
         return 1;
     }
@@ -159,7 +164,7 @@ int main(int argc, char** argv)
     std::vector<double> b(n, rh);
     std::printf(" I\n");
     mfd_handle* h = nullptr;
-    rc = mfd_create(&h, nb, MFD_FLAG_GRAPH);
+    rc = mfd_create(&h, nb, MFD_FLAG_GRAPH | (natural ? 0 : MFD_FLAG_ORDER_ND));
     if (rc != MFD_OK && !(analyze_only && rc == MFD_ERR_NODEVICE)) {
         mfd_destroy(h);
         return fail("mfd_create", rc);

@@ -1151,6 +1151,16 @@ int mfd_analyze(mfd_handle* h, int n, const int* ccs_col, const int* ccs_row, in
     double t0 = now_ms();
     h->analyzed = h->factored = false;
     h->from_parts = false;
+    std::vector<int> nd;
+    if (!perm && (h->flags & MFD_FLAG_ORDER_ND)) {  // use_metis = 1 (hppllt.f90:47-51)
+        if (n < 1) return MFD_ERR_ARG;
+        nd.resize(n);
+        const double t1 = now_ms();
+        int rc = mfd_order_nd(n, ccs_col, ccs_row, nd.data());
+        if (rc != MFD_OK) return rc;
+        h->times[MFD_TIME_ORDERING] = (now_ms() - t1) / 1e3;
+        perm = nd.data();
+    }
     int rc = analyze(n, ccs_col, ccs_row, max_zero, perm, h->S);
     if (rc != MFD_OK) return rc;
     rc = finish_analysis(h);
@@ -1162,10 +1172,28 @@ int mfd_analyze(mfd_handle* h, int n, const int* ccs_col, const int* ccs_row, in
 int mfd_analyze_batch(mfd_handle* h, int nsys, const int* n, const int* const* ccs_col, const int* const* ccs_row,
                       int max_zero, const int* const* perm)
 {
-    if (!h) return MFD_ERR_ARG;
+    if (!h || nsys < 1 || !n || !ccs_col || !ccs_row) return MFD_ERR_ARG;
     double t0 = now_ms();
     h->analyzed = h->factored = false;
     h->from_parts = false;
+    std::vector<std::vector<int>> nd;
+    std::vector<const int*> perms;
+    if (h->flags & MFD_FLAG_ORDER_ND) {  // systems without an ordering get the library's
+        const double t1 = now_ms();
+        nd.resize(nsys);
+        perms.resize(nsys);
+        for (int k = 0; k < nsys; ++k) {
+            perms[k] = perm ? perm[k] : nullptr;
+            if (perms[k]) continue;
+            if (n[k] < 1 || !ccs_col[k] || !ccs_row[k]) return MFD_ERR_ARG;
+            nd[k].resize(n[k]);
+            int rc = mfd_order_nd(n[k], ccs_col[k], ccs_row[k], nd[k].data());
+            if (rc != MFD_OK) return rc;
+            perms[k] = nd[k].data();
+        }
+        h->times[MFD_TIME_ORDERING] = (now_ms() - t1) / 1e3;
+        perm = perms.data();
+    }
     int rc = analyze_batch(nsys, n, ccs_col, ccs_row, max_zero, perm, h->S);
     if (rc != MFD_OK) return rc;
     rc = finish_analysis(h);

@@ -8,9 +8,10 @@
 ! factorization and the solves on one gfx950 GPU.
 !
 ! Differences, all additive:
-!   - hppllt_set_ordering(perm) supplies the fill-reducing ordering (new ->
-!     old, 1-based) that the reference takes from METIS when
-!     options(4) = use_metis = 1 (METIS is not part of this library);
+!   - with options(4) = use_metis = 1 the fill-reducing ordering comes from
+!     the library's graph nested dissection (mfd_order_nd) where the
+!     reference calls METIS_NodeND, unless hppllt_set_ordering(perm) (new ->
+!     old, 1-based) supplies one;
 !   - hppllt_set_precision(p): 0 = FP64 (default), 1 = FP32 fronts + FP64
 !     iterative refinement;
 !   - hppllt_solve_multi(b(:,:)) solves several right-hand sides at once;
@@ -33,7 +34,7 @@ module hppllt
 
   integer, parameter :: NUM_OPTIONS = 4
   integer, parameter :: OPTION_NB = 1, OPTION_MAX_ZERO = 2, OPTION_USE_STARPU = 3, OPTION_USE_METIS = 4
-  integer(c_int), parameter :: FLAG_GRAPH = 1
+  integer(c_int), parameter :: FLAG_GRAPH = 1, FLAG_ORDER_ND = 4
   integer(c_int), parameter :: ERR_NODEVICE = -7
 
   type(c_ptr), save :: handle = c_null_ptr
@@ -145,7 +146,11 @@ contains
       return
     endif
     allocate(options, source=input_options)
-    rc = mfd_create(handle, int(options(OPTION_NB), c_int), FLAG_GRAPH)
+    if (options(OPTION_USE_METIS) == 1) then
+      rc = mfd_create(handle, int(options(OPTION_NB), c_int), ior(FLAG_GRAPH, FLAG_ORDER_ND))
+    else
+      rc = mfd_create(handle, int(options(OPTION_NB), c_int), FLAG_GRAPH)
+    endif
     ! Without a GPU the handle still serves the analysis and the getters.
     if (rc == ERR_NODEVICE) then
       status = rc
@@ -154,9 +159,9 @@ contains
     endif
   end subroutine
 
-  ! Ordering for use_metis = 1: perm(k) is the original index of the k-th
-  ! unknown in the new order (1-based), the orientation of reordering_perm
-  ! in the reference (hppllt.f90:47-51).
+  ! Ordering for use_metis = 1 in place of the library's own: perm(k) is the
+  ! original index of the k-th unknown in the new order (1-based), the
+  ! orientation of reordering_perm in the reference (hppllt.f90:47-51).
   subroutine hppllt_set_ordering(perm)
     integer, intent(in) :: perm(:)
     if (allocated(ordering)) deallocate(ordering)
@@ -180,12 +185,7 @@ contains
       return
     endif
     perm = c_null_ptr
-    if (options(OPTION_USE_METIS) == 1) then
-      if (.not. allocated(ordering)) then
-        print *, "hppllt_analyze: use_metis = 1 needs an ordering (hppllt_set_ordering)"
-        status = -1
-        return
-      endif
+    if (options(OPTION_USE_METIS) == 1 .and. allocated(ordering)) then
       if (size(ordering) /= n) then
         print *, "hppllt_analyze: ordering has the wrong length"
         status = -1

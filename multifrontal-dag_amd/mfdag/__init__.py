@@ -24,6 +24,7 @@ FP64 = 0
 FP32_IR = 1
 FLAG_GRAPH = 1
 FLAG_PROFILE = 2
+FLAG_ORDER_ND = 4
 KERNEL_CLASSES = ["assemble", "extend_add", "small_front", "potrf", "trsm", "update", "schur", "solve_fwd",
                   "solve_bwd", "refine"]
 
@@ -82,6 +83,7 @@ def lib():
             "mfd_get_stats": (i, [P, C.POINTER(MfdStats)]),
             "mfd_get_kernel_time": (i, [P, i, dp, ip]),
             "mfd_get_kernel_work": (i, [P, i, i, dp, dp]),
+            "mfd_order_nd": (i, [i, ip, ip, ip]),
             "mfd_mm_size": (i, [C.c_char_p, ip, lp]),
             "mfd_mm_read": (i, [C.c_char_p, i, i64, ip, ip, dp]),
             "mfd_error_string": (C.c_char_p, [i]),
@@ -100,6 +102,18 @@ def _ptr(a, ct):
 
 def _i32(a):
     return np.ascontiguousarray(np.asarray(a), dtype=np.int32)
+
+
+def order_nd(ccs_col, ccs_row):
+    """Graph nested dissection ordering (new -> old, 1-based) of the pattern
+    of a 1-based lower CCS: the library's stand-in for METIS_NodeND."""
+    col, row = _i32(ccs_col), _i32(ccs_row)
+    n = len(col) - 1
+    perm = np.empty(n, np.int32)
+    rc = lib().mfd_order_nd(n, _ptr(col, C.c_int), _ptr(row, C.c_int), _ptr(perm, C.c_int))
+    if rc != MFD_OK:
+        raise MfdError(rc, "mfd_order_nd")
+    return perm
 
 
 def read_matrix_market(path):
@@ -130,10 +144,10 @@ class Solver:
     created on a machine without a GPU (analysis and structure queries work;
     numeric calls raise)."""
 
-    def __init__(self, nb=64, graph=False, profile=False, host_only=False):
+    def __init__(self, nb=64, graph=False, profile=False, host_only=False, order_nd=False):
         L = lib()
         h = C.c_void_p()
-        flags = (FLAG_GRAPH if graph else 0) | (FLAG_PROFILE if profile else 0)
+        flags = (FLAG_GRAPH if graph else 0) | (FLAG_PROFILE if profile else 0) | (FLAG_ORDER_ND if order_nd else 0)
         rc = L.mfd_create(C.byref(h), int(nb), flags)
         self.h = h
         self.nb = int(nb)

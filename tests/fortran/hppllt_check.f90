@@ -124,6 +124,22 @@ program hppllt_check
     if (err > 1d-10) stop 1
   endif
   call hppllt_finalize()
+
+  ! use_metis = 1 without a caller ordering: the library's own ordering
+  call hppllt_init([nb, max_zero, 0, 1])
+  call hppllt_analyze(ccs_col, ccs_row(1:nnz), n)
+  if (hppllt_get_status() /= 0) stop 1
+  print '(a,i0)', "nd_cost ", hppllt_get_cost()
+  if (trim(mode) == "solve") then
+    call matvec(xt, b)
+    call hppllt_factorize(a)
+    call hppllt_solve(b)
+    if (hppllt_get_status() /= 0) stop 1
+    err = maxval(abs(b - xt)) / maxval(abs(xt))
+    print '(a,es10.3)', "nd_x_err ", err
+    if (err > 1d-10) stop 1
+  endif
+  call hppllt_finalize()
   print '(a)', "ok"
 
 contains

@@ -3,7 +3,8 @@ from a Fortran program through ISO_C_BINDING, as a caller of the reference
 module would call it (tests/fortran/hppllt_check.f90).
 
 CPU: the analysis getters the Fortran caller sees equal the C-ABI's on the
-same matrix and orderings (natural, and a caller ordering with use_metis = 1).
+same matrix and orderings (natural, a caller ordering with use_metis = 1, and
+use_metis = 1 without one: the library's nested dissection).
 GPU: the same program factorizes and solves (FP64, 3 right-hand sides,
 FP32 + refinement) and checks x against x_true."""
 import os
@@ -71,6 +72,11 @@ def test_fortran_analysis_matches_capi(k, nb, max_zero):
     s.analyze(cp, rw, max_zero, red_black(k))
     assert int(out["perm_cost"]) == s.cost()
     s.close()
+    # use_metis = 1 without a caller ordering: mfd_order_nd's
+    s = mfdag.Solver(nb=nb, host_only=True)
+    s.analyze(cp, rw, max_zero, mfdag.order_nd(cp, rw))
+    assert int(out["nd_cost"]) == s.cost()
+    s.close()
 
 
 @pytest.mark.gpu
@@ -81,3 +87,4 @@ def test_fortran_factorize_solve(k, nb, max_zero):
     assert float(out["multi_x_err"]) <= 1e-12
     assert 0 <= float(out["fp32ir_backward_error"]) <= 1e-14
     assert float(out["perm_x_err"]) <= 1e-12
+    assert float(out["nd_x_err"]) <= 1e-12

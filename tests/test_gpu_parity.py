@@ -202,6 +202,32 @@ def test_fp32_refinement_reaches_fp64_accuracy():
     assert host / 10 <= st["refine_residual"] <= host * 10
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_refactorize_same_structure_new_values(graph):
+    """Values-only refactorization on one analysis (the reference calls
+    hppllt_factorize again with the structure of hppllt_analyze,
+    hppllt.f90:123-150): new values, then the first values again, then the
+    FP32+IR form and FP64 again; each factor against the oracle's for its own
+    values, each x to the backward-error bound."""
+    log("refactorize graph", graph)
+    p1 = P.weighted_laplacian_3d(12, seed=3)
+    p2 = P.weighted_laplacian_3d(12, seed=4)
+    assert np.array_equal(p1.ccs_row, p2.ccs_row) and not np.array_equal(p1.val, p2.val)
+    s = mfdag.Solver(nb=32, graph=graph)
+    s.analyze(p1.ccs_col, p1.ccs_row, 16, p1.perm)
+    for prob, precision in ((p1, mfdag.FP64), (p2, mfdag.FP64), (p1, mfdag.FP64), (p2, mfdag.FP32_IR),
+                            (p2, mfdag.FP64)):
+        s.factorize(prob.val, precision=precision)
+        if precision == mfdag.FP64:
+            f, st, info = oracle_for(s, prob.val)
+            assert info == 0
+            assert front_errors(s, f, st).max() <= 1e-12
+        xt, b = P.rhs(prob, 2, seed=7)
+        x = s.solve(b)
+        assert P.backward_error(prob, x, b) <= 1e-14
+    s.close()
+
+
 def test_not_positive_definite_reports_column():
     p = P.laplacian_2d(12)
     val = p.val.copy()

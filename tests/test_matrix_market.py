@@ -126,10 +126,10 @@ def parse_record(path):
     return fields
 
 
-def check_record_structure(fields, name, max_zero, nb):
+def check_record_structure(fields, name, max_zero, nb, natural):
     n, col, row, _ = mfdag.read_matrix_market(os.path.join(GOLDEN, name))
     s = mfdag.Solver(nb=nb, host_only=True)
-    s.analyze(col, row, max_zero, None)
+    s.analyze(col, row, max_zero, None if natural else mfdag.order_nd(col, row))
     assert int(fields[0]) == max_zero and int(fields[1]) == nb and int(fields[2]) == 1
     assert int(fields[3]) == s.nonzero()
     assert int(fields[4]) == s.cost()
@@ -139,13 +139,14 @@ def check_record_structure(fields, name, max_zero, nb):
 
 
 @pytest.mark.parametrize("name", MATRICES)
-def test_cli_analyze_record(tmp_path, name):
+@pytest.mark.parametrize("natural", [True, False])
+def test_cli_analyze_record(tmp_path, name, natural):
     rec = tmp_path / "rec.txt"
-    r = run_cli([os.path.join(GOLDEN, name), 10, 10, rec, "--analyze-only"])
+    r = run_cli([os.path.join(GOLDEN, name), 10, 10, rec, "--analyze-only"] + (["--natural"] if natural else []))
     assert r.returncode == 0
     assert r.stdout.split() == ["I", "A"]
     fields = parse_record(rec)
-    check_record_structure(fields, name, 10, 10)
+    check_record_structure(fields, name, 10, 10, natural)
     for f in fields[8:15]:
         assert float(f) >= 0.0
     assert fields[15].strip() == "NaN"
@@ -161,6 +162,8 @@ def test_cli_argument_errors(tmp_path):
     perm.write_text("1 2 3\n")
     r = run_cli([os.path.join(GOLDEN, MATRICES[0]), 10, 10, tmp_path / "rec.txt", "--analyze-only", "--perm", perm])
     assert r.returncode != 0
+    r = run_cli([os.path.join(GOLDEN, MATRICES[0]), 10, 10, tmp_path / "rec.txt", "--natural", "--perm", perm])
+    assert r.returncode != 0 and "command line argument error" in r.stdout
 
 
 def test_cli_caller_ordering(tmp_path):
@@ -189,14 +192,17 @@ def example_error(M, x, rh):
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", MATRICES)
 @pytest.mark.parametrize("fp32", [False, True])
-def test_cli_end_to_end(tmp_path, name, fp32):
+@pytest.mark.parametrize("natural", [True, False])
+def test_cli_end_to_end(tmp_path, name, fp32, natural):
+    """hppllt_test.f90:12-15's four runs: options(4) = 0 (--natural) and 1
+    (the library's ordering in place of METIS)."""
     rec = tmp_path / "rec.txt"
-    args = [os.path.join(GOLDEN, name), 10, 10, rec] + (["--fp32"] if fp32 else [])
+    args = [os.path.join(GOLDEN, name), 10, 10, rec] + (["--fp32"] if fp32 else []) + (["--natural"] if natural else [])
     r = run_cli(args)
     assert r.returncode == 0, r.stderr
     assert r.stdout.split() == ["I", "A", "F", "S"]
     fields = parse_record(rec)
-    check_record_structure(fields, name, 10, 10)
+    check_record_structure(fields, name, 10, 10, natural)
     err = float(fields[15])
     if not fp32:
         # the reference test's bound (hppllt_test.f90:37)
@@ -206,7 +212,7 @@ def test_cli_end_to_end(tmp_path, name, fp32):
         # ||b - A x||_inf <= 1e-14 (||A||_inf ||x||_inf + ||b||_inf); with
         # ||b||_inf <= ||A||_inf ||x||_inf that bounds this unscaled measure
         # by 2e-14 sqrt(n) ||A||_inf (bcsstk01: ||A||_inf = 3.6e9).
-        _, _, _, _, M = independent_ccs(args[0])
+        _, _, _, _, M = independent_ccs(os.path.join(GOLDEN, name))
         A_full = M + sp.tril(M, -1).T
         norm_a = abs(A_full).sum(axis=1).max()
         assert 0.0 <= err <= 2e-14 * np.sqrt(M.shape[0]) * norm_a, err
