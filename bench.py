@@ -109,6 +109,40 @@ def batch_leg(args):
     return out
 
 
+def nd_leg(args, p, b):
+    """The headline matrix and right-hand sides under the library's graph
+    nested dissection (mfd_order_nd, the stand-in for the reference's METIS
+    call) instead of the generator's geometric one; FP64 and FP32 + refinement,
+    timed like the main leg."""
+    import torch
+    import mfdag
+    from mfdag import problems as P
+    t0 = time.perf_counter()
+    perm = mfdag.order_nd(p.ccs_col, p.ccs_row)
+    order_s = time.perf_counter() - t0
+    s = mfdag.Solver(nb=args.nb, graph=not args.no_graph)
+    s.analyze(p.ccs_col, p.ccs_row, args.max_zero, perm)
+    log(f"graph ND: ordering {order_s:.2f}s, fronts={s.num_node}, cost={s.cost():.3e}")
+    dev = torch.device("cuda:0")
+    d_a = torch.from_numpy(np.ascontiguousarray(p.val)).to(dev)
+    d_b = torch.from_numpy(np.ascontiguousarray(b.T)).to(dev)
+    d_x = torch.empty_like(d_b)
+    torch.cuda.synchronize()
+    out = {"ordering": "mfd_order_nd (graph nested dissection)", "order_s": round(order_s, 3),
+           "fronts": s.num_node, "cost_flops": s.cost()}
+    for name, prec in (("fp64", mfdag.FP64), ("fp32_ir", mfdag.FP32_IR)):
+        wall, f, so = timed_steps(s, d_a, d_b, d_x, prec, args.steps, args.warmup)
+        x = d_x.cpu().numpy().T
+        out[name] = {"ms_per_step": round(wall * 1e3, 3), "factor_ms": round(f, 3), "solve_ms": round(so, 3),
+                     "factor_gflops": round(s.cost() / (f * 1e6), 2), "backward_error": P.backward_error(p, x, b)}
+        if prec == mfdag.FP32_IR:
+            out[name]["refine_iters"] = s.stats()["refine_iters"]
+        log(f"graph ND {name}: step {wall * 1e3:.2f} ms (factor {f:.2f} ms, solve {so:.2f} ms), "
+            f"eta {out[name]['backward_error']:.2e}")
+    s.close()
+    return out
+
+
 def roofline(p, b, max_zero, nb, precision):
     """One profiled factorization + solve: per-class time (HIP events on the
     library's stream, one pair per launch; the solve sweeps as a whole)
@@ -208,6 +242,7 @@ def main():
     ap.add_argument("--batch", type=int, default=512, help="systems in the batching leg (0: skip it)")
     ap.add_argument("--batch-max-zero", type=int, default=4096)
     ap.add_argument("--batch-only", action="store_true", help="run only the batching leg")
+    ap.add_argument("--no-nd-leg", action="store_true", help="skip the leg under the library's graph ordering")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the profiled FP64 pass the roofline entry comes from (for rocprofv3)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -279,6 +314,7 @@ def main():
     roof, classes = roofline(p, b, args.max_zero, args.nb, mfdag.FP64)
     log("kernel classes (FP64):", json.dumps({k: {kk: round(v, 4) for kk, v in r.items()} for k, r in classes.items()}))
 
+    nd = None if args.no_nd_leg else nd_leg(args, p, b)
     batch = batch_leg(args) if args.batch > 0 else None
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_size, args.max_zero, args.nb, args.seed)
 
@@ -300,6 +336,7 @@ def main():
         "roofline": roof,
         "kernel_classes_fp64": {k: {"ms": round(r["ms"], 4), "launches": r["launches"]} for k, r in classes.items()},
         "cpu_baseline": cpu,
+        "graph_nd": nd,
         "batch": batch,
     }
     print(json.dumps(out), flush=True)
