@@ -49,6 +49,7 @@ struct Launch {
 
 // A step's update tiles join the step's launch (k_upd_diag, one workgroup
 // per CU at its LDS) up to this many; more run as their own launch.
+constexpr int kFillGroups = 32;  // most level groups the front zeroing is split into
 constexpr int kFuseTiles = 1024;
 
 // Step launches with at least this many diagonal tasks run their GEMM
@@ -97,6 +98,16 @@ struct mfd_handle {
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;  // large in-panel updates beside the diagonal chain
     hipEvent_t fork = nullptr, sev[2] = {};
+    // Front-pool zeroing and assembly in groups of levels beside the
+    // factorization: group g's fronts (fill tasks [fill_gptr[g],
+    // fill_gptr[g+1]), entries [scat_gptr[g], scat_gptr[g+1])) are first
+    // written by a launch of level group_level[g] or later; the main stream
+    // waits for fill_ev[g] before the first launch of that level
+    // (level_launch).
+    hipStream_t fillst = nullptr;
+    std::vector<hipEvent_t> fill_ev;
+    std::vector<int> fill_gptr, group_level, level_launch, front_group;
+    std::vector<int64_t> scat_gptr;
     hipError_t last_hip = hipSuccess;
     int64_t mem_limit = 0, device_bytes = 0, front_bytes = 0;
     std::vector<void*> allocs;
@@ -418,6 +429,29 @@ void build_plan(mfd_handle* h)
         std::vector<int> pos(h->level_ptr.begin(), h->level_ptr.end() - 1);
         for (int s = 0; s < m; ++s) h->level_nodes[pos[lev[s]]++] = s;
     }
+    // fill groups: a front is first written by its own level's launches or,
+    // earlier, by a child's extend-add
+    std::vector<int> group(m, 0);
+    {
+        const int G = std::min(h->num_levels, kFillGroups);
+        h->group_level.resize(G);
+        for (int g = 0; g < G; ++g) h->group_level[g] = (int)((int64_t)g * h->num_levels / G);
+        for (int s = 0; s < m; ++s) {
+            int ready = lev[s];
+            for (int ch : S.children[s]) ready = std::min(ready, lev[ch - 1]);
+            group[s] = (int)(std::upper_bound(h->group_level.begin(), h->group_level.end(), ready) -
+                             h->group_level.begin()) - 1;
+        }
+        h->fill.clear();
+        h->fill_gptr.assign(1, 0);
+        for (int g = 0; g < G; ++g) {
+            for (int s2 = 0; s2 < m; ++s2)
+                if (group[s2] == g)
+                    for (int j0 = 0; j0 < h->nodes[s2].n; j0 += 64) h->fill.push_back(make_int2(s2, j0));
+            h->fill_gptr.push_back((int)h->fill.size());
+        }
+    }
+    h->front_group = group;
     // rank of each node among its parent's children (0 for roots)
     std::vector<int> rank(m, 0);
     for (int s = 0; s < m; ++s)
@@ -432,16 +466,15 @@ void build_plan(mfd_handle* h)
     h->ext.clear();
     h->small.clear();
     h->diag.clear();
-    h->fill.clear();
-    for (int s = 0; s < m; ++s)
-        for (int j0 = 0; j0 < h->nodes[s].n; j0 += 64) h->fill.push_back(make_int2(s, j0));
     h->gemm.clear();
     h->fseq.clear();
     int nside = 0;
     auto push = [&](int kind, int64_t first, int64_t end, int wide = 0) {
         if (end > first) h->fseq.push_back(Launch{kind, first, (int)(end - first), wide});
     };
+    h->level_launch.assign(h->num_levels, 0);
     for (int L = 0; L < h->num_levels; ++L) {
+        h->level_launch[L] = (int)h->fseq.size();
         const int* lv = h->level_nodes.data() + h->level_ptr[L];
         const int cnt = h->level_ptr[L + 1] - h->level_ptr[L];
         // extend-add of the small children (a large child adds its update
@@ -675,6 +708,11 @@ int setup_device(mfd_handle* h)
     MFD_TRY(upload(h, &h->d_small, h->small));
     MFD_TRY(upload(h, &h->d_diag, h->diag));
     MFD_TRY(upload(h, &h->d_fill, h->fill));
+    while (h->fill_ev.size() < h->group_level.size()) {
+        hipEvent_t ev;
+        HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        h->fill_ev.push_back(ev);
+    }
     MFD_TRY(upload(h, &h->d_gemm, h->gemm));
     MFD_TRY(upload(h, &h->d_level_nodes, h->level_nodes));
     MFD_TRY(upload(h, &h->d_child_ptr, h->child_ptr));
@@ -684,19 +722,27 @@ int setup_device(mfd_handle* h)
     MFD_TRY(upload(h, &h->d_fi, h->fi));
     MFD_TRY(upload(h, &h->d_fa, h->fa));
     MFD_TRY(upload(h, &h->d_bg, h->bg));
-    // assembly map: A entry -> front position (coefficient_setter_m.f90:26-41)
+    // assembly map: A entry -> front position (coefficient_setter_m.f90:26-41),
+    // the entries in the fill groups' order
     const int64_t nnz = (int64_t)S.snidx.size();
     std::vector<int64_t> dst(nnz), src(nnz);
-    for (int s = 0; s < S.num_node; ++s) {
-        const NodeDev& d = h->nodes[s];
-        for (int j = S.first[s]; j < S.first[s + 1]; ++j) {
-            int64_t q = j - S.first[s];
-            for (int64_t p = S.a_colptr[j - 1]; p < S.a_colptr[j]; ++p) {
-                dst[p] = d.front + (S.snidx[p] - 1) + q * d.n;
-                src[p] = S.ccs_perm[p] - 1;
+    h->scat_gptr.assign(1, 0);
+    int64_t e = 0;
+    for (size_t g = 0; g < h->group_level.size(); ++g) {
+        for (int s = 0; s < S.num_node; ++s) {
+            if (h->front_group[s] != (int)g) continue;
+            const NodeDev& d = h->nodes[s];
+            for (int j = S.first[s]; j < S.first[s + 1]; ++j) {
+                int64_t q = j - S.first[s];
+                for (int64_t p = S.a_colptr[j - 1]; p < S.a_colptr[j]; ++p, ++e) {
+                    dst[e] = d.front + (S.snidx[p] - 1) + q * d.n;
+                    src[e] = S.ccs_perm[p] - 1;
+                }
             }
         }
+        h->scat_gptr.push_back(e);
     }
+    if (e != nnz) return MFD_ERR_INTERNAL;
     MFD_TRY(upload(h, &h->d_dst, dst));
     MFD_TRY(upload(h, &h->d_src, src));
     std::vector<int> perm0(S.n);
@@ -781,15 +827,32 @@ template <typename T> void enqueue_factor(mfd_handle* h, bool profile)
     };
     launch_set_int(h->d_info, INT_MAX, st);
     mark(0);
-    launch_fill_lower<T>(pool, h->d_nodes, h->d_fill, (int)h->fill.size(), st);
-    launch_scatter_a<T>(pool, h->d_a, h->d_dst, h->d_src, (int64_t)h->S.snidx.size(), st);
+    // Zeroing and assembly of the fronts: beside the factorization, group by
+    // group, when the fill stream and its events exist; otherwise (and when
+    // profiling) all of it first, on the main stream.
+    const int G = (int)h->group_level.size();
+    const bool split_fill = !profile && h->fillst && (int)h->fill_ev.size() >= G;
+    hipStream_t fst = split_fill ? h->fillst : st;
+    if (split_fill) {
+        (void)hipEventRecord(h->fork, st);
+        (void)hipStreamWaitEvent(fst, h->fork, 0);
+    }
+    for (int g = 0; g < G; ++g) {
+        launch_fill_lower<T>(pool, h->d_nodes, h->d_fill + h->fill_gptr[g], h->fill_gptr[g + 1] - h->fill_gptr[g], fst);
+        launch_scatter_a<T>(pool, h->d_a, h->d_dst + h->scat_gptr[g], h->d_src + h->scat_gptr[g],
+                            h->scat_gptr[g + 1] - h->scat_gptr[g], fst);
+        if (split_fill) (void)hipEventRecord(h->fill_ev[g], fst);
+    }
     mark(0);
     // Side launches fork from the main stream's current position and run
     // beside it; a step's launches join the side launch they depend on, and
     // any other launch joins every side launch before it.
     const bool two = !profile && h->side;
-    int last_side = -1, joined = -1;
-    for (const Launch& l : h->fseq) {
+    int last_side = -1, joined = -1, gnext = split_fill ? 0 : G;
+    for (size_t i = 0; i < h->fseq.size(); ++i) {
+        const Launch& l = h->fseq[i];
+        for (; gnext < G && h->level_launch[h->group_level[gnext]] <= (int)i; ++gnext)
+            (void)hipStreamWaitEvent(st, h->fill_ev[gnext], 0);
         if (two && l.side) {
             (void)hipEventRecord(h->fork, st);
             (void)hipStreamWaitEvent(h->side, h->fork, 0);
@@ -809,6 +872,7 @@ template <typename T> void enqueue_factor(mfd_handle* h, bool profile)
         mark(l.kind);
     }
     if (two && last_side > joined) (void)hipStreamWaitEvent(st, h->sev[last_side & 1], 0);
+    for (; gnext < G; ++gnext) (void)hipStreamWaitEvent(st, h->fill_ev[gnext], 0);
 }
 
 void collect_profile(mfd_handle* h, const std::vector<int>& kinds)
@@ -1118,6 +1182,7 @@ int mfd_create(mfd_handle** out, int nb, int flags)
             h->device_ok = true;
             for (auto& e : h->ev) (void)hipEventCreate(&e);
             if (hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess) h->side = nullptr;
+            if (hipStreamCreateWithFlags(&h->fillst, hipStreamNonBlocking) != hipSuccess) h->fillst = nullptr;
             (void)hipEventCreateWithFlags(&h->fork, hipEventDisableTiming);
             for (auto& e : h->sev) (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
         }
@@ -1139,6 +1204,8 @@ void mfd_destroy(mfd_handle* h)
         for (auto e : h->sev)
             if (e) (void)hipEventDestroy(e);
         if (h->side) (void)hipStreamDestroy(h->side);
+        for (auto e : h->fill_ev) (void)hipEventDestroy(e);
+        if (h->fillst) (void)hipStreamDestroy(h->fillst);
         (void)hipStreamDestroy(h->stream);
     }
     delete h->layout;
