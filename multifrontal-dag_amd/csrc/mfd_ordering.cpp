@@ -568,7 +568,9 @@ struct Dissector {
             thin(verts, cand);
             int na = 0, nb = 0, ns = 0;
             for (char l : cand) na += l == 0, nb += l == 1, ns += l == 2;
-            if (na == 0 || nb == 0) continue;
+            // a split leaving one side over 90% would let the recursion run
+            // as deep as the piece is large
+            if (na == 0 || nb == 0 || 10 * std::max(na, nb) > 9 * m) continue;
             const bool bal = 5 * std::min(na, nb) >= m;
             if (lab.empty() || (bal != best_bal ? bal : ns < best_ns)) lab = cand, best_ns = ns, best_bal = bal;
         }

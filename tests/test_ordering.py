@@ -80,6 +80,15 @@ def test_is_permutation(case):
     assert np.array_equal(p, mfdag.order_nd(col, row))
 
 
+def test_large_star_and_path():
+    """Graphs whose breadth-first levels split badly (a star: one centre,
+    all other vertices one level) or that are deep (a path) order without
+    running the recursion as deep as the graph is large."""
+    for n, edges in ((20000, [(0, i) for i in range(1, 20000)]), (100000, [(i, i + 1) for i in range(99999)])):
+        col, row = ccs_from_edges(n, edges)
+        assert is_perm(mfdag.order_nd(col, row), n)
+
+
 def test_path_is_dissected():
     """A path of 1000 vertices: nested dissection keeps the fill at most the
     separators' (every separator is one vertex), so nnz(L) stays O(n)."""
