@@ -133,3 +133,35 @@ def test_batch_fp32_refined():
         assert P.backward_error(p, x[off:off + p.n], b[off:off + p.n]) <= 1e-14
         off += p.n
     s.close()
+
+
+@pytest.mark.gpu
+def test_batch_graph_replays_at_scale():
+    """A batch the size of the benchmark's fronts (L up to 160, max_zero
+    4096): the captured schedule (main, side and fill streams) replayed three
+    times, with eager factorizations and solves between the replays, gives
+    the eager results bit for bit, and every system its backward-error
+    bound."""
+    systems = P.batch_anisotropic(160, seed=13)
+    bp = P.batch_problem(systems)
+    pairs = [(p.ccs_col, p.ccs_row) for p in systems]
+    perms = [p.perm for p in systems]
+    eager = mfdag.Solver(nb=64, graph=False)
+    graph = mfdag.Solver(nb=64, graph=True)
+    for s in (eager, graph):
+        s.analyze_batch(pairs, 4096, perms)
+    xt, b = batch_rhs(systems, 2, 9)
+    assert eager.factorize(bp.val) == 0
+    x_ref = eager.solve(b.copy())
+    for rep in range(3):
+        assert graph.factorize(bp.val) == 0
+        x = graph.solve(b.copy())
+        assert np.array_equal(x, x_ref), rep
+        assert eager.factorize(bp.val) == 0
+        assert np.array_equal(eager.solve(b.copy()), x_ref), rep
+    off = 0
+    for p in systems:
+        assert P.backward_error(p, x_ref[off:off + p.n], b[off:off + p.n]) <= 1e-14
+        off += p.n
+    eager.close()
+    graph.close()
