@@ -186,6 +186,20 @@ int mfd_get_kernel_work(mfd_handle* h, int k, int elem_bytes, double* flops, dou
    1-based, the orientation mfd_analyze's perm argument takes.  Host only. */
 int mfd_order_nd(int n, const int* ccs_col, const int* ccs_row, int* perm);
 
+/* Subtree partition of an assembly tree (tree_pruning, tree_pruning_m.f90:
+   13-74; host only): subtree roots whose subtrees, dealt out largest first
+   to the least loaded of `workers` >= p bins, leave the lightest bin more
+   than tol (0 <= tol < 1) of the heaviest's work, work being the
+   reference's cost formula (cost_calculator_m.f90:9-20) summed over a
+   subtree.  Input: per node (postordered, parents after children) c, w and
+   the 1-based parent (0 for a root), as mfd_get_structure returns them.
+   Output: subtrees (num_node entries of room) receives the 1-based roots in
+   the reference's list order, count their number, and bins (nullable) the
+   bin of each (0-based; the number of bins is the largest + 1).  A forest
+   starts from all of its roots. */
+int mfd_tree_pruning(int num_node, const int* c, const int* w, const int* parent, double tol, int p, int* subtrees,
+                     int* bins, int* count);
+
 /* MatrixMarket input, as the reference's driver reads its matrices
    (example.f90:63-145; no counterpart in the hppllt module): a symmetric
    matrix stored as its lower triangle, "row col value" lines in coordinate

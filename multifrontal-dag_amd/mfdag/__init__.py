@@ -84,6 +84,7 @@ def lib():
             "mfd_get_kernel_time": (i, [P, i, dp, ip]),
             "mfd_get_kernel_work": (i, [P, i, i, dp, dp]),
             "mfd_order_nd": (i, [i, ip, ip, ip]),
+            "mfd_tree_pruning": (i, [i, ip, ip, ip, d, i, ip, ip, ip]),
             "mfd_mm_size": (i, [C.c_char_p, ip, lp]),
             "mfd_mm_read": (i, [C.c_char_p, i, i64, ip, ip, dp]),
             "mfd_error_string": (C.c_char_p, [i]),
@@ -114,6 +115,23 @@ def order_nd(ccs_col, ccs_row):
     if rc != MFD_OK:
         raise MfdError(rc, "mfd_order_nd")
     return perm
+
+
+def tree_pruning(c, w, parent, tol, p):
+    """Subtree partition of an assembly tree (the reference's tree_pruning):
+    c, w, parent per node (postordered, 1-based parents, 0 for roots).
+    Returns (subtree roots, 1-based, in the reference's list order; the bin
+    of each, 0-based)."""
+    c, w, pa = _i32(c), _i32(w), _i32(parent)
+    m = len(c)
+    sub = np.empty(m, np.int32)
+    bins = np.empty(m, np.int32)
+    cnt = C.c_int()
+    rc = lib().mfd_tree_pruning(m, _ptr(c, C.c_int), _ptr(w, C.c_int), _ptr(pa, C.c_int), float(tol), int(p),
+                                _ptr(sub, C.c_int), _ptr(bins, C.c_int), C.byref(cnt))
+    if rc != MFD_OK:
+        raise MfdError(rc, "mfd_tree_pruning")
+    return sub[:cnt.value].copy(), bins[:cnt.value].copy()
 
 
 def read_matrix_market(path):

@@ -554,3 +554,60 @@ def cost(node_sizes, work_sizes):
 def nonzero(node_sizes, work_sizes):
     """cost_calculator_m.f90:22-33."""
     return sum(sum(range(w + 1, w + c + 1)) for c, w in zip(node_sizes, work_sizes))
+
+
+# ------------------------------------------------------------ tree pruning
+def count_op_count(cc, node_sizes):
+    """tree_pruning_m.f90:76-88: sum of m^2 for m = cc+1 .. cc+size."""
+    out = []
+    for l, size in zip(cc, node_sizes):
+        k = size + l
+        out.append(k * (k + 1) * (2 * k + 1) // 6 - l * (l + 1) * (2 * l + 1) // 6)
+    return out
+
+
+def generate_tree_weights(fcnt, tree_child):
+    """tree_pruning_m.f90:90-110 (children precede parents)."""
+    nodewt = []
+    for i, f in enumerate(fcnt):
+        nodewt.append(f + sum(nodewt[ch - 1] for ch in tree_child[i]))
+    return nodewt
+
+
+def partition_tree(lst, nodewt, tree_child, ip):
+    """tree_pruning_m.f90:112-134 with doubly_linked_list_m's add (at the
+    head) and find_node_with_max_op (:136-154, the first of the largest).
+    Returns False when the list runs out."""
+    while len(lst) < ip:
+        node, best = 0, 0
+        for v in lst:
+            if nodewt[v - 1] > best:
+                node, best = v, nodewt[v - 1]
+        if node == 0:
+            return False
+        for ch in tree_child[node - 1]:
+            lst.insert(0, ch)
+        lst.remove(node)
+    return True
+
+
+def tree_pruning(cc, node_sizes, tree_child, tol, p, roots=None):
+    """tree_pruning_m.f90:13-74: cc = work sizes, tree_child 1-based jagged.
+    roots: the list to start from (the reference starts from node n)."""
+    fcnt = count_op_count(cc, node_sizes)
+    nodewt = generate_tree_weights(fcnt, tree_child)
+    n = len(cc)
+    lst = list(roots) if roots is not None else [n]
+    ratio = 0.0
+    ip = p - 1
+    while ratio <= tol:
+        ip += 1
+        if not partition_tree(lst, nodewt, tree_child, ip):
+            return tree_pruning(cc, node_sizes, tree_child, 0.0, ip - 1, roots)
+        # op_heap (largest first) into bin_heap (least loaded first)
+        bins = [0] * ip
+        for wgt in sorted((nodewt[v - 1] for v in lst), reverse=True):
+            k = bins.index(min(bins))
+            bins[k] += wgt
+        ratio = min(bins) / max(bins)
+    return lst
