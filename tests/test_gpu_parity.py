@@ -3,6 +3,7 @@ the reference's seq_factorize / seq_forward / seq_backward) and against the
 reference's own known answers.  Tolerances: factor fronts and solutions
 within 1e-12 relative (normwise per front / per solution), backward error
 <= 1e-14; integer-valued known answers exactly."""
+import ctypes
 import sys
 
 import numpy as np
@@ -159,6 +160,33 @@ def test_multiple_rhs_and_device_entry_points():
         xk = s.solve(b[:, k])
         assert np.array_equal(xk, x[:, k])
     assert P.backward_error(p, x, b) <= 1e-14
+
+
+@pytest.mark.parametrize("precision", [mfdag.FP64, mfdag.FP32_IR])
+def test_more_right_hand_sides_than_one_group(precision):
+    """11 right-hand sides: the solve kernels take them 8 at a time, so the
+    second group is partial; each column equals its own single solve bit for
+    bit (FP64; the refined form is checked by its backward error)."""
+    p = P.weighted_laplacian_3d(11, seed=5)
+    s = mfdag.Solver(nb=32, graph=True)
+    s.analyze(p.ccs_col, p.ccs_row, 16, p.perm)
+    s.factorize(p.val, precision=precision)
+    xt, b = P.rhs(p, 11, seed=6)
+    x = s.solve(b)
+    if precision == mfdag.FP64:
+        for k in (0, 7, 8, 10):
+            assert np.array_equal(s.solve(b[:, k]), x[:, k]), k
+    assert P.backward_error(p, x, b) <= 1e-14
+    assert np.max(np.abs(x - xt)) <= 1e-8 * np.max(np.abs(xt))
+    # ldb > n through the C-ABI: the same solution, rows past n untouched
+    ld = p.n + 5
+    buf = np.full((ld, 11), 7.0, order="F")
+    buf[:p.n] = b
+    rc = mfdag.lib().mfd_solve(s.h, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 11, ld)
+    assert rc == 0
+    assert np.array_equal(buf[:p.n], x)
+    assert np.all(buf[p.n:] == 7.0)
+    s.close()
 
 
 def test_graph_replay_matches_eager():
