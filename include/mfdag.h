@@ -112,7 +112,10 @@ int mfd_set_structure(mfd_handle* h, int num_node, const int* supernode_size, co
                       const int* supernodal_index);
 
 /* hppllt_factorize (hppllt.f90:123-150): a holds the values in the caller's
-   CCS order (after mfd_set_structure: in supernodal CCS order). */
+   CCS order (after mfd_set_structure: in supernodal CCS order).  With
+   MFD_FP32_IR, an FP32 factorization that meets a non-positive pivot is
+   redone in FP64 (mfd_stats.precision then reads MFD_FP64); a positive
+   return means the FP64 factorization failed too. */
 int mfd_factorize(mfd_handle* h, const double* a, int precision);
 /* Same, with the values already in device memory (HBM-resident inputs). */
 int mfd_factorize_device(mfd_handle* h, const double* d_a, int precision);
@@ -158,7 +161,7 @@ int mfd_get_structure(mfd_handle* h, int* supernode_size, int* work_size, int* p
 typedef struct mfd_stats {
     double analyze_ms, assemble_ms, factor_ms, solve_ms;  /* last calls, wall (HIP events) */
     double factor_gflops;        /* cost / factor time (cost_calculator_m.f90:9-20) */
-    int precision;               /* of the current factor */
+    int precision;               /* of the current factor (MFD_FP64 after an FP32 factorization met a non-positive pivot) */
     int refine_iters;            /* last solve (< 0: -1 - iterations before the FP64 fallback) */
     /* last FP32_IR solve, for its final x (-1 after an FP64 solve, which
        computes no residual): */

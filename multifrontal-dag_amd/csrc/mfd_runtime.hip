@@ -939,6 +939,13 @@ int factorize_common(mfd_handle* h, int precision)
     if (precision == MFD_FP32_IR) {
         if (!h->d_pool32) MFD_TRY(dalloc(h, &h->d_pool32, h->pool_elems));
         rc = run_factor<float>(h);
+        if (rc > 0) {
+            // a pivot not positive in FP32: the FP64 refactorization the
+            // refinement falls back to when it stalls; only an FP64 failure
+            // says that the matrix is not positive definite
+            precision = MFD_FP64;
+            rc = run_factor<double>(h);
+        }
     } else {
         rc = run_factor<double>(h);
     }

@@ -256,6 +256,51 @@ def test_refactorize_same_structure_new_values(graph):
     s.close()
 
 
+def test_fp32_ir_falls_back_to_fp64_on_a_nearly_singular_matrix():
+    """A weighted graph Laplacian with a 1e-9 shift (condition ~1e12): FP32
+    cannot factor it usefully (a pivot of the shift's size drowns in FP32
+    rounding of the ~1e3 entries), so the precision form either refactors in
+    FP64 at once (non-positive FP32 pivot) or after refinement stalls; either
+    way the solve reaches the FP64 bound."""
+    p = P.weighted_laplacian_3d(12, seed=3, shift=1e-9)
+    s = mfdag.Solver(nb=32)
+    s.analyze(p.ccs_col, p.ccs_row, 16, p.perm)
+    assert s.factorize(p.val, precision=mfdag.FP32_IR) == 0
+    xt, b = P.rhs(p, 1, seed=8)
+    x = s.solve(b)
+    st = s.stats()
+    log("nearly singular: precision", st["precision"], "refine_iters", st["refine_iters"])
+    assert st["precision"] == mfdag.FP64 or st["refine_iters"] < 0
+    assert P.backward_error(p, x, b) <= 1e-14
+    s.close()
+
+
+def test_fp32_pivot_failure_refactors_in_fp64():
+    """An SPD matrix whose FP32 factorization meets a zero pivot: a 10 x 10
+    Laplacian plus a pair of unknowns with A = [[1, -(1 - 2^-30)], [., 1]]
+    (eigenvalue 2^-30; its off-diagonal rounds to -1 in FP32, leaving the
+    pair's second pivot exactly 0 there) tied to the grid by a 1e-8
+    coupling.  FP64 factors it; the precision form must refactor in FP64
+    instead of reporting a non-positive-definite matrix."""
+    g = P.laplacian_2d(10, leaf=1000)
+    n = g.n + 2
+    cols = np.repeat(np.arange(g.n), np.diff(g.ccs_col))
+    rows = g.ccs_row - 1
+    rows = np.concatenate([rows, [g.n, g.n + 1, g.n + 1, g.n]])
+    cols = np.concatenate([cols, [g.n, g.n + 1, g.n, 5]])
+    vals = np.concatenate([g.val, [1.0, 1.0, -(1.0 - 2.0 ** -30), -1e-8]])
+    cc, rr, vv = P.lower_ccs(n, rows, cols, vals)
+    s = mfdag.Solver(nb=16)
+    s.analyze(cc, rr, 0, None)
+    assert s.factorize(vv, precision=mfdag.FP32_IR) == 0
+    assert s.stats()["precision"] == mfdag.FP64
+    prob = P.Problem(n, cc, rr, vv)
+    xt, b = P.rhs(prob, 1, seed=4)
+    x = s.solve(b)
+    assert P.backward_error(prob, x, b) <= 1e-14
+    s.close()
+
+
 def test_not_positive_definite_reports_column():
     p = P.laplacian_2d(12)
     val = p.val.copy()
@@ -270,6 +315,10 @@ def test_not_positive_definite_reports_column():
     assert e.value.code >= 1
     with pytest.raises(mfdag.MfdError):
         s.solve(np.ones(p.n))   # no valid factor
+    # the precision form reports it too (its FP64 refactorization fails as well)
+    with pytest.raises(mfdag.NotPositiveDefinite) as e32:
+        s.factorize(val, precision=mfdag.FP32_IR)
+    assert e32.value.code == e.value.code
 
 
 def test_memory_budget_is_an_error(monkeypatch):
