@@ -301,6 +301,29 @@ def test_fp32_pivot_failure_refactors_in_fp64():
     s.close()
 
 
+def test_reanalysis_on_one_handle():
+    """One handle (captured graphs on) analysed again and again: a 3D
+    problem, a 2D one of another size, the 3D one again, then a batch; each
+    factorization and solve is correct, so nothing captured for an earlier
+    analysis is replayed."""
+    p3 = P.weighted_laplacian_3d(10, seed=2)
+    p2 = P.laplacian_2d(37)
+    systems = P.batch_anisotropic(6, seed=3, lo=12, hi=40)
+    s = mfdag.Solver(nb=32, graph=True)
+    for prob in (p3, p2, p3):
+        s.analyze(prob.ccs_col, prob.ccs_row, 16, prob.perm)
+        for precision in (mfdag.FP64, mfdag.FP32_IR):
+            s.factorize(prob.val, precision=precision)
+            xt, b = P.rhs(prob, 2, seed=1)
+            assert P.backward_error(prob, s.solve(b), b) <= 1e-14, (prob.name, precision)
+    s.analyze_batch([(q.ccs_col, q.ccs_row) for q in systems], 16, [q.perm for q in systems])
+    bp = P.batch_problem(systems)
+    s.factorize(bp.val)
+    xt, b = P.rhs(bp, 1, seed=2)
+    assert P.backward_error(bp, s.solve(b), b) <= 1e-14
+    s.close()
+
+
 def test_not_positive_definite_reports_column():
     p = P.laplacian_2d(12)
     val = p.val.copy()
