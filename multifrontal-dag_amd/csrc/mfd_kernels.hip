@@ -939,15 +939,6 @@ __global__ __launch_bounds__(kThreads) void k_fwd_add(double* zpool, int nrhs, c
     for (int r = 0; r < nrhs; ++r) Zp[p + (int64_t)r * pa.n] += Zc[(ch.c + i) + (int64_t)r * ch.n];
 }
 
-// Stage inv(L_tt) in LDS as LI[col][row] = L^{-1}[row][col] (the pool holds
-// L^{-T} column-major, i.e. inv[r + 64 c] = L^{-1}[c][r]).
-template <typename T>
-__device__ void load_inverse(const T* inv, double (*LI)[65])
-{
-#pragma unroll
-    for (int e = threadIdx.x; e < 64 * 64; e += kThreads) LI[e & 63][e >> 6] = (double)inv[e];
-}
-
 // Forward sweep, one launch per block step t (DTRSV "U","T" and DGEMV "T",
 // solve_kernel_m.f90:4-23), pipelined one block behind:
 //   - the block task (row0 < 0) solves block t: z = Z_t - L[B_t, B_{t-1}]
@@ -1031,11 +1022,13 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(const T* pool, double* zp
     const T* inv = pool + nd.inv + (int64_t)(t.k0 / 64) * 4096;
     // block (B_t, B_{t-1}): rows k0.., columns k0 - 64.. (only read when kp > 0)
     const T* Lq = kp > 0 ? pool + nd.front + t.k0 + (int64_t)(t.k0 - 64) * n : Lb;
+    // entries outside the kb x kb block read entry 0 (one line for all of
+    // them instead of the whole padded 64 x 64 block) and are zeroed below
     T li[16], lp[16];
 #pragma unroll
     for (int p = 0; p < 16; ++p) {
         const int e = tid + kThreads * p, i = e & 63, k = e >> 6;
-        li[p] = inv[e];
+        li[p] = ld_off(inv, (unsigned)((i < kb && k < kb) ? e : 0));
         lp[p] = ld_off(Lq, (unsigned)(min(i, kb - 1) + min(k, max(kp, 1) - 1) * n));
     }
     // the first right-hand-side group's z and y, in flight with the above
@@ -1054,6 +1047,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(const T* pool, double* zp
     for (int p = 0; p < 16; ++p) {
         const int e = tid + kThreads * p, i = e & 63, k = e >> 6;
         lp[p] = (i < kb && k < kp) ? lp[p] : T(0);
+        li[p] = (i < kb && k < kb) ? li[p] : T(0);
     }
 #pragma unroll
     for (int p = 0; p < 16; ++p) {
@@ -1248,11 +1242,12 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zp
     // block task: every global read first
     const int kn = t.pad;  // rows of block t+1 (0: none)
     const T* inv = pool + nd.inv + (int64_t)(t.k0 / 64) * 4096;
+    // entries outside the kb x kb block read entry 0 and are zeroed below
     T li[16], ln[16];
 #pragma unroll
     for (int p = 0; p < 16; ++p) {
         const int e = tid + kThreads * p, i = e & 63, k = e >> 6;
-        li[p] = inv[e];
+        li[p] = ld_off(inv, (unsigned)((i < kb && k < kb) ? e : 0));
         // (row k0 itself when there is no next block: always inside the front)
         ln[p] = ld_off(Lb, (unsigned)(t.k0 + (kn > 0 ? 64 + min(i, kn - 1) : 0) + min(k, kb - 1) * n));
     }
@@ -1281,6 +1276,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zp
     for (int p = 0; p < 16; ++p) {
         const int e = tid + kThreads * p, i = e & 63, k = e >> 6;
         ln[p] = (i < kn && k < kb) ? ln[p] : T(0);
+        li[p] = (i < kb && k < kb) ? li[p] : T(0);
     }
 #pragma unroll
     for (int p = 0; p < 16; ++p) {
