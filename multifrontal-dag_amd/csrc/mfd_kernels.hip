@@ -969,6 +969,78 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(const T* pool, double* zp
     double* Z = zpool + nd.z * nrhs;
     double* Y = Z + (int64_t)n * nrhs;
     const T* Lb = pool + nd.front + (int64_t)t.k0 * n;
+    if (t.pad0 == 1) {
+        // a front of one pivot block (k0 = 0, kb = c): y = inv(L_00) z, then
+        // its update rows 64 at a time, Z[rows] -= L[rows, 0:c] y
+        const T* inv = pool + nd.inv;
+        T li[16];
+#pragma unroll
+        for (int p = 0; p < 16; ++p) {
+            const int e = tid + kThreads * p, i = e & 63, k = e >> 6;
+            li[p] = ld_off(inv, (unsigned)((i < kb && k < kb) ? e : 0));
+        }
+#pragma unroll
+        for (int p = 0; p < 16; ++p) {
+            const int e = tid + kThreads * p, i = e & 63, k = e >> 6;
+            li[p] = (i < kb && k < kb) ? li[p] : T(0);
+        }
+        for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
+            const int nr = min(kRhsGroup, nrhs - r0);
+#pragma unroll
+            for (int p = 0; p < 16; ++p) {
+                const int e = tid + kThreads * p;
+                LI[e & 63][e >> 6] = (double)li[p];
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int e = tid + kThreads * h, k = e & 63, r = e >> 6;
+                zt[k][r] = (k < kb && r < nr) ? Z[k + (int64_t)(r0 + r) * n] : 0.0;
+            }
+            __syncthreads();
+            for (int r = ty; r < kRhsGroup; r += 4) {
+                double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int k = 0; k < 64; ++k) acc[k & 3] += LI[k][lane] * zt[k][r];
+                const double y = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+                if (lane < kb && r < nr) Y[lane + (int64_t)(r0 + r) * c] = y;
+                yt[lane][r] = (lane < kb && r < nr) ? y : 0.0;
+            }
+            __syncthreads();
+            for (int rr = c; rr < n; rr += 64) {
+                const int row = rr + lane;
+                const bool live = row < n;
+                const int rowc = min(row, n - 1);
+                T l[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) l[j] = ld_off(Lb, (unsigned)(rowc + min(16 * ty + j, kb - 1) * n));
+                const double z0 = (live && ty < nr) ? Z[row + (int64_t)(r0 + ty) * n] : 0.0;
+                const double z1 = (live && ty + 4 < nr) ? Z[row + (int64_t)(r0 + ty + 4) * n] : 0.0;
+                double acc[kRhsGroup];
+#pragma unroll
+                for (int r = 0; r < kRhsGroup; ++r) acc[r] = 0.0;
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const double lk = (double)l[j];
+#pragma unroll
+                    for (int r = 0; r < kRhsGroup; ++r) acc[r] += lk * yt[16 * ty + j][r];
+                }
+#pragma unroll
+                for (int r = 0; r < kRhsGroup; ++r) u.red[ty][lane][r] = acc[r];
+                __syncthreads();
+                if (live) {
+                    if (ty < nr)
+                        Z[row + (int64_t)(r0 + ty) * n] =
+                            z0 - ((u.red[0][lane][ty] + u.red[1][lane][ty]) + (u.red[2][lane][ty] + u.red[3][lane][ty]));
+                    if (ty + 4 < nr)
+                        Z[row + (int64_t)(r0 + ty + 4) * n] =
+                            z1 - ((u.red[0][lane][ty + 4] + u.red[1][lane][ty + 4]) +
+                                  (u.red[2][lane][ty + 4] + u.red[3][lane][ty + 4]));
+                }
+                __syncthreads();
+            }
+        }
+        return;
+    }
     if (t.row0 >= 0) {
         // 64 rows; thread (row lane, quarter ty) takes columns 16 ty .. 16 ty + 15
         const int row = t.row0 + lane;

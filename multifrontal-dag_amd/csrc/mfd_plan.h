@@ -46,7 +46,9 @@ struct DiagTask {
 // One block step of the blocked supernodal solves: the pivot block
 // [k0, k0+kb) of a front and, for the row-chunk tasks, the rows from row0
 // (forward: 64 rows; backward: 256; row0 < 0: the block's own task).
-// Forward: a block task's pad is the order of the previous pivot block.
+// Forward: a block task's pad is the order of the previous pivot block;
+// pad0 = 1 marks a front of one pivot block, whose task also applies it to
+// the front's update rows.
 // Backward: a chunk task writes columns 16 pad0 .. 16 pad0 + 15 of
 // partial-sum slot `slot` (64 x nrhs doubles); a block task reads the nchunk
 // slots from `slot` and pad is the order of the next pivot block (0 for the

@@ -50,6 +50,7 @@ struct Launch {
 // A step's update tiles join the step's launch (k_upd_diag, one workgroup
 // per CU at its LDS) up to this many; more run as their own launch.
 constexpr int kFillGroups = 32;  // most level groups the front zeroing is split into
+constexpr int kWholeRows = 128;   // update rows of a one-block front solved by its block task
 constexpr int kFuseTiles = 1024;
 
 // Step launches with at least this many diagonal tasks run their GEMM
@@ -309,11 +310,14 @@ void build_solve_plan(mfd_handle* h)
                 const int s = h->level_nodes[k];
                 const NodeDev& d = h->nodes[s];
                 const int T = (d.c + 63) / 64;
+                // a front of one block with few update rows: one task solves
+                // the block and applies it to those rows
+                const bool whole = T == 1 && d.w <= kWholeRows;
                 if (t <= T - 1) {
                     const int k0 = 64 * t, kb = std::min(64, d.c - k0);
-                    h->fstep.push_back(SolveTask{s, k0, kb, -1, 0, 0, 0, t > 0 ? 64 : 0});
+                    h->fstep.push_back(SolveTask{s, k0, kb, -1, 0, 0, whole ? 1 : 0, t > 0 ? 64 : 0});
                 }
-                if (t >= 1 && t - 1 <= T - 1) {
+                if (!whole && t >= 1 && t - 1 <= T - 1) {
                     // block t - 1 applied to the rows after block t
                     const int k0 = 64 * (t - 1), kb = std::min(64, d.c - k0);
                     for (int r0 = std::min(64 * (t + 1), d.c); r0 < d.n; r0 += 64)
