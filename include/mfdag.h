@@ -218,6 +218,32 @@ int mfd_tree_pruning(int num_node, const int* c, const int* w, const int* parent
 int mfd_mm_size(const char* path, int* n, int64_t* nnz);
 int mfd_mm_read(const char* path, int n, int64_t nnz, int* ccs_col, int* ccs_row, double* ccs_val);
 
+/* Schedule tuning (no counterpart in the reference, whose schedule is
+   StarPU's): the in-panel update of the large fronts runs every `period`
+   64-column steps with K = 64 period (default 2, 1..8), the choice the
+   tuner (mfdag/tune.py) makes.  On an analysed handle the schedule is
+   rebuilt at once and the factorization must be redone (mfd_solve returns
+   MFD_ERR_STATE until then); otherwise it applies to the next analysis. */
+int mfd_set_update_period(mfd_handle* h, int period);
+int mfd_get_update_period(mfd_handle* h);
+
+/* The tuner's cost model (no counterpart in the reference): a perceptron
+   with MFD_MLP_INPUTS inputs, MFD_MLP_HIDDEN tanh units and one linear
+   output, trained and evaluated by HIP kernels in FP64.  params
+   (MFD_MLP_PARAMS): W1[j][i] at j*MFD_MLP_INPUTS + i, then b1, w2 (one per
+   hidden unit), b2.  x: n x MFD_MLP_INPUTS row-major; t: n targets.  Loss =
+   mean over samples of (y - t)^2.  mfd_mlp_gradient returns its gradient
+   (MFD_MLP_PARAMS) and the loss; mfd_mlp_train runs `epochs` full-batch
+   Adam steps (learning rate lr; moments start at zero) on params in place
+   and returns the loss before the last step.  Host buffers; synchronous;
+   1 <= n <= 2^20. */
+#define MFD_MLP_INPUTS 4
+#define MFD_MLP_HIDDEN 32
+#define MFD_MLP_PARAMS (MFD_MLP_HIDDEN * (MFD_MLP_INPUTS + 2) + 1)
+int mfd_mlp_predict(int n, const double* x, const double* params, double* y);
+int mfd_mlp_gradient(int n, const double* x, const double* t, const double* params, double* grad, double* loss);
+int mfd_mlp_train(int n, const double* x, const double* t, double* params, int epochs, double lr, double* loss);
+
 const char* mfd_error_string(int code);
 
 #ifdef __cplusplus

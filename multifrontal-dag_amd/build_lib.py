@@ -15,7 +15,7 @@ SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "lib", "libmfdag.so")
 SOURCES = ["mfd_analysis.cpp", "mfd_layout.cpp", "mfd_mmio.cpp", "mfd_ordering.cpp", "mfd_subtree.cpp",
-           "mfd_kernels.hip", "mfd_runtime.hip"]
+           "mfd_kernels.hip", "mfd_runtime.hip", "mfd_mlp.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 instructions are disabled for all device code.
 FLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=gfx950", "-Wall",

@@ -61,11 +61,13 @@ constexpr int kApartDiag = 512;
 // into four 16-column tasks (shorter tasks where the step is latency-bound).
 constexpr int kSplitChunks = 256;
 
-// The in-panel update of the large fronts runs every kUpdPeriod steps with
-// K = kUpdPeriod panels (fewer passes over the trailing columns); the TRSM
-// tiles and the diagonal task of a step apply the panels still pending for
-// their columns themselves (at most kUpdPeriod, 64 columns each).
+// The in-panel update of the large fronts runs every `period` steps with
+// K = period panels (fewer passes over the trailing columns); the TRSM tiles
+// and the diagonal task of a step apply the panels still pending for their
+// columns themselves (at most period, 64 columns each).  The handle's period
+// defaults to kUpdPeriod; mfd_set_update_period changes it (1..kMaxPeriod).
 constexpr int kUpdPeriod = 2;
+constexpr int kMaxPeriod = 8;
 
 // Update and Schur GEMMs: 128 x 128 tiles (k_gemm128) or 64 x 64 (k_gemm).
 constexpr bool kUseWide = false;
@@ -95,6 +97,7 @@ double now_ms()
 
 struct mfd_handle {
     int nb = 0, flags = 0;
+    int upd_period = kUpdPeriod;
     bool device_ok = false;
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;  // large in-panel updates beside the diagonal chain
@@ -398,6 +401,7 @@ void build_plan(mfd_handle* h)
 {
     const Structure& S = h->S;
     const int m = S.num_node;
+    const int period = h->upd_period;
     h->nodes.assign(m, NodeDev{});
     int64_t off = 0, zoff = 0, inv_blocks = 0;
     for (int s = 0; s < m; ++s) {
@@ -529,10 +533,10 @@ void build_plan(mfd_handle* h)
         for (int t = 0; t < steps; ++t) {
             const int k0 = t * kPanel;
             // panels not yet applied to block column t (and to block t+1):
-            // the in-panel update runs every kUpdPeriod steps, on the
-            // kUpdPeriod panels before it
-            const int pend = t == 0 ? 0 : ((t % kUpdPeriod) == 0 ? kUpdPeriod : t % kUpdPeriod) * kPanel;
-            const bool upd = t >= kUpdPeriod && t % kUpdPeriod == 0;
+            // the in-panel update runs every period steps, on the period
+            // panels before it
+            const int pend = t == 0 ? 0 : ((t % period) == 0 ? period : t % period) * kPanel;
+            const bool upd = t >= period && t % period == 0;
             int64_t f1 = (int64_t)h->gemm.size();
             for (int s : large) {
                 const NodeDev& d = h->nodes[s];
@@ -564,7 +568,7 @@ void build_plan(mfd_handle* h)
             }
             int64_t f3 = (int64_t)h->gemm.size();
             if (upd) {
-                const int kq = k0 - kUpdPeriod * kPanel;  // panels t-kUpdPeriod .. t-1
+                const int kq = k0 - period * kPanel;  // panels t-period .. t-1
                 for (int s : large) {
                     const NodeDev& d = h->nodes[s];
                     for (int c0 = k0 + kPanel; c0 < d.c; c0 += kTile) {
@@ -581,7 +585,7 @@ void build_plan(mfd_handle* h)
                             g.ldc = g.lda = g.ldb = d.n;
                             g.m = m;
                             g.n = nc;
-                            g.k = kUpdPeriod * kPanel;
+                            g.k = period * kPanel;
                             g.tri = rs == c0;
                             h->gemm.push_back(g);
                         }
@@ -1290,6 +1294,25 @@ int mfd_set_structure(mfd_handle* h, int num_node, const int* c, const int* w, c
     h->from_parts = true;
     return finish_analysis(h);
 }
+
+int mfd_set_update_period(mfd_handle* h, int period)
+{
+    if (!h || period < 1 || period > kMaxPeriod) return MFD_ERR_ARG;
+    if (period == h->upd_period) return MFD_OK;
+    h->upd_period = period;
+    if (!h->analyzed) return MFD_OK;
+    // the schedule of the current analysis is rebuilt; its factor is gone
+    h->factored = false;
+    if (!h->device_ok) return MFD_OK;
+    int rc = setup_device(h);
+    if (rc != MFD_OK) {
+        free_device(h);
+        h->analyzed = false;
+    }
+    return rc;
+}
+
+int mfd_get_update_period(mfd_handle* h) { return h ? h->upd_period : MFD_ERR_ARG; }
 
 int mfd_factorize(mfd_handle* h, const double* a, int precision)
 {

@@ -87,6 +87,11 @@ def lib():
             "mfd_tree_pruning": (i, [i, ip, ip, ip, d, i, ip, ip, ip]),
             "mfd_mm_size": (i, [C.c_char_p, ip, lp]),
             "mfd_mm_read": (i, [C.c_char_p, i, i64, ip, ip, dp]),
+            "mfd_set_update_period": (i, [P, i]),
+            "mfd_get_update_period": (i, [P]),
+            "mfd_mlp_predict": (i, [i, dp, dp, dp]),
+            "mfd_mlp_gradient": (i, [i, dp, dp, dp, dp, dp]),
+            "mfd_mlp_train": (i, [i, dp, dp, dp, i, d, dp]),
             "mfd_error_string": (C.c_char_p, [i]),
         }
         for name, (res, args) in sig.items():
@@ -154,6 +159,48 @@ def read_matrix_market(path):
     return n.value, col, row, val
 
 
+MLP_INPUTS, MLP_HIDDEN = 4, 32
+MLP_PARAMS = MLP_HIDDEN * (MLP_INPUTS + 2) + 1
+
+
+def _f64(a):
+    return np.ascontiguousarray(np.asarray(a), dtype=np.float64)
+
+
+def _mlp_check(rc, what):
+    if rc != MFD_OK:
+        raise MfdError(rc, what)
+
+
+def mlp_predict(params, x):
+    """The cost model's outputs for the rows of x (n x MLP_INPUTS), on the GPU."""
+    x, p = _f64(x), _f64(params)
+    y = np.empty(len(x))
+    _mlp_check(lib().mfd_mlp_predict(len(x), _ptr(x, C.c_double), _ptr(p, C.c_double), _ptr(y, C.c_double)),
+               "mfd_mlp_predict")
+    return y
+
+
+def mlp_gradient(params, x, t):
+    """(gradient, loss) of the mean squared error at params, on the GPU."""
+    x, t, p = _f64(x), _f64(t), _f64(params)
+    g = np.empty(MLP_PARAMS)
+    loss = C.c_double()
+    _mlp_check(lib().mfd_mlp_gradient(len(x), _ptr(x, C.c_double), _ptr(t, C.c_double), _ptr(p, C.c_double),
+                                      _ptr(g, C.c_double), C.byref(loss)), "mfd_mlp_gradient")
+    return g, loss.value
+
+
+def mlp_train(params, x, t, epochs, lr):
+    """(trained params, loss before the last step): full-batch Adam on the GPU."""
+    x, t = _f64(x), _f64(t)
+    p = _f64(params).copy()
+    loss = C.c_double()
+    _mlp_check(lib().mfd_mlp_train(len(x), _ptr(x, C.c_double), _ptr(t, C.c_double), _ptr(p, C.c_double),
+                                   int(epochs), float(lr), C.byref(loss)), "mfd_mlp_train")
+    return p, loss.value
+
+
 class Solver:
     """One analysis + factorization + solves (one C-ABI handle).
 
@@ -188,6 +235,15 @@ class Solver:
         if rc < 0:
             raise MfdError(rc, what)
         return rc
+
+    @property
+    def update_period(self):
+        """In-panel update period of the large fronts (64-column steps)."""
+        return lib().mfd_get_update_period(self.h)
+
+    @update_period.setter
+    def update_period(self, period):
+        self._check(lib().mfd_set_update_period(self.h, int(period)), "mfd_set_update_period")
 
     # -------------------------------------------------------------- analysis
     def analyze(self, ccs_col, ccs_row, max_zero=0, perm=None):
