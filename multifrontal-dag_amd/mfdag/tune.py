@@ -42,20 +42,27 @@ def features(n, period, w, elem_bytes):
     return [math.log2(n), float(period), math.log2(w + 1), elem_bytes / 4.0]
 
 
-def measure(solver, a, precision, periods=PERIODS, reps=3):
+def measure(solver, a, precision, periods=PERIODS, reps=3, rhs=None):
     """{period: median factor ms} for an analysed solver (graph replays when
     the solver was created with graph=True; the first factorization of each
-    period, which captures the graph, is not timed).  Leaves the solver with
-    the fastest period and a factorization made with it."""
+    period, which captures the graph, is not timed).  With rhs, the time is
+    that of factor + solve: in the FP32+IR form the period changes the
+    factor's rounding and with it, now and then, the number of refinement
+    iterations.  Leaves the solver with the fastest period and a
+    factorization made with it."""
     times = {}
     for p in periods:
         solver.update_period = p
         solver.factorize(a, precision)
         t = []
-        for _ in range(reps):
+        for _ in range(reps + (rhs is not None)):
             solver.factorize(a, precision)
-            t.append(solver.stats()["factor_ms"])
-        times[p] = float(np.median(t))
+            ms = solver.stats()["factor_ms"]
+            if rhs is not None:
+                solver.solve(rhs)
+                ms += solver.stats()["solve_ms"]
+            t.append(ms)
+        times[p] = float(np.median(t[-reps:]))
     best = min(times, key=times.get)
     solver.update_period = best
     solver.factorize(a, precision)
