@@ -67,6 +67,18 @@ def timed_steps(s, d_a, d_b, d_x, precision, steps, warmup):
     return wall, float(np.median(fms)), float(np.median(sms))
 
 
+def tune_leg(args, s, a, b, prec):
+    """The exhaustive tuner's update period for one leg and form (by factor +
+    solve time), set on the solver; None with --no-tune."""
+    if args.no_tune:
+        return None
+    from mfdag import tune
+    t = tune.measure(s, a, prec, rhs=b)
+    best = min(t, key=t.get)
+    s.update_period = best
+    return {"period": best, "step_ms": {str(k): round(v, 3) for k, v in t.items()}}
+
+
 def batch_leg(args):
     """The batching workload: 512 anisotropic 2D Laplacians (L ~ U{16..160},
     y-coupling log-uniform in [1e-3, 1], A_ii = 2 + 2 c_y), one right-hand
@@ -95,12 +107,13 @@ def batch_leg(args):
                        f"A_ii=2+2c_y, ND per system, max_zero={args.batch_max_zero}, 1 RHS each",
            "n": bp.n, "fronts": s.num_node, "cost_flops": s.cost(), "analyze_s": round(analyze_s, 3)}
     for name, prec in (("fp64", mfdag.FP64), ("fp32_ir", mfdag.FP32_IR)):
+        tuned = tune_leg(args, s, bp.val, b, prec)
         wall, f, so = timed_steps(s, d_a, d_b, d_x, prec, args.steps, args.warmup)
         x = d_x.cpu().numpy().T
         st = s.stats()
         out[name] = {"ms_per_step": round(wall * 1e3, 3), "factor_ms": round(f, 3), "solve_ms": round(so, 3),
                      "factor_gflops": round(s.cost() / (f * 1e6), 2),
-                     "backward_error": P.batch_backward_error(systems, x, b)}
+                     "backward_error": P.batch_backward_error(systems, x, b), "tuner": tuned}
         if prec == mfdag.FP32_IR:
             out[name]["refine_iters"] = st["refine_iters"]
         log(f"batch {name}: step {wall * 1e3:.2f} ms (factor {f:.2f} ms, solve {so:.2f} ms), "
@@ -131,10 +144,12 @@ def nd_leg(args, p, b):
     out = {"ordering": "mfd_order_nd (graph nested dissection)", "order_s": round(order_s, 3),
            "fronts": s.num_node, "cost_flops": s.cost()}
     for name, prec in (("fp64", mfdag.FP64), ("fp32_ir", mfdag.FP32_IR)):
+        tuned = tune_leg(args, s, p.val, b, prec)
         wall, f, so = timed_steps(s, d_a, d_b, d_x, prec, args.steps, args.warmup)
         x = d_x.cpu().numpy().T
         out[name] = {"ms_per_step": round(wall * 1e3, 3), "factor_ms": round(f, 3), "solve_ms": round(so, 3),
-                     "factor_gflops": round(s.cost() / (f * 1e6), 2), "backward_error": P.backward_error(p, x, b)}
+                     "factor_gflops": round(s.cost() / (f * 1e6), 2), "backward_error": P.backward_error(p, x, b),
+                     "tuner": tuned}
         if prec == mfdag.FP32_IR:
             out[name]["refine_iters"] = s.stats()["refine_iters"]
         log(f"graph ND {name}: step {wall * 1e3:.2f} ms (factor {f:.2f} ms, solve {so:.2f} ms), "
