@@ -103,6 +103,26 @@ def test_arguments_checked_without_compute():
         assert e.value.code == mfdag.MFD_ERR_NODEVICE
 
 
+def test_regret_and_gain_bookkeeping():
+    class Fixed:
+        def __init__(self, p):
+            self.p = p
+
+        def choose(self, n, w, elem_bytes, periods):
+            assert periods == (1, 2, 3)
+            return self.p
+
+    recs = [dict(problem="a", precision=0, period=k, n=900, w=0, elem_bytes=8, ms=ms)
+            for k, ms in ((1, 10.0), (2, 8.0), (3, 9.0))]
+    recs += [dict(problem="a", precision=1, period=k, n=900, w=0, elem_bytes=4, ms=ms)
+             for k, ms in ((1, 5.0), (2, 6.0), (3, 7.0))]
+    r = tune.regret(Fixed(3), recs)
+    assert [(q["precision"], q["best_period"]) for q in r] == [(0, 2), (1, 1)]
+    assert r[0]["regret"] == pytest.approx(9.0 / 8.0 - 1.0) and r[0]["gain"] == pytest.approx(8.0 / 9.0)
+    assert r[1]["regret"] == pytest.approx(0.4) and r[1]["gain"] == pytest.approx(6.0 / 7.0)
+    assert tune.features(1024, 3, 0, 8) == [10.0, 3.0, 0.0, 2.0]
+
+
 # ------------------------------------------------------------------- GPU
 @pytest.mark.gpu
 def test_mlp_kernels_against_oracle():
