@@ -73,7 +73,7 @@ def tune_leg(args, s, a, b, prec):
     if args.no_tune:
         return None
     from mfdag import tune
-    t = tune.measure(s, a, prec, rhs=b)
+    t = tune.measure(s, a, prec, tune.PERIODS + (tune.AUTO,), rhs=b)
     best = min(t, key=t.get)
     s.update_period = best
     return {"period": best, "step_ms": {str(k): round(v, 3) for k, v in t.items()}}
@@ -320,7 +320,7 @@ def main():
         from mfdag import tune
         tuned = {}
         for name, prec in (("fp64", mfdag.FP64), ("fp32_ir", mfdag.FP32_IR)):
-            t = tune.measure(s, p.val, prec, rhs=b)
+            t = tune.measure(s, p.val, prec, tune.PERIODS + (tune.AUTO,), rhs=b)
             tuned[name + "_step_ms"] = {str(k): round(v, 3) for k, v in t.items()}
             tuned[name + "_period"] = min(t, key=t.get)
         log("tuner:", json.dumps(tuned))

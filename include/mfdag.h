@@ -220,8 +220,9 @@ int mfd_mm_read(const char* path, int n, int64_t nnz, int* ccs_col, int* ccs_row
 
 /* Schedule tuning (no counterpart in the reference, whose schedule is
    StarPU's): the in-panel update of the large fronts runs every `period`
-   64-column steps with K = 64 period (default 2, 1..8), the choice the
-   tuner (mfdag/tune.py) makes.  On an analysed handle the schedule is
+   64-column steps with K = 64 period (default 2, 1..8; 0 = per level: 1
+   where the level's longest front has fewer than 48 steps, else 2), the
+   choice the tuner (mfdag/tune.py) makes.  On an analysed handle the schedule is
    rebuilt at once and the factorization must be redone (mfd_solve returns
    MFD_ERR_STATE until then); otherwise it applies to the next analysis. */
 int mfd_set_update_period(mfd_handle* h, int period);

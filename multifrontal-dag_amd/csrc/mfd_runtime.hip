@@ -65,9 +65,11 @@ constexpr int kSplitChunks = 256;
 // K = period panels (fewer passes over the trailing columns); the TRSM tiles
 // and the diagonal task of a step apply the panels still pending for their
 // columns themselves (at most period, 64 columns each).  The handle's period
-// defaults to kUpdPeriod; mfd_set_update_period changes it (1..kMaxPeriod).
+// defaults to kUpdPeriod; mfd_set_update_period changes it (1..kMaxPeriod, or
+// 0: chosen per level from the length of its longest chain).
 constexpr int kUpdPeriod = 2;
 constexpr int kMaxPeriod = 8;
+constexpr int kAutoSteps = 48;  // 3072 columns
 
 // Update and Schur GEMMs: 128 x 128 tiles (k_gemm128) or 64 x 64 (k_gemm).
 constexpr bool kUseWide = false;
@@ -401,7 +403,7 @@ void build_plan(mfd_handle* h)
 {
     const Structure& S = h->S;
     const int m = S.num_node;
-    const int period = h->upd_period;
+    const int period_set = h->upd_period;
     h->nodes.assign(m, NodeDev{});
     int64_t off = 0, zoff = 0, inv_blocks = 0;
     for (int s = 0; s < m; ++s) {
@@ -524,6 +526,9 @@ void build_plan(mfd_handle* h)
         // with K = c closes the level.
         int steps = 0;
         for (int s : large) steps = std::max(steps, (h->nodes[s].c + kPanel - 1) / kPanel);
+        // period 0 (automatic): 1 for levels whose longest chain is short,
+        // where light step tasks pay most, 2 for the long ones
+        const int period = period_set > 0 ? period_set : (steps >= kAutoSteps ? 2 : 1);
         {
             int64_t f0 = (int64_t)h->diag.size();
             for (int s : large) h->diag.push_back(DiagTask{s, 0, std::min(kPanel, h->nodes[s].c), 0, 0, 0, 0, 0});
@@ -1297,7 +1302,7 @@ int mfd_set_structure(mfd_handle* h, int num_node, const int* c, const int* w, c
 
 int mfd_set_update_period(mfd_handle* h, int period)
 {
-    if (!h || period < 1 || period > kMaxPeriod) return MFD_ERR_ARG;
+    if (!h || period < 0 || period > kMaxPeriod) return MFD_ERR_ARG;
     if (period == h->upd_period) return MFD_OK;
     h->upd_period = period;
     if (!h->analyzed) return MFD_OK;

@@ -25,6 +25,7 @@ import numpy as np
 from . import FP64, FP32_IR, MLP_HIDDEN, MLP_INPUTS, MLP_PARAMS, mlp_predict, mlp_train
 
 PERIODS = (1, 2, 3, 4)
+AUTO = 0  # per level: 1 for levels whose longest front has < 48 steps, else 2
 DEFAULT_PERIOD = 2
 
 

@@ -85,7 +85,7 @@ def test_arguments_checked_without_compute():
     assert e.value.code == mfdag.MFD_ERR_ARG
     s = mfdag.Solver(host_only=True)
     assert s.update_period == 2
-    for bad in (0, 9):
+    for bad in (-1, 9):
         with pytest.raises(mfdag.MfdError):
             s.update_period = bad
     s.update_period = 3
@@ -155,7 +155,7 @@ def test_every_update_period_gives_the_oracle_factor():
     f, st, info = oracle_for(s, p.val)
     assert info == 0
     xt, b = P.rhs(p, 2, seed=4)
-    for period in (1, 3, 4, 8, 2):
+    for period in (1, 3, 4, 8, 0, 2):
         s.update_period = period
         with pytest.raises(mfdag.MfdError):  # the schedule changed: factor again first
             s.solve(b)
