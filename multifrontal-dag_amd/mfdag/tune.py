@@ -16,7 +16,7 @@ Two ways to choose it:
     problem's front with the most pivot columns -- order n, update width
     K (64 period), update rows w, bytes per entry -- and picks the period with the smallest
     prediction, without a sweep.  The measured tuner is its fallback and the
-    judge of its regret (regret below).
+    yardstick of its regret (regret below).
 """
 import math
 

@@ -8,7 +8,7 @@ of a network with zero output weights, Adam's first step -- and the C-ABI's
 argument checks (no compute without a GPU: valid calls report no device).
 GPU: the kernels against the restatement (prediction, gradient, 300 Adam
 steps), every period giving the oracle's factor, and the tuner: records
-from the exhaustive sweep, the model trained on some problems and judged by
+from the exhaustive sweep, the model trained on some problems and measured by
 its regret on others."""
 import sys
 
