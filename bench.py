@@ -74,7 +74,7 @@ def tune_leg(args, s, a, b, prec):
         return None
     from mfdag import tune
     t = tune.measure(s, a, prec, tune.PERIODS + (tune.AUTO,), rhs=b)
-    best = min(t, key=t.get)
+    best = tune.pick(t)
     s.update_period = best
     return {"period": best, "step_ms": {str(k): round(v, 3) for k, v in t.items()}}
 
@@ -322,7 +322,7 @@ def main():
         for name, prec in (("fp64", mfdag.FP64), ("fp32_ir", mfdag.FP32_IR)):
             t = tune.measure(s, p.val, prec, tune.PERIODS + (tune.AUTO,), rhs=b)
             tuned[name + "_step_ms"] = {str(k): round(v, 3) for k, v in t.items()}
-            tuned[name + "_period"] = min(t, key=t.get)
+            tuned[name + "_period"] = tune.pick(t)
         log("tuner:", json.dumps(tuned))
         s.update_period = tuned["fp64_period"]
     wall64, f64, s64 = timed_steps(s, d_a, d_b, d_x, mfdag.FP64, args.steps, args.warmup)

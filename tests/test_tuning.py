@@ -121,6 +121,9 @@ def test_regret_and_gain_bookkeeping():
     assert r[0]["regret"] == pytest.approx(9.0 / 8.0 - 1.0) and r[0]["gain"] == pytest.approx(8.0 / 9.0)
     assert r[1]["regret"] == pytest.approx(0.4) and r[1]["gain"] == pytest.approx(6.0 / 7.0)
     assert tune.features(1024, 3, 0, 8) == [10.0, 3.0, 0.0, 2.0]
+    assert tune.pick({1: 10.0, 2: 10.04, 3: 11.0}) == 2      # within 0.5%: the default stays
+    assert tune.pick({1: 9.9, 2: 10.0, 3: 11.0}) == 1        # 1% faster: moves
+    assert tune.pick({1: 9.9, 3: 11.0}) == 1
 
 
 # ------------------------------------------------------------------- GPU
