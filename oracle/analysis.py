@@ -219,6 +219,19 @@ def iperm_of(perm):
     return ip
 
 
+def permutate(perm, data):
+    """perm_m.f90 permutate_DP / permutate_int: out(i) = in(perm(i))."""
+    return [data[p - 1] for p in perm]
+
+
+def inverse_permutate(perm, data):
+    """perm_m.f90 inverse_permutate: out(perm(i)) = in(i)."""
+    out = [None] * len(data)
+    for i, p in enumerate(perm):
+        out[p - 1] = data[i]
+    return out
+
+
 def perm_product(first, second):
     """perm_m.f90:25-36: out(i) = first(second(i))."""
     return [first[s - 1] for s in second]
@@ -264,6 +277,17 @@ def reordering_ccs(ccs, perm, iperm):
         out.append([t[0] for t in col])
         cperm.extend(t[1] for t in col)
     return out, cperm
+
+
+def reordering_ccs_val(origin_lengths, reorder_lengths, origin_val, perm):
+    """reordering_m.f90:150-170: column perm(c) of the values moves, whole,
+    to column c."""
+    of = sets_from_lengths(origin_lengths)
+    out = []
+    for c in range(1, len(reorder_lengths) + 1):
+        o = perm[c - 1]
+        out.extend(origin_val[of[o - 1] - 1:of[o] - 1])
+    return out
 
 
 def reordering_tree(parent, perm, iperm):
@@ -401,23 +425,53 @@ def compute_merge_lists(cc, tree_child, lengths, max_zero):
     return merge, sons
 
 
-def relax_supernode(fund_lengths, fund_cc, fund_tree_child, fund_ccs, max_zero):
-    """analyse_phase_m.f90:91-108 with relaxed_supernode_m.f90:15-112."""
-    merge, sons = compute_merge_lists(fund_cc, fund_tree_child, fund_lengths, max_zero)
-    nf = len(fund_lengths)
-    fmap = [i for i in range(1, nf + 1) if merge.head[i] != 0]          # build_map
+def build_map(merge):
+    """relaxed_supernode_m.f90:15-31: the lists still holding nodes, in index
+    order (each is one relaxed supernode)."""
+    return [i for i in range(1, merge.num_elements() + 1) if merge.head[i] != 0]
+
+
+def create_node_sets(fund_lengths, fmap, merge):
+    """relaxed_supernode_m.f90:64-86: columns of each relaxed supernode."""
+    return [sum(fund_lengths[x - 1] for x in merge.items(m)) for m in fmap]
+
+
+def create_perm(fund_lengths, fmap, merge):
+    """relaxed_supernode_m.f90:33-62: each merge list pushed onto a stack in
+    list order and popped (last pushed first), each popped fundamental
+    supernode contributing its columns in order."""
     fund_first = sets_from_lengths(fund_lengths)
-    rel_lengths = [sum(fund_lengths[x - 1] for x in merge.items(m)) for m in fmap]  # create_node_sets
-    perm = []                                                            # create_perm
+    perm = []
     for m in fmap:
-        stack = list(merge.items(m))
+        stack = []
+        for node in merge.items(m):
+            stack.append(node)
         while stack:
             node = stack.pop()
             perm.extend(range(fund_first[node - 1], fund_first[node]))
-    num_child = [len(sons.items(m)) for m in fmap]                       # count_num_child
+    return perm
+
+
+def count_num_child(sons, fmap):
+    """relaxed_supernode_m.f90:88-99: only the son lists' lengths are used."""
+    return [len(sons.items(m)) for m in fmap]
+
+
+def build_cc(fund_cc, fmap):
+    """relaxed_supernode_m.f90:101-112."""
+    return [fund_cc[m - 1] for m in fmap]
+
+
+def relax_supernode(fund_lengths, fund_cc, fund_tree_child, fund_ccs, max_zero):
+    """analyse_phase_m.f90:91-108 with relaxed_supernode_m.f90:15-112."""
+    merge, sons = compute_merge_lists(fund_cc, fund_tree_child, fund_lengths, max_zero)
+    fmap = build_map(merge)
+    rel_lengths = create_node_sets(fund_lengths, fmap, merge)
+    perm = create_perm(fund_lengths, fmap, merge)
+    num_child = count_num_child(sons, fmap)
     parent = parent_in_postordering_tree(num_child)
     tree_child = tree_child_from_num_child(num_child, parent)
-    cc = [fund_cc[m - 1] for m in fmap]                                  # build_cc
+    cc = build_cc(fund_cc, fmap)
     iperm = iperm_of(perm)
     ccs = repostordering_ccs(fund_ccs, perm, iperm)
     return dict(lengths=rel_lengths, perm=perm, tree_child=tree_child, cc=cc, ccs=ccs,

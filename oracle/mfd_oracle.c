@@ -41,6 +41,13 @@ static int div_ceiling(int a, int p) { return (a - 1) / p + 1; }
 static int mod2(int a, int p) { return (a - 1) % p + 1; }
 /* block_size_calculator_m.f90:12-21 */
 static int block_size(int idx, int nb, int n) { return (i64)idx * nb > n ? n % nb : nb; }
+/* get_block_size2, block_size_calculator_m.f90:23-36: a first block of
+   first_block entries (0: none), then blocks of nb */
+static int block_size2(int idx, int nb, int n, int first_block)
+{
+    if (first_block == 0) return block_size(idx, nb, n);
+    return idx == 1 ? first_block : block_size(idx - 1, nb, n - first_block);
+}
 
 /* ------------------------------------------------------------ tile kernels */
 #define CM(a, ld, r, q) (a)[((i64)(q) - 1) * (ld) + ((r) - 1)]
@@ -210,6 +217,10 @@ static int work_block_size(const orc_t *h, int idx, int s)
     return idx == work_start(h, s) ? h->wsz[s] : block_size(idx, h->nb, n_of(h, s));
 }
 static int divisible(const orc_t *h, int s) { return h->ss[s] == 0; }
+/* node_data_m.f90 get_matrix_num: the tile holding front position idx */
+static int matrix_num(const orc_t *h, int idx) { return div_ceiling(idx, h->nb); }
+/* node_data_m.f90 get_work_num: the tile holding update row idx of node s */
+static int work_num(const orc_t *h, int idx, int s) { return div_ceiling(h->c[s] + idx, h->nb); }
 static int exist_border(const orc_t *h, int s) { return !divisible(h, s) && h->w[s] != 0; }
 enum { EX_SUPERNODE = 1, EX_BORDER = 2, EX_WORK = 3 };
 static int extractor_type(const orc_t *h, int idx, int s)
@@ -374,8 +385,8 @@ orc_t *orc_create(int num_node, const int *c, const int *w, const int *parent, i
         if (s < num_node) {
             int prev_p = 0, prev_c = 0;
             for (int i = 1; i <= h->w[s]; ++i) {
-                int pn = (h->lidx[h->li_off[s] + i] - 1) / nb + 1;
-                int cn = (h->c[s] + i - 1) / nb + 1;
+                int pn = matrix_num(h, h->lidx[h->li_off[s] + i]);
+                int cn = work_num(h, i, s);
                 if (pn != prev_p || cn != prev_c) { prev_p = pn; prev_c = cn; ++ng; }
             }
         }
@@ -391,8 +402,8 @@ orc_t *orc_create(int num_node, const int *c, const int *w, const int *parent, i
         int prev_p = 0, prev_c = 0;
         i64 g = h->grp_off[s];
         for (int i = 1; i <= h->w[s]; ++i) {
-            int pn = (h->lidx[h->li_off[s] + i] - 1) / nb + 1;
-            int cn = (h->c[s] + i - 1) / nb + 1;
+            int pn = matrix_num(h, h->lidx[h->li_off[s] + i]);
+            int cn = work_num(h, i, s);
             if (pn != prev_p || cn != prev_c) {
                 prev_p = pn; prev_c = cn; ++g;
                 h->grp_start[g] = i;
@@ -732,6 +743,46 @@ i64 orc_tile(orc_t *h, int which, int s, int i, int j, int *rows, int *cols)
     }
 }
 int orc_num_matrix_block(orc_t *h, int s) { return num_matrix_block(h, s); }
+/* The layout quantities the reference's layout tests assert, for the
+   checker's known-answer tests (node s and indices 1-based):
+    0 divisible(s)              1 border_supernode_size(s)   2 border_work_size(s)
+    3 num_matrix_block(s)       4 work_start_index(s)        5 matrix_block_size(i, s)
+    6 supernode_block_size(i,s) 7 work_block_size(i, s)      8 matrix_num(i)
+    9 work_num(i, s)
+   10/11/12 entries of node s in the supernode / work / border factor array
+   13/14/15 0-based start of tile (i, j) of node s within its part of that array
+   16/17/18 entries of node s in the supernode / work / border rhs array
+   19/20/21 0-based start of block i of node s within its part of that rhs array
+   (get_block_size with a first block is orc_block_size) */
+i64 orc_layout(orc_t *h, int what, int s, int i, int j)
+{
+    switch (what) {
+    case 0: return divisible(h, s);
+    case 1: return h->ss[s];
+    case 2: return h->wsz[s];
+    case 3: return num_matrix_block(h, s);
+    case 4: return work_start(h, s);
+    case 5: return matrix_block_size(h, i, s);
+    case 6: return supernode_block_size(h, i, s);
+    case 7: return work_block_size(h, i, s);
+    case 8: return matrix_num(h, i);
+    case 9: return work_num(h, i, s);
+    case 10: return sn_node_size(h, s);
+    case 11: return wk_node_size(h, s);
+    case 12: return bd_node_size(h, s);
+    case 13: return sn_tile_pos(h, s, i, j);
+    case 14: return wk_tile_pos(h, s, i, j);
+    case 15: return bd_tile_pos(h, s, i, j);
+    case 16: return rsn_size(h, s);
+    case 17: return rwk_size(h, s);
+    case 18: return rbd_size(h, s);
+    case 19: return rh_supernode(h, s, i) - (h->rsn + h->rsn_off[s]);
+    case 20: return rh_work(h, s, i) - (h->rwk + h->rwk_off[s]);
+    case 21: return rh_border(h, s, i) - (h->rbd + h->rbd_off[s]);
+    default: return -1;
+    }
+}
+int orc_block_size(int idx, int nb, int n, int first_block) { return block_size2(idx, nb, n, first_block); }
 int orc_work_start(orc_t *h, int s) { return work_start(h, s); }
 int orc_num_groups(orc_t *h, int s) { return (int)(h->grp_off[s + 1] - h->grp_off[s]); }
 /* group g of node s: fills start_row, parent tile, child tile, block offset,

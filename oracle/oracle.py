@@ -51,6 +51,10 @@ def lib():
         L.orc_group.restype = i
         L.orc_group.argtypes = [P, i, i] + [C.POINTER(i)] * 4 + [ip]
         L.orc_front.argtypes = [P, i, dp]
+        L.orc_layout.restype = C.c_int64
+        L.orc_layout.argtypes = [P, i, i, i, i]
+        L.orc_block_size.restype = i
+        L.orc_block_size.argtypes = [i, i, i, i]
         # tile kernels
         L.orc_dpotrf_u.argtypes = [i, dp, C.POINTER(i)]
         L.orc_dtrsm_lutn.argtypes = [i, i, dp, dp]
@@ -144,6 +148,11 @@ class OracleFactor:
             n = L.orc_group(self.h, node, g, C.byref(a), C.byref(b), C.byref(c), C.byref(d), buf)
             out.append((buf[:n].tolist(), a.value, d.value, b.value, c.value))
         return out
+
+    def layout(self, what, node, i=0, j=0):
+        """One layout quantity of node_data / the extractors (orc_layout's
+        table in mfd_oracle.c)."""
+        return int(lib().orc_layout(self.h, int(what), int(node), int(i), int(j)))
 
     def front(self, node):
         """Dense n x n (column-major, lower part) front of `node` after
