@@ -384,8 +384,16 @@ def supernodal_ccs(first, ccs):
     return out
 
 
-def compute_merge_lists(cc, tree_child, lengths, max_zero):
-    """src/main/algorithms/relax_supernode_m.f90:13-90."""
+def compute_merge_lists(cc, tree_child, lengths, max_zero, son_merge_fifo=False):
+    """src/main/algorithms/relax_supernode_m.f90:13-90.
+
+    The son lists of the merged children are joined to node k's list in the
+    order the stack pops them (:52-55).  That order cannot change any output:
+    choose_merge_node (:66-85) only walks the list of the node being
+    processed, and only before the joins of that node; afterwards the lists
+    are read only for their lengths (count_num_child,
+    relaxed_supernode_m.f90:87-99).  son_merge_fifo=True joins them in the
+    other order; the tests use it to show that the analysis is the same."""
     n = len(cc)
     sons = LinkedLists(n)
     merge = LinkedLists(n)
@@ -420,7 +428,7 @@ def compute_merge_lists(cc, tree_child, lengths, max_zero):
                 num_cols[k] += num_cols[merge_node]
             else:
                 while stack:
-                    sons.merge(stack.pop(), k)
+                    sons.merge(stack.pop(0) if son_merge_fifo else stack.pop(), k)
                 break
     return merge, sons
 
@@ -462,9 +470,9 @@ def build_cc(fund_cc, fmap):
     return [fund_cc[m - 1] for m in fmap]
 
 
-def relax_supernode(fund_lengths, fund_cc, fund_tree_child, fund_ccs, max_zero):
+def relax_supernode(fund_lengths, fund_cc, fund_tree_child, fund_ccs, max_zero, son_merge_fifo=False):
     """analyse_phase_m.f90:91-108 with relaxed_supernode_m.f90:15-112."""
-    merge, sons = compute_merge_lists(fund_cc, fund_tree_child, fund_lengths, max_zero)
+    merge, sons = compute_merge_lists(fund_cc, fund_tree_child, fund_lengths, max_zero, son_merge_fifo)
     fmap = build_map(merge)
     rel_lengths = create_node_sets(fund_lengths, fmap, merge)
     perm = create_perm(fund_lengths, fmap, merge)
@@ -501,7 +509,7 @@ def symbolic_factorize(ccs_a, first, cc, tree_child):
     return [sorted(r) for r in rows_l]
 
 
-def analyze_phase(ccs, max_zero):
+def analyze_phase(ccs, max_zero, son_merge_fifo=False):
     """src/main/analyse_phase_m.f90:22-89."""
     n = len(ccs)
     crs = ccs_to_crs(ccs)
@@ -522,7 +530,7 @@ def analyze_phase(ccs, max_zero):
     fparent = parent_in_postordering_tree(fnum_child)
     ftree_child = tree_child_from_num_child(fnum_child, fparent)
     fcc = supernodal_column_count(ffirst, cc_node)
-    rel = relax_supernode(flen, fcc, ftree_child, pccs, max_zero)
+    rel = relax_supernode(flen, fcc, ftree_child, pccs, max_zero, son_merge_fifo)
     rfirst = sets_from_lengths(rel["lengths"])
     sccs = supernodal_ccs(rfirst, rel["ccs"])
     l_structure = symbolic_factorize(sccs, rfirst, rel["cc"], rel["tree_child"])

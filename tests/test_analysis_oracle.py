@@ -170,3 +170,45 @@ def test_block_stencil_generator_is_spd_and_ordered():
     perm = p.perm - 1
     assert sorted(perm) == list(range(p.n))
     assert np.all(perm.reshape(-1, 3) // 3 == (perm.reshape(-1, 3) // 3)[:, :1])
+
+
+def _nd_lower_ccs(p):
+    """Lower CCS (diagonal first) of a generator's matrix in its nested
+    dissection order."""
+    n = p.n
+    perm = p.perm - 1
+    iperm = np.empty(n, int)
+    iperm[perm] = np.arange(n)
+    cols = [set() for _ in range(n)]
+    for j in range(n):
+        for k in range(p.ccs_col[j] - 1, p.ccs_col[j + 1] - 1):
+            a, b = iperm[j], iperm[p.ccs_row[k] - 1]
+            cols[min(a, b)].add(max(a, b) + 1)
+    return [sorted(c | {j + 1}) for j, c in enumerate(cols)]
+
+
+@pytest.mark.parametrize("gen,size,max_zero", [("laplacian_2d", 16, 1024), ("laplacian_2d", 20, 128),
+                                               ("weighted_laplacian_3d", 8, 512)])
+def test_son_list_join_order_is_unobservable(gen, size, max_zero):
+    """compute_merge_lists joins the merged children's son lists in stack
+    order (relax_supernode_m.f90:52-55); the reference's own test leaves the
+    son lists unasserted (relax_supernode_test.f90:2).  The order cannot
+    reach any output, since the lists are afterwards read only for their
+    lengths: joining them in the opposite order gives the same analysis.
+    The cases are ones where the two orders give differently ordered lists."""
+    from mfdag import problems as P
+    ccs = _nd_lower_ccs(getattr(P, gen)(size))
+    a = A.analyze_phase(ccs, max_zero)
+    b = A.analyze_phase(ccs, max_zero, son_merge_fifo=True)
+    assert a.keys() == b.keys()
+    for key in a:
+        assert a[key] == b[key], key
+    fa = a["fundamental_lengths"]
+    m1, s1 = A.compute_merge_lists(a["fundamental_cc"], a["fundamental_tree_child"], fa, max_zero)
+    m2, s2 = A.compute_merge_lists(a["fundamental_cc"], a["fundamental_tree_child"], fa, max_zero, True)
+    differ = 0
+    for i in range(1, len(fa) + 1):
+        assert sorted(s1.items(i)) == sorted(s2.items(i))
+        assert m1.items(i) == m2.items(i)
+        differ += s1.items(i) != s2.items(i)
+    assert differ > 0
