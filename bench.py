@@ -27,6 +27,9 @@ PEAK_HBM_GBS = 8000.0
 MFMA_CLASSES = {"small_front", "potrf", "trsm", "update", "schur"}
 
 
+ETA_BOUND = 1e-14  # normwise backward error every solve must meet
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -372,7 +375,21 @@ def main():
         "graph_nd": nd,
         "batch": batch,
     }
+    # every leg's solution must meet the backward-error bound the metric
+    # names; a leg that misses it makes the whole line invalid
+    etas = {"fp64": eta64, "fp32_ir": eta32}
+    for leg, d in (("graph_nd", nd), ("batch", batch)):
+        for form in ("fp64", "fp32_ir"):
+            if d and form in d:
+                etas[f"{leg}.{form}"] = d[form]["backward_error"]
+    bad = {k: v for k, v in etas.items() if not (v <= ETA_BOUND)}
+    out["valid"] = not bad
+    if bad:
+        out["invalid_reason"] = f"backward error above {ETA_BOUND:g}: {bad}"
     print(json.dumps(out), flush=True)
+    if bad:
+        log(out["invalid_reason"])
+        sys.exit(1)
 
 
 if __name__ == "__main__":

@@ -44,6 +44,15 @@ typedef struct mfd_handle mfd_handle;
 /* precision forms for mfd_factorize */
 #define MFD_FP64 0     /* plain form: FP64 fronts, FP64 solves */
 #define MFD_FP32_IR 1  /* FP32 fronts on MFMA + FP64 iterative refinement */
+/* FP32 fronts of the diagonally scaled matrix S A S (S: powers of two, so
+   that every scaled diagonal lies in [0.5, 2)) whose in-panel and Schur
+   updates multiply FP16- (resp. BF16-) rounded operands on the 16-bit MFMA
+   with FP32 accumulation; diagonal blocks and panel TRSM in FP32; FP64
+   iterative refinement against the unscaled FP64 matrix.  For
+   well-conditioned matrices: a stalled refinement refactors with FP32 fronts,
+   and then in FP64. */
+#define MFD_FP16_IR 2
+#define MFD_BF16_IR 3
 
 /* mfd_create flags */
 #define MFD_FLAG_GRAPH 1    /* replay each schedule from a captured hipGraph */
@@ -112,10 +121,11 @@ int mfd_set_structure(mfd_handle* h, int num_node, const int* supernode_size, co
                       const int* supernodal_index);
 
 /* hppllt_factorize (hppllt.f90:123-150): a holds the values in the caller's
-   CCS order (after mfd_set_structure: in supernodal CCS order).  With
-   MFD_FP32_IR, an FP32 factorization that meets a non-positive pivot is
-   redone in FP64 (mfd_stats.precision then reads MFD_FP64); a positive
-   return means the FP64 factorization failed too. */
+   CCS order (after mfd_set_structure: in supernodal CCS order).  A
+   lower-precision factorization that meets a non-positive pivot is redone
+   in the next form (FP16/BF16 -> FP32 -> FP64; mfd_stats.precision reads the
+   form that holds the factor); a positive return means the FP64
+   factorization failed too. */
 int mfd_factorize(mfd_handle* h, const double* a, int precision);
 /* Same, with the values already in device memory (HBM-resident inputs). */
 int mfd_factorize_device(mfd_handle* h, const double* d_a, int precision);
@@ -131,6 +141,21 @@ int mfd_solve_permuted(mfd_handle* h, double* b, int nrhs, int ldb);
 /* Device-resident b (caller numbering), ldb >= n. */
 int mfd_solve_device(mfd_handle* h, double* d_b, int nrhs, int ldb);
 
+/* Errors of the last solve, per right-hand side (nrhs must be that solve's;
+   any output may be NULL), evaluated at this call in FP64 on the GPU from
+   the solve's b and x and the FP64 values of A (the solve itself is not
+   slowed):
+     eta[k]  = ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf), the
+               normwise backward error;
+     res2[k] = ||b - A x||_2 / ||x||_2, the reference driver's measure
+               (calc_error, example.f90:147-180);
+     ferr[k] = ||d||_inf / ||x||_inf with A d = b - A x solved by the
+               current factor: an estimate of the forward error
+               ||x - A^{-1} b||_inf / ||x||_inf (one more solve sweep).
+   The reference computes only res2, in its driver, for one right-hand side.
+   MFD_ERR_STATE before a solve or after a new factorization. */
+int mfd_get_solve_errors(mfd_handle* h, int nrhs, double* eta, double* res2, double* ferr);
+
 /* Factor in the reference's tile storage (App. layout of factors_m.f90):
    supernode array (hppllt_get_array_size(1) entries) and work array
    (hppllt_get_array_size(2)); either may be NULL.  The border array is
@@ -139,7 +164,8 @@ int mfd_solve_device(mfd_handle* h, double* d_b, int nrhs, int ldb);
 int mfd_export_tiles(mfd_handle* h, double* supernode, double* work);
 /* Dense front of node (1-based): column-major n x n, n = supernode_size +
    work_size; lower part holds L (first supernode_size columns) and the
-   update matrix (trailing block). */
+   update matrix (trailing block).  After an MFD_FP16_IR / MFD_BF16_IR
+   factorization these are the fronts of the scaled matrix S A S. */
 int mfd_export_front(mfd_handle* h, int node, double* out);
 
 /* Getters: hppllt_get_cost / get_nonzero / get_array_size / get_time_info
@@ -161,10 +187,10 @@ int mfd_get_structure(mfd_handle* h, int* supernode_size, int* work_size, int* p
 typedef struct mfd_stats {
     double analyze_ms, assemble_ms, factor_ms, solve_ms;  /* last calls, wall (HIP events) */
     double factor_gflops;        /* cost / factor time (cost_calculator_m.f90:9-20) */
-    int precision;               /* of the current factor (MFD_FP64 after an FP32 factorization met a non-positive pivot) */
-    int refine_iters;            /* last solve (< 0: -1 - iterations before the FP64 fallback) */
-    /* last FP32_IR solve, for its final x (-1 after an FP64 solve, which
-       computes no residual): */
+    int precision;               /* form of the current factor (after any fallback) */
+    int refine_iters;            /* last solve (< 0: the solve changed form; -1 - iterations before that) */
+    /* last refined solve, for its final x (-1 after an FP64 solve, which
+       computes no residual; mfd_get_solve_errors gives them for any form): */
     double backward_error;       /* max over rhs of ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf) */
     double refine_residual;      /* max over rhs of ||b - A x||_2 / ||x||_2 (example.f90:147-180) */
     int64_t front_bytes;         /* device bytes of the front pool */

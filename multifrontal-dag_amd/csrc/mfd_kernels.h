@@ -13,6 +13,15 @@ template <typename T> void launch_fill(T* p, int64_t n, hipStream_t st);
 template <typename T> void launch_fill_lower(T* pool, const NodeDev* nodes, const int2* tasks, int count, hipStream_t st);
 template <typename T>
 void launch_scatter_a(T* pool, const double* a, const int64_t* dst, const int64_t* src, int64_t nnz, hipStream_t st);
+// Diagonal scaling of the 16-bit forms (powers of two: S A S is exact):
+// s[j] from A(j,j) = a[diag_src[j]] so that s_j^2 A(j,j) lies in [0.5, 2);
+// the scaled assembly pool[dst[k]] = s[r] s[c] a[src[k]], (r, c) = rc[k];
+// out = diag(s) in per right-hand side.
+void launch_diag_scale(double* s, const double* a, const int64_t* diag_src, int n, hipStream_t st);
+template <typename T>
+void launch_scatter_a_scaled(T* pool, const double* a, const int64_t* dst, const int64_t* src, const int2* rc,
+                             const double* s, int64_t nnz, hipStream_t st);
+void launch_scale_rows(double* out, const double* in, const double* s, int n, int nrhs, hipStream_t st);
 template <typename T>
 void launch_extend_add(T* pool, const int* lidx, const NodeDev* nodes, const ExtTask* tasks, int count, hipStream_t st);
 template <typename T>
@@ -23,9 +32,11 @@ template <typename T>
 void launch_upd_diag(T* pool, const NodeDev* nodes, const DiagTask* diags, int ndiag, const GemmTask* tiles, int ntiles,
                      int* info, hipStream_t st);
 // kind 0: panel TRSM tiles; 1: in-panel update; 2: Schur update (mode 3
-// tiles read local_index from lidx)
+// tiles read local_index from lidx).  lowp (FP32 fronts, kinds 1 and 2): 1
+// multiplies FP16-rounded operands, 2 BF16-rounded ones, on the 16-bit MFMA
+// with FP32 accumulation; 0 the front's own precision.
 template <typename T>
-void launch_gemm(T* pool, const GemmTask* tasks, int count, int kind, const int* lidx, hipStream_t st);
+void launch_gemm(T* pool, const GemmTask* tasks, int count, int kind, const int* lidx, hipStream_t st, int lowp = 0);
 template <typename T> void launch_gemm128(T* pool, const GemmTask* tasks, int count, hipStream_t st);
 void launch_fwd_init(double* zpool, const double* b, int ldb, int nrhs, const NodeDev* nodes, const int2* tasks,
                      int count, hipStream_t st);

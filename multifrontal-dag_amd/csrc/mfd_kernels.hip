@@ -48,6 +48,31 @@ template <> struct Mfma<float> {
     static __device__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
+// 16-bit operands with FP32 accumulation (the FP16_IR / BF16_IR forms):
+// D(16x16) += A(16x16) B(16x16); lane l supplies A[l%16][4(l/16) + i] and
+// B[4(l/16) + i][l%16], i < 4; the accumulator layout is the f32 one above.
+// Operands are rounded to nearest even from FP32.
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef short s4 __attribute__((ext_vector_type(4)));
+enum LowPrec { kNative = 0, kF16 = 1, kBF16 = 2 };
+__device__ __forceinline__ short bf16_bits(float x)
+{
+    unsigned u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+    return (short)(u >> 16);
+}
+template <int LP> struct Mfma16;
+template <> struct Mfma16<kF16> {
+    typedef h4 op_t;
+    static __device__ _Float16 cvt(float x) { return (_Float16)x; }
+    static __device__ f4 mma(op_t a, op_t b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0); }
+};
+template <> struct Mfma16<kBF16> {
+    typedef s4 op_t;
+    static __device__ short cvt(float x) { return bf16_bits(x); }
+    static __device__ f4 mma(op_t a, op_t b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+};
+
 __device__ inline double dsqrt(double x) { return sqrt(x); }
 __device__ inline float dsqrt(float x) { return sqrtf(x); }
 
@@ -352,6 +377,47 @@ __global__ __launch_bounds__(kThreads) void k_scatter_a(T* pool, const double* a
     for (; k < nnz; k += stride) pool[dst[k]] = T(a[src[k]]);
 }
 
+// Diagonal scaling of the 16-bit forms: s[j] = 2^-e with e = floor(E / 2)
+// for A(j,j) = m 2^E (0.5 <= m < 1), so that s_j^2 A(j,j) lies in [0.5, 2);
+// a diagonal that is not positive and finite keeps s[j] = 1.  Powers of two
+// make the scaled matrix S A S exact.  diag_src[j]: caller index of A(j,j)
+// (j postordered).
+__global__ __launch_bounds__(kThreads) void k_diag_scale(double* s, const double* a, const int64_t* diag_src, int n)
+{
+    const int j = blockIdx.x * kThreads + threadIdx.x;
+    if (j >= n) return;
+    const double d = a[diag_src[j]];
+    double v = 1.0;
+    if (d > 0.0 && d <= 1.7976931348623157e308) {
+        int E;
+        (void)frexp(d, &E);
+        const int e = E >= 0 ? E / 2 : -((1 - E) / 2);  // floor(E / 2)
+        v = ldexp(1.0, -e);
+    }
+    s[j] = v;
+}
+
+// k_scatter_a for the scaled forms: pool[dst[k]] = s[r] s[c] a[src[k]] with
+// (r, c) = rc[k], the entry's global (postordered, 0-based) row and column.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_scatter_a_scaled(T* pool, const double* a, const int64_t* dst,
+                                                               const int64_t* src, const int2* rc, const double* s,
+                                                               int64_t nnz)
+{
+    const int64_t k = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (k >= nnz) return;
+    const int2 q = rc[k];
+    pool[dst[k]] = (T)(a[src[k]] * s[q.x] * s[q.y]);
+}
+
+// out[i + k n] = s[i] in[i + k n] (out may be in).
+__global__ __launch_bounds__(kThreads) void k_scale_rows(double* out, const double* in, const double* s, int n, int nrhs)
+{
+    const int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (e >= (int64_t)n * nrhs) return;
+    out[e] = s[e % n] * in[e];
+}
+
 // ------------------------------------------------------------- extend-add
 // One 64 x 64 tile (ti >= tj) of the child's update matrix, lower part only,
 // added into the parent front at the rows/columns local_index gives (used
@@ -415,10 +481,14 @@ template <typename T> struct TileLds {
 //
 // tb != 0 takes B transposed in memory (B[col][kk] at B + col * ldb + kk);
 // its chunks are then read 16 entries at a time along kk.
-template <typename T>
+//
+// LP = kF16 / kBF16 (FP32 operands only) rounds each operand to 16 bits as
+// it leaves LDS and multiplies on the 16-bit MFMA with FP32 accumulation.
+template <typename T, int LP = kNative>
 __device__ __forceinline__ void tile_mma(const T* A, int lda, const T* B, int ldb, bool tb, int m, int n, int k,
                                          TileLds<T>& L, typename Mfma<T>::acc_t (&c)[2][2])
 {
+    static_assert(LP == kNative || sizeof(T) == 4, "16-bit operands come from FP32 fronts");
     typedef typename Mfma<T>::acc_t acc_t;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wr = (wv & 1) * 32, wc = (wv >> 1) * 32;
@@ -452,15 +522,32 @@ __device__ __forceinline__ void tile_mma(const T* A, int lda, const T* B, int ld
         }
         __syncthreads();
         if (k0 + 16 < k) load(k0 + 16);
+        if constexpr (LP == kNative) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int kr = q * 4 + (lane >> 4);
-            const T a0 = L.As[kr][wr + (lane & 15)], a1 = L.As[kr][wr + 16 + (lane & 15)];
-            const T b0 = L.Bs[kr][wc + (lane & 15)], b1 = L.Bs[kr][wc + 16 + (lane & 15)];
-            c00 = Mfma<T>::mma(a0, b0, c00);
-            c01 = Mfma<T>::mma(a0, b1, c01);
-            c10 = Mfma<T>::mma(a1, b0, c10);
-            c11 = Mfma<T>::mma(a1, b1, c11);
+            for (int q = 0; q < 4; ++q) {
+                const int kr = q * 4 + (lane >> 4);
+                const T a0 = L.As[kr][wr + (lane & 15)], a1 = L.As[kr][wr + 16 + (lane & 15)];
+                const T b0 = L.Bs[kr][wc + (lane & 15)], b1 = L.Bs[kr][wc + 16 + (lane & 15)];
+                c00 = Mfma<T>::mma(a0, b0, c00);
+                c01 = Mfma<T>::mma(a0, b1, c01);
+                c10 = Mfma<T>::mma(a1, b0, c10);
+                c11 = Mfma<T>::mma(a1, b1, c11);
+            }
+        } else {
+            typedef Mfma16<LP> M16;
+            typename M16::op_t a0, a1, b0, b1;
+            const int kr = 4 * (lane >> 4);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                a0[i] = M16::cvt(L.As[kr + i][wr + (lane & 15)]);
+                a1[i] = M16::cvt(L.As[kr + i][wr + 16 + (lane & 15)]);
+                b0[i] = M16::cvt(L.Bs[kr + i][wc + (lane & 15)]);
+                b1[i] = M16::cvt(L.Bs[kr + i][wc + 16 + (lane & 15)]);
+            }
+            c00 = M16::mma(a0, b0, c00);
+            c01 = M16::mma(a0, b1, c01);
+            c10 = M16::mma(a1, b0, c10);
+            c11 = M16::mma(a1, b1, c11);
         }
         __syncthreads();
     }
@@ -470,12 +557,12 @@ __device__ __forceinline__ void tile_mma(const T* A, int lda, const T* B, int ld
     c[1][1] = c11;
 }
 // The same product staged as Cs[col][row]; ends with a barrier.
-template <typename T>
+template <typename T, int LP = kNative>
 __device__ __forceinline__ void tile_product(const T* A, int lda, const T* B, int ldb, bool tb, int m, int n, int k,
                                              TileLds<T>& L, T (*Cs)[kTile + 1])
 {
     typename Mfma<T>::acc_t c[2][2];
-    tile_mma<T>(A, lda, B, ldb, tb, m, n, k, L, c);
+    tile_mma<T, LP>(A, lda, B, ldb, tb, m, n, k, L, c);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int wr = (wv & 1) * 32, wc = (wv >> 1) * 32, q0 = wc + (lane & 15);
 #pragma unroll
@@ -583,11 +670,11 @@ __device__ __forceinline__ void stage_acc(const typename Mfma<T>::acc_t (&c)[2][
 // (TRSM, in-panel update, Schur update), so that profiles tell them apart.
 // The result goes through LDS so the write of C is coalesced along columns
 // (written straight from the accumulators it would be 32-byte pieces).
-template <typename T>
+template <typename T, int LP = kNative>
 __device__ __forceinline__ void gemm_task(T* pool, const GemmTask& t, TileLds<T>& L, T (*Cs)[kTile + 1],
                                           const int* lidx = nullptr)
 {
-    tile_product<T>(pool + t.a, t.lda, pool + t.b, t.ldb, t.mode == 1, t.m, t.n, t.k, L, Cs);
+    tile_product<T, LP>(pool + t.a, t.lda, pool + t.b, t.ldb, t.mode == 1, t.m, t.n, t.k, L, Cs);
     T* C = pool + t.c;
     const int tid = threadIdx.x, row = tid & 63;
     if (t.mode == 2) {
@@ -639,9 +726,12 @@ __device__ __forceinline__ void gemm_task(T* pool, const GemmTask& t, TileLds<T>
     }
 }
 
-template <typename T, int KIND>
+// LP != kNative: the update and Schur tiles (KIND 1, 2) of the FP16_IR /
+// BF16_IR forms multiply 16-bit operands.
+template <typename T, int KIND, int LP = kNative>
 __global__ __launch_bounds__(kThreads) void k_gemm(T* pool, const GemmTask* tasks, const int* lidx)
 {
+    static_assert(LP == kNative || KIND != 0, "the panel TRSM stays in the front's precision");
     if constexpr (KIND == 0) {
         // panel TRSM tiles (modes 1 and 2): mode 2 keeps its first product
         // in Cs while staging the inverse through L, so the two are separate
@@ -656,7 +746,7 @@ __global__ __launch_bounds__(kThreads) void k_gemm(T* pool, const GemmTask* task
             TileLds<T> L;
             T Cs[kTile][kTile + 1];
         } u;
-        gemm_task<T>(pool, tasks[blockIdx.x], u.L, u.Cs, lidx);
+        gemm_task<T, LP>(pool, tasks[blockIdx.x], u.L, u.Cs, lidx);
     }
 }
 
@@ -1586,15 +1676,50 @@ void launch_upd_diag(T* pool, const NodeDev* nodes, const DiagTask* diags, int n
                            info);
 }
 template <typename T>
-void launch_gemm(T* pool, const GemmTask* tasks, int count, int kind, const int* lidx, hipStream_t st)
+void launch_gemm(T* pool, const GemmTask* tasks, int count, int kind, const int* lidx, hipStream_t st, int lowp)
 {
     if (count <= 0) return;
+    if constexpr (sizeof(T) == 4) {
+        if (lowp == kF16 && kind != 0) {
+            if (kind == 1)
+                hipLaunchKernelGGL((k_gemm<T, 1, kF16>), dim3(count), dim3(kThreads), 0, st, pool, tasks, lidx);
+            else
+                hipLaunchKernelGGL((k_gemm<T, 2, kF16>), dim3(count), dim3(kThreads), 0, st, pool, tasks, lidx);
+            return;
+        }
+        if (lowp == kBF16 && kind != 0) {
+            if (kind == 1)
+                hipLaunchKernelGGL((k_gemm<T, 1, kBF16>), dim3(count), dim3(kThreads), 0, st, pool, tasks, lidx);
+            else
+                hipLaunchKernelGGL((k_gemm<T, 2, kBF16>), dim3(count), dim3(kThreads), 0, st, pool, tasks, lidx);
+            return;
+        }
+    }
     if (kind == 0)
         hipLaunchKernelGGL((k_gemm<T, 0>), dim3(count), dim3(kThreads), 0, st, pool, tasks, lidx);
     else if (kind == 1)
         hipLaunchKernelGGL((k_gemm<T, 1>), dim3(count), dim3(kThreads), 0, st, pool, tasks, lidx);
     else
         hipLaunchKernelGGL((k_gemm<T, 2>), dim3(count), dim3(kThreads), 0, st, pool, tasks, lidx);
+}
+void launch_diag_scale(double* s, const double* a, const int64_t* diag_src, int n, hipStream_t st)
+{
+    if (n > 0) hipLaunchKernelGGL(k_diag_scale, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, st, s, a, diag_src, n);
+}
+template <typename T>
+void launch_scatter_a_scaled(T* pool, const double* a, const int64_t* dst, const int64_t* src, const int2* rc,
+                             const double* s, int64_t nnz, hipStream_t st)
+{
+    if (nnz > 0)
+        hipLaunchKernelGGL(k_scatter_a_scaled<T>, dim3((unsigned)((nnz + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                           st, pool, a, dst, src, rc, s, nnz);
+}
+void launch_scale_rows(double* out, const double* in, const double* s, int n, int nrhs, hipStream_t st)
+{
+    const int64_t m = (int64_t)n * nrhs;
+    if (m > 0)
+        hipLaunchKernelGGL(k_scale_rows, dim3((unsigned)((m + kThreads - 1) / kThreads)), dim3(kThreads), 0, st, out, in,
+                           s, n, nrhs);
 }
 template <typename T> void launch_gemm128(T* pool, const GemmTask* tasks, int count, hipStream_t st)
 {
@@ -1654,8 +1779,10 @@ void launch_scatter_rows(double* out, int ld_out, const double* in, int ld_in, c
     template void launch_small_front<T>(T*, const NodeDev*, const int*, int, int*, hipStream_t);                       \
     template void launch_diag<T>(T*, const NodeDev*, const DiagTask*, int, int*, hipStream_t);                        \
     template void launch_upd_diag<T>(T*, const NodeDev*, const DiagTask*, int, const GemmTask*, int, int*, hipStream_t); \
-    template void launch_gemm<T>(T*, const GemmTask*, int, int, const int*, hipStream_t);                                          \
+    template void launch_gemm<T>(T*, const GemmTask*, int, int, const int*, hipStream_t, int);                         \
     template void launch_gemm128<T>(T*, const GemmTask*, int, hipStream_t);                                            \
+    template void launch_scatter_a_scaled<T>(T*, const double*, const int64_t*, const int64_t*, const int2*,           \
+                                             const double*, int64_t, hipStream_t);                                    \
     template void launch_fwd_step<T>(const T*, double*, int, const NodeDev*, const SolveTask*, int, hipStream_t);       \
     template void launch_bwd_step<T>(const T*, double*, double*, double*, int, int, const NodeDev*, const SolveTask*,  \
                                      int, hipStream_t);

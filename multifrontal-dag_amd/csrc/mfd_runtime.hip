@@ -120,6 +120,10 @@ struct mfd_handle {
 
     bool analyzed = false, factored = false, from_parts = false;
     int precision = MFD_FP64;
+    // 16-bit operand mode of the current FP32-pool factorization: 0 (FP32
+    // fronts), 1 (MFD_FP16_IR), 2 (MFD_BF16_IR); the 16-bit forms factor
+    // the diagonally scaled matrix S A S (scale vector d_dscale).
+    int lowp = 0;
     Structure S;
     RefLayout* layout = nullptr;
 
@@ -153,6 +157,9 @@ struct mfd_handle {
     GemmTask* d_gemm = nullptr;
     int *d_level_nodes = nullptr, *d_child_ptr = nullptr, *d_child_list = nullptr;
     int64_t *d_dst = nullptr, *d_src = nullptr;
+    int2* d_scat_rc = nullptr;       // global (row, col) of each scattered entry, in d_dst's order
+    int64_t* d_diag_src = nullptr;   // caller index of A(j,j), j postordered
+    double* d_dscale = nullptr;      // diagonal scaling of the 16-bit forms
     int* d_perm = nullptr;  // new -> old, 0-based
     double* d_pool64 = nullptr;
     float* d_pool32 = nullptr;
@@ -168,7 +175,7 @@ struct mfd_handle {
     double *d_r = nullptr, *d_d = nullptr;
     unsigned long long* d_norms = nullptr;
 
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};  // FP64, FP32, FP16, BF16 factorizations
     struct SolveGraph {
         int prec, nrhs;
         const double* bp;
@@ -178,6 +185,10 @@ struct mfd_handle {
     std::vector<SolveGraph> sgraphs;
     hipEvent_t ev[4] = {};
     std::vector<hipEvent_t> prof_ev;
+
+    // errors of the last solve, per right-hand side (backward_error)
+    int solved_nrhs = 0;  // nrhs of the last solve; 0: no solve since the last factorization
+    std::vector<double> err_eta, err_res2, err_ferr;
 
     double times[8] = {0};
     mfd_stats st{};
@@ -236,6 +247,9 @@ void free_device(mfd_handle* h)
     h->d_fi = h->d_fa = h->d_bg = nullptr;
     h->d_part = nullptr;
     h->d_dst = h->d_src = nullptr;
+    h->d_scat_rc = nullptr;
+    h->d_diag_src = nullptr;
+    h->d_dscale = nullptr;
     h->d_perm = nullptr;
     h->d_pool64 = nullptr;
     h->d_pool32 = nullptr;
@@ -738,7 +752,8 @@ int setup_device(mfd_handle* h)
     // assembly map: A entry -> front position (coefficient_setter_m.f90:26-41),
     // the entries in the fill groups' order
     const int64_t nnz = (int64_t)S.snidx.size();
-    std::vector<int64_t> dst(nnz), src(nnz);
+    std::vector<int64_t> dst(nnz), src(nnz), diag_src(S.n, 0);
+    std::vector<int2> rc(nnz);
     h->scat_gptr.assign(1, 0);
     int64_t e = 0;
     for (size_t g = 0; g < h->group_level.size(); ++g) {
@@ -750,6 +765,11 @@ int setup_device(mfd_handle* h)
                 for (int64_t p = S.a_colptr[j - 1]; p < S.a_colptr[j]; ++p, ++e) {
                     dst[e] = d.front + (S.snidx[p] - 1) + q * d.n;
                     src[e] = S.ccs_perm[p] - 1;
+                    // global row: a pivot row of this front or one of its update rows
+                    const int r = S.snidx[p];
+                    const int grow = r <= S.c[s] ? S.first[s] + r - 1 : S.l_rows[S.l_ptr[s] + (r - S.c[s] - 1)];
+                    rc[e] = make_int2(grow - 1, j - 1);
+                    if (grow == j) diag_src[j - 1] = src[e];
                 }
             }
         }
@@ -758,6 +778,9 @@ int setup_device(mfd_handle* h)
     if (e != nnz) return MFD_ERR_INTERNAL;
     MFD_TRY(upload(h, &h->d_dst, dst));
     MFD_TRY(upload(h, &h->d_src, src));
+    MFD_TRY(upload(h, &h->d_scat_rc, rc));
+    MFD_TRY(upload(h, &h->d_diag_src, diag_src));
+    MFD_TRY(dalloc(h, &h->d_dscale, S.n));
     std::vector<int> perm0(S.n);
     for (int i = 0; i < S.n; ++i) perm0[i] = S.perm[i] - 1;
     MFD_TRY(upload(h, &h->d_perm, perm0));
@@ -785,7 +808,7 @@ int ensure_rhs(mfd_handle* h, int nrhs)
     MFD_TRY(dalloc(h, &h->d_xp, n * nrhs));
     MFD_TRY(dalloc(h, &h->d_r, n * nrhs));
     MFD_TRY(dalloc(h, &h->d_d, n * nrhs));
-    MFD_TRY(dalloc(h, &h->d_norms, 3 * nrhs + 1 + 2 * kSumParts * nrhs));
+    MFD_TRY(dalloc(h, &h->d_norms, 4 * nrhs + 1 + 2 * kSumParts * nrhs));
     MFD_TRY(dalloc(h, &h->d_part, h->part_slots * 64 * nrhs));
     drop_solve_graphs(h);  // captured solves refer to the old buffers
     h->nrhs_cap = nrhs;
@@ -817,7 +840,8 @@ template <typename T> void enqueue_launch(mfd_handle* h, const Launch& l, hipStr
             launch_gemm128<T>(pool, h->d_gemm + l.first, l.count, st);
         else
             launch_gemm<T>(pool, h->d_gemm + l.first, l.count,
-                           l.kind == MFD_K_TRSM ? 0 : l.kind == MFD_K_UPDATE ? 1 : 2, h->d_lidx, st);
+                           l.kind == MFD_K_TRSM ? 0 : l.kind == MFD_K_UPDATE ? 1 : 2, h->d_lidx, st,
+                           sizeof(T) == 4 ? h->lowp : 0);
         break;
     default: break;
     }
@@ -850,10 +874,16 @@ template <typename T> void enqueue_factor(mfd_handle* h, bool profile)
         (void)hipEventRecord(h->fork, st);
         (void)hipStreamWaitEvent(fst, h->fork, 0);
     }
+    const bool scaled = sizeof(T) == 4 && h->lowp != 0;
+    if (scaled) launch_diag_scale(h->d_dscale, h->d_a, h->d_diag_src, h->S.n, fst);
     for (int g = 0; g < G; ++g) {
         launch_fill_lower<T>(pool, h->d_nodes, h->d_fill + h->fill_gptr[g], h->fill_gptr[g + 1] - h->fill_gptr[g], fst);
-        launch_scatter_a<T>(pool, h->d_a, h->d_dst + h->scat_gptr[g], h->d_src + h->scat_gptr[g],
-                            h->scat_gptr[g + 1] - h->scat_gptr[g], fst);
+        const int64_t e0 = h->scat_gptr[g], ne = h->scat_gptr[g + 1] - h->scat_gptr[g];
+        if (scaled)
+            launch_scatter_a_scaled<T>(pool, h->d_a, h->d_dst + e0, h->d_src + e0, h->d_scat_rc + e0, h->d_dscale, ne,
+                                       fst);
+        else
+            launch_scatter_a<T>(pool, h->d_a, h->d_dst + e0, h->d_src + e0, ne, fst);
         if (split_fill) (void)hipEventRecord(h->fill_ev[g], fst);
     }
     mark(0);
@@ -909,7 +939,7 @@ template <typename T> int run_factor(mfd_handle* h)
 {
     const bool profile = (h->flags & MFD_FLAG_PROFILE) != 0;
     const bool graph = (h->flags & MFD_FLAG_GRAPH) != 0 && !profile;
-    const int gi = sizeof(T) == 8 ? 0 : 1;
+    const int gi = sizeof(T) == 8 ? 0 : 1 + h->lowp;
     HIP_TRY(hipEventRecord(h->ev[0], h->stream));
     if (graph) {
         if (!h->graph[gi]) {
@@ -946,22 +976,27 @@ int factorize_common(mfd_handle* h, int precision)
 {
     if (!h->analyzed) return MFD_ERR_STATE;
     if (!h->device_ok) return MFD_ERR_NODEVICE;
-    if (precision != MFD_FP64 && precision != MFD_FP32_IR) return MFD_ERR_UNSUPPORTED;
+    if (precision < MFD_FP64 || precision > MFD_BF16_IR) return MFD_ERR_UNSUPPORTED;
     h->factored = false;
-    int rc;
+    h->solved_nrhs = 0;
+    int rc = MFD_OK;
+    if (precision == MFD_FP16_IR || precision == MFD_BF16_IR) {
+        if (!h->d_pool32) MFD_TRY(dalloc(h, &h->d_pool32, h->pool_elems));
+        h->lowp = precision == MFD_FP16_IR ? 1 : 2;
+        rc = run_factor<float>(h);
+        h->lowp = 0;
+        // a pivot not positive with 16-bit updates: FP32 fronts next
+        if (rc > 0) precision = MFD_FP32_IR;
+    }
     if (precision == MFD_FP32_IR) {
         if (!h->d_pool32) MFD_TRY(dalloc(h, &h->d_pool32, h->pool_elems));
         rc = run_factor<float>(h);
-        if (rc > 0) {
-            // a pivot not positive in FP32: the FP64 refactorization the
-            // refinement falls back to when it stalls; only an FP64 failure
-            // says that the matrix is not positive definite
-            precision = MFD_FP64;
-            rc = run_factor<double>(h);
-        }
-    } else {
-        rc = run_factor<double>(h);
+        // a pivot not positive in FP32: the FP64 refactorization the
+        // refinement falls back to when it stalls; only an FP64 failure
+        // says that the matrix is not positive definite
+        if (rc > 0) precision = MFD_FP64;
     }
+    if (precision == MFD_FP64) rc = run_factor<double>(h);
     if (rc < 0) return rc;
     h->precision = precision;
     h->factored = rc == MFD_OK;
@@ -1041,46 +1076,57 @@ template <typename T> int run_solve(mfd_handle* h, const double* bp, double* xp,
     return MFD_OK;
 }
 
-// Normwise backward error per right-hand side, max over them:
-// eta = ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf), and res2 = the
-// max of ||b - A x||_2 / ||x||_2 (the measure of example.f90:147-180; both
-// norms are unchanged by the analysis permutation).  The sums of squares
-// come back as per-block partials added here in a fixed order.
-int backward_error(mfd_handle* h, const double* bp, const double* xp, double* r, int nrhs, double* eta, double* res2)
+// Normwise backward error per right-hand side:
+// eta = ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf), and res2 =
+// ||b - A x||_2 / ||x||_2 (the measure of example.f90:147-180; both norms are
+// unchanged by the analysis permutation), each per right-hand side into
+// h->err_eta / h->err_res2, and their maxima over the right-hand sides into
+// *eta / *res2.  With d != nullptr, also ||d||_inf / ||x||_inf per
+// right-hand side into h->err_ferr.  r receives b - A x.  The sums of
+// squares come back as per-block partials added here in a fixed order.
+int backward_error(mfd_handle* h, const double* bp, const double* xp, double* r, int nrhs, double* eta, double* res2,
+                   const double* d = nullptr)
 {
     const int n = h->S.n;
     hipStream_t st = h->stream;
-    const int nm = 3 * nrhs + 1, parts = sumsq_parts(n);
+    const int nm = 4 * nrhs + 1, parts = sumsq_parts(n);
     double* sq = reinterpret_cast<double*>(h->d_norms + nm);
     launch_residual(r, bp, xp, n, nrhs, h->d_csr_ptr, h->d_csr_col, h->d_csr_src, h->d_a, st);
     launch_zero_u64(h->d_norms, nm, st);
     launch_absmax(h->d_norms, r, n, n, nrhs, st);
     launch_absmax(h->d_norms + nrhs, xp, n, n, nrhs, st);
     launch_absmax(h->d_norms + 2 * nrhs, bp, n, n, nrhs, st);
-    launch_norm_inf(h->d_norms + 3 * nrhs, n, h->d_csr_ptr, h->d_csr_src, h->d_a, st);
+    if (d) launch_absmax(h->d_norms + 3 * nrhs, d, n, n, nrhs, st);
+    launch_norm_inf(h->d_norms + 4 * nrhs, n, h->d_csr_ptr, h->d_csr_src, h->d_a, st);
     launch_sumsq(sq, r, n, n, nrhs, st);
     launch_sumsq(sq + (int64_t)parts * nrhs, xp, n, n, nrhs, st);
     std::vector<unsigned long long> v(nm + 2 * parts * nrhs);
     HIP_TRY(hipMemcpyAsync(v.data(), h->d_norms, v.size() * 8, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
-    auto d = [&](int i) {
+    auto val = [&](int i) {
         double x;
         memcpy(&x, &v[i], 8);
         return x;
     };
     auto sum = [&](int first) {
         double s = 0;
-        for (int p = 0; p < parts; ++p) s += d(first + p);
+        for (int p = 0; p < parts; ++p) s += val(first + p);
         return s;
     };
-    double an = d(3 * nrhs), worst = 0, worst2 = 0;
+    auto ratio = [](double num, double den) { return den > 0 ? num / den : (num > 0 ? INFINITY : 0.0); };
+    h->err_eta.assign(nrhs, 0.0);
+    h->err_res2.assign(nrhs, 0.0);
+    if (d) h->err_ferr.assign(nrhs, 0.0);
+    double an = val(4 * nrhs), worst = 0, worst2 = 0;
     for (int k = 0; k < nrhs; ++k) {
-        double den = an * d(nrhs + k) + d(2 * nrhs + k);
-        double e = den > 0 ? d(k) / den : (d(k) > 0 ? INFINITY : 0.0);
+        const double e = ratio(val(k), an * val(nrhs + k) + val(2 * nrhs + k));
         if (!(e <= worst)) worst = e;  // NaN propagates
         const double rr = sum(nm + k * parts), xx = sum(nm + (nrhs + k) * parts);
-        double e2 = xx > 0 ? std::sqrt(rr) / std::sqrt(xx) : (rr > 0 ? INFINITY : 0.0);
+        const double e2 = ratio(std::sqrt(rr), std::sqrt(xx));
         if (!(e2 <= worst2)) worst2 = e2;
+        h->err_eta[k] = e;
+        h->err_res2[k] = e2;
+        if (d) h->err_ferr[k] = ratio(val(3 * nrhs + k), val(nrhs + k));
     }
     *eta = worst;
     *res2 = worst2;
@@ -1090,14 +1136,55 @@ int backward_error(mfd_handle* h, const double* bp, const double* xp, double* r,
 constexpr int kMaxRefine = 10;
 constexpr double kRefineTarget = 1e-14;  // the backward error the FP64 form is held to
 
-// x for b, both postordered (ld n) in device memory.  FP32 factors are
-// refined in FP64 (residual with the FP64 matrix, correction with the FP32
-// factor) until the backward error reaches the target or stops improving;
-// if it then still exceeds the target, the matrix is refactored in FP64.
-int solve_permuted_device(mfd_handle* h, int nrhs)
+bool scaled_form(int precision) { return precision == MFD_FP16_IR || precision == MFD_BF16_IR; }
+
+// d = F^{-1} r for the current lower-precision factor F: the 16-bit forms
+// factor S A S, so d = S (S A S)^{-1} S r, with r scaled in place.
+int correction_solve(mfd_handle* h, double* r, double* d, int nrhs)
+{
+    const int n = h->S.n;
+    if (scaled_form(h->precision)) launch_scale_rows(r, r, h->d_dscale, n, nrhs, h->stream);
+    MFD_TRY(run_solve<float>(h, r, d, nrhs));
+    if (scaled_form(h->precision)) launch_scale_rows(d, d, h->d_dscale, n, nrhs, h->stream);
+    return MFD_OK;
+}
+
+// FP64 iterative refinement of x (h->d_xp) for b (h->d_bp) with the current
+// FP32-pool factor: the first solve, then r = b - A x with the FP64 matrix,
+// the correction from the factor and x += d, until the backward error
+// reaches the target or stops halving.  Returns the iterations taken.
+int refine(mfd_handle* h, int nrhs, double* eta, double* res2, int* iters)
 {
     const int64_t n = h->S.n;
     hipStream_t st = h->stream;
+    if (scaled_form(h->precision)) {
+        // x = S (S A S)^{-1} S b, with S b formed in the residual buffer
+        launch_scale_rows(h->d_r, h->d_bp, h->d_dscale, (int)n, nrhs, st);
+        MFD_TRY(run_solve<float>(h, h->d_r, h->d_xp, nrhs));
+        launch_scale_rows(h->d_xp, h->d_xp, h->d_dscale, (int)n, nrhs, st);
+    } else {
+        MFD_TRY(run_solve<float>(h, h->d_bp, h->d_xp, nrhs));
+    }
+    double prev = INFINITY;
+    int it = 0;
+    for (;;) {
+        MFD_TRY(backward_error(h, h->d_bp, h->d_xp, h->d_r, nrhs, eta, res2));
+        if (!(*eta > kRefineTarget) || it >= kMaxRefine || !(*eta < 0.5 * prev)) break;
+        prev = *eta;
+        MFD_TRY(correction_solve(h, h->d_r, h->d_d, nrhs));
+        for (int k = 0; k < nrhs; ++k) launch_axpy(h->d_xp + k * n, h->d_d + k * n, n, st);
+        ++it;
+    }
+    *iters = it;
+    return MFD_OK;
+}
+
+// x for b, both postordered (ld n) in device memory.  The lower-precision
+// forms are refined in FP64 (refine); when refinement stalls above the
+// target the matrix is refactored in the next form -- FP16 / BF16 updates,
+// then FP32 fronts, then FP64 -- and solved again.
+int solve_permuted_device(mfd_handle* h, int nrhs)
+{
     h->st.refine_iters = 0;
     h->st.backward_error = -1;
     h->st.refine_residual = -1;
@@ -1105,30 +1192,28 @@ int solve_permuted_device(mfd_handle* h, int nrhs)
         MFD_TRY(run_solve<double>(h, h->d_bp, h->d_xp, nrhs));
         return MFD_OK;
     }
-    MFD_TRY(run_solve<float>(h, h->d_bp, h->d_xp, nrhs));
-    double eta = 0, res2 = 0, prev = INFINITY;
-    int it = 0;
+    const int first = h->precision;
+    double eta = 0, res2 = 0;
+    int total = 0;
     for (;;) {
-        MFD_TRY(backward_error(h, h->d_bp, h->d_xp, h->d_r, nrhs, &eta, &res2));
-        if (!(eta > kRefineTarget) || it >= kMaxRefine || !(eta < 0.5 * prev)) break;
-        prev = eta;
-        MFD_TRY(run_solve<float>(h, h->d_r, h->d_d, nrhs));
-        for (int k = 0; k < nrhs; ++k) launch_axpy(h->d_xp + k * n, h->d_d + k * n, n, st);
-        ++it;
+        int it = 0;
+        MFD_TRY(refine(h, nrhs, &eta, &res2, &it));
+        total += it;
+        if (eta <= kRefineTarget) break;
+        // refinement stalled: refactor in the next form
+        const int next = h->precision == MFD_FP32_IR ? MFD_FP64 : MFD_FP32_IR;
+        int rc = factorize_common(h, next);
+        if (rc != MFD_OK) return rc;
+        if (h->precision == MFD_FP64) {
+            MFD_TRY(run_solve<double>(h, h->d_bp, h->d_xp, nrhs));
+            MFD_TRY(backward_error(h, h->d_bp, h->d_xp, h->d_r, nrhs, &eta, &res2));
+            break;
+        }
     }
-    h->st.refine_iters = it;
     h->st.backward_error = eta;
     h->st.refine_residual = res2;
-    if (!(eta <= kRefineTarget)) {
-        // refinement stalled: FP64 refactorization fallback
-        int rc = factorize_common(h, MFD_FP64);
-        if (rc != MFD_OK) return rc;
-        MFD_TRY(run_solve<double>(h, h->d_bp, h->d_xp, nrhs));
-        MFD_TRY(backward_error(h, h->d_bp, h->d_xp, h->d_r, nrhs, &eta, &res2));
-        h->st.backward_error = eta;
-        h->st.refine_residual = res2;
-        h->st.refine_iters = -1 - it;  // negative: fell back to FP64
-    }
+    // negative: the form changed (-1 - iterations taken before the last change)
+    h->st.refine_iters = h->precision == first ? total : -1 - total;
     return MFD_OK;
 }
 
@@ -1163,6 +1248,7 @@ int solve_host(mfd_handle* h, double* b, int nrhs, int ldb, bool permuted, bool 
     HIP_TRY(hipEventElapsedTime(&ms, h->ev[2], h->ev[3]));
     h->st.solve_ms = ms;
     h->times[MFD_TIME_SOLVE] = (now_ms() - t0) / 1e3;
+    h->solved_nrhs = nrhs;
     return MFD_OK;
 }
 
@@ -1363,6 +1449,31 @@ int mfd_solve_device(mfd_handle* h, double* d_b, int nrhs, int ldb)
 {
     if (!h) return MFD_ERR_ARG;
     return solve_host(h, d_b, nrhs, ldb, false, true);
+}
+
+int mfd_get_solve_errors(mfd_handle* h, int nrhs, double* eta, double* res2, double* ferr)
+{
+    if (!h) return MFD_ERR_ARG;
+    if (!h->device_ok) return MFD_ERR_NODEVICE;
+    if (!h->factored || h->solved_nrhs == 0) return MFD_ERR_STATE;
+    if (nrhs != h->solved_nrhs) return MFD_ERR_ARG;
+    double e = 0, e2 = 0;
+    MFD_TRY(backward_error(h, h->d_bp, h->d_xp, h->d_r, nrhs, &e, &e2));
+    if (ferr) {
+        // d = A^{-1} (b - A x) by the current factor, then ||d|| / ||x||
+        if (h->precision == MFD_FP64)
+            MFD_TRY(run_solve<double>(h, h->d_r, h->d_d, nrhs));
+        else
+            MFD_TRY(correction_solve(h, h->d_r, h->d_d, nrhs));
+        MFD_TRY(backward_error(h, h->d_bp, h->d_xp, h->d_r, nrhs, &e, &e2, h->d_d));
+    }
+    HIP_TRY(hipGetLastError());
+    for (int k = 0; k < nrhs; ++k) {
+        if (eta) eta[k] = h->err_eta[k];
+        if (res2) res2[k] = h->err_res2[k];
+        if (ferr) ferr[k] = h->err_ferr[k];
+    }
+    return MFD_OK;
 }
 
 int mfd_export_front(mfd_handle* h, int node, double* out)

@@ -13,7 +13,10 @@
 !     reference calls METIS_NodeND, unless hppllt_set_ordering(perm) (new ->
 !     old, 1-based) supplies one;
 !   - hppllt_set_precision(p): 0 = FP64 (default), 1 = FP32 fronts + FP64
-!     iterative refinement;
+!     iterative refinement, 2 / 3 = FP16 / BF16 updates of the diagonally
+!     scaled fronts + FP64 iterative refinement;
+!   - hppllt_get_solve_errors(eta, res2, ferr): per right-hand side backward
+!     error, ||b - Ax||_2/||x||_2 and forward error estimate of the last solve;
 !   - hppllt_solve_multi(b(:,:)) solves several right-hand sides at once;
 !   - hppllt_get_status() returns the last library return code (0 = ok, > 0 =
 !     1-based global column of a non-positive pivot, < 0 = error); the
@@ -30,7 +33,7 @@ module hppllt
   public :: hppllt_init, hppllt_analyze, hppllt_factorize, hppllt_solve, hppllt_finalize
   public :: hppllt_get_time_info, hppllt_get_cost, hppllt_get_nonzero, hppllt_get_array_size
   public :: hppllt_solve_multi, hppllt_set_ordering, hppllt_set_precision, hppllt_get_status
-  public :: hppllt_get_nonzero8, hppllt_get_backward_error
+  public :: hppllt_get_nonzero8, hppllt_get_backward_error, hppllt_get_solve_errors
 
   integer, parameter :: NUM_OPTIONS = 4
   integer, parameter :: OPTION_NB = 1, OPTION_MAX_ZERO = 2, OPTION_USE_STARPU = 3, OPTION_USE_METIS = 4
@@ -42,6 +45,7 @@ module hppllt
   integer(c_int), allocatable, target, save :: ordering(:)
   integer(c_int), save :: precision = 0
   integer(c_int), save :: status = 0
+  integer(c_int), save :: last_nrhs = 0
 
   type, bind(C) :: mfd_stats
     real(c_double) :: analyze_ms, assemble_ms, factor_ms, solve_ms
@@ -103,6 +107,12 @@ module hppllt
       import :: c_ptr, c_int, mfd_stats
       type(c_ptr), value :: h
       type(mfd_stats) :: st
+    end function
+    integer(c_int) function mfd_get_solve_errors(h, nrhs, eta, res2, ferr) bind(C, name="mfd_get_solve_errors")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: nrhs
+      type(c_ptr), value :: eta, res2, ferr
     end function
     type(c_ptr) function mfd_error_string(code) bind(C, name="mfd_error_string")
       import :: c_ptr, c_int
@@ -207,12 +217,14 @@ contains
   subroutine hppllt_solve(b)
     double precision, contiguous :: b(:)
     call report(mfd_solve(handle, b, 1_c_int, int(size(b), c_int)), "hppllt_solve")
+    last_nrhs = 1
   end subroutine
 
   ! Several right-hand sides: b(n, nrhs) is overwritten with X.
   subroutine hppllt_solve_multi(b)
     double precision, contiguous :: b(:, :)
     call report(mfd_solve(handle, b, int(size(b, 2), c_int), int(size(b, 1), c_int)), "hppllt_solve_multi")
+    last_nrhs = int(size(b, 2), c_int)
   end subroutine
 
   ! hppllt.f90:175-184; frees the device state.
@@ -256,13 +268,31 @@ contains
     rc = status
   end function
 
-  ! Normwise backward error of the last solve (max over right-hand sides);
-  ! -1 after an FP64 solve (only FP32 + refinement solves compute it).
+  ! Normwise backward error of the last solve (max over right-hand sides),
+  ! in either precision form; -1 before a solve.
   double precision function hppllt_get_backward_error() result(eta)
-    type(mfd_stats) :: st
+    double precision, allocatable, target :: e(:)
     integer(c_int) :: rc
-    rc = mfd_get_stats(handle, st)
-    eta = st%backward_error
+    eta = -1d0
+    if (last_nrhs < 1) return
+    allocate(e(last_nrhs))
+    rc = mfd_get_solve_errors(handle, last_nrhs, c_loc(e), c_null_ptr, c_null_ptr)
+    if (rc == 0) eta = maxval(e)
   end function
+
+  ! Errors of the last solve per right-hand side (each array sized to that
+  ! solve's number of right-hand sides): eta the normwise backward error,
+  ! res2 the reference driver's ||b - Ax||_2 / ||x||_2 (example.f90:147-180),
+  ! ferr a forward error estimate (one more solve with the current factor).
+  subroutine hppllt_get_solve_errors(eta, res2, ferr)
+    double precision, contiguous, intent(out), target :: eta(:), res2(:), ferr(:)
+    if (size(eta) /= last_nrhs .or. size(res2) /= last_nrhs .or. size(ferr) /= last_nrhs) then
+      print *, "hppllt_get_solve_errors: arrays must hold one entry per right-hand side of the last solve"
+      status = -1
+      return
+    endif
+    call report(mfd_get_solve_errors(handle, last_nrhs, c_loc(eta), c_loc(res2), c_loc(ferr)), &
+                "hppllt_get_solve_errors")
+  end subroutine
 
 end module

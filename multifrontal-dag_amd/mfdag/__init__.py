@@ -22,6 +22,8 @@ MFD_ERR_NODEVICE = -7
 MFD_ERR_UNSUPPORTED = -8
 FP64 = 0
 FP32_IR = 1
+FP16_IR = 2
+BF16_IR = 3
 FLAG_GRAPH = 1
 FLAG_PROFILE = 2
 FLAG_ORDER_ND = 4
@@ -72,6 +74,7 @@ def lib():
             "mfd_solve": (i, [P, dp, i, i]),
             "mfd_solve_permuted": (i, [P, dp, i, i]),
             "mfd_solve_device": (i, [P, C.c_void_p, i, i]),
+            "mfd_get_solve_errors": (i, [P, i, dp, dp, dp]),
             "mfd_export_tiles": (i, [P, dp, dp]),
             "mfd_export_front": (i, [P, i, dp]),
             "mfd_get_cost": (i64, [P]),
@@ -344,6 +347,20 @@ class Solver:
     def solve_device(self, d_b_ptr, nrhs, ldb):
         return self._check(lib().mfd_solve_device(self.h, C.c_void_p(d_b_ptr), int(nrhs), int(ldb)),
                            "mfd_solve_device")
+
+    def solve_errors(self, nrhs=1, forward=True):
+        """Errors of the last solve per right-hand side, computed on the GPU in
+        FP64: dict of arrays eta (normwise backward error), res2 (the
+        reference driver's ||b - Ax||_2/||x||_2) and, with forward=True, ferr
+        (forward error estimate from one more solve with the current factor)."""
+        eta, res2 = np.zeros(nrhs), np.zeros(nrhs)
+        ferr = np.zeros(nrhs) if forward else None
+        self._check(lib().mfd_get_solve_errors(self.h, int(nrhs), _ptr(eta, C.c_double), _ptr(res2, C.c_double),
+                                               _ptr(ferr, C.c_double)), "mfd_get_solve_errors")
+        out = dict(eta=eta, res2=res2)
+        if forward:
+            out["ferr"] = ferr
+        return out
 
     def export_tiles(self):
         sn = np.zeros(max(self.array_size(1), 1))

@@ -8,11 +8,11 @@ diagonal tasks.  Short periods keep the critical-path tasks light; long ones
 make fewer, deeper passes over the trailing columns.
 
 Two ways to choose it:
-  * measure (the exhaustive tuner): factor the analysed problem with every
-    period and keep the fastest; it records one sample per (problem,
-    precision, period);
+  * measure (the exhaustive tuner): factor (and, given right-hand sides,
+    solve) the analysed problem with every period and keep the fastest; it
+    records one sample per (problem, precision, period);
   * CostModel: a perceptron (the mfd_mlp_* HIP kernels) trained on such
-    records predicts the factorization time from the features of the
+    records predicts the factor + solve time from the features of the
     problem's front with the most pivot columns -- order n, update width
     K (64 period), update rows w, bytes per entry -- and picks the period with the smallest
     prediction, without a sweep.  The measured tuner is its fallback and the
@@ -80,14 +80,16 @@ def measure(solver, a, precision, periods=PERIODS, reps=3, rhs=None):
     return times
 
 
-def record(name, solver, a, precisions=(FP64, FP32_IR), periods=PERIODS, reps=3):
+def record(name, solver, a, rhs, precisions=(FP64, FP32_IR), periods=PERIODS, reps=3):
     """Tuner records of one analysed problem: dicts with the problem name,
-    precision, period, front features and measured ms."""
+    precision, period, front features and measured ms of factor + solve of
+    rhs (the quantity the period is chosen for: in the FP32+IR form it also
+    moves the number of refinement iterations)."""
     n, c, w = largest_front(solver)
     out = []
     for prec in precisions:
         eb = 8 if prec == FP64 else 4
-        for p, ms in measure(solver, a, prec, periods, reps).items():
+        for p, ms in measure(solver, a, prec, periods, reps, rhs=rhs).items():
             out.append(dict(problem=name, precision=prec, period=p, n=n, c=c, w=w, elem_bytes=eb, ms=ms,
                             x=features(n, p, w, eb)))
     return out

@@ -17,6 +17,7 @@ program hppllt_check
   double precision, allocatable :: xt(:), b(:), bm(:, :), xm(:, :)
   character(len=32) :: arg, mode
   double precision :: err
+  double precision :: eta3(3), res3(3), ferr3(3)
 
   call get_command_argument(1, arg); read (arg, *) k
   call get_command_argument(2, arg); read (arg, *) nb
@@ -79,6 +80,11 @@ program hppllt_check
     err = maxval(abs(bm - xm)) / maxval(abs(xm))
     print '(a,es10.3)', "multi_x_err ", err
     if (err > 1d-10) stop 1
+    call hppllt_get_solve_errors(eta3, res3, ferr3)
+    if (hppllt_get_status() /= 0) stop 1
+    print '(a,3es10.3)', "multi_backward_error ", eta3
+    print '(a,3es10.3)', "multi_forward_error ", ferr3
+    if (maxval(eta3) > 1d-14 .or. maxval(ferr3) > 1d-10 .or. hppllt_get_backward_error() /= maxval(eta3)) stop 1
 
     call hppllt_set_precision(1)
     call hppllt_factorize(a)

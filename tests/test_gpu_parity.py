@@ -416,3 +416,50 @@ def test_config_block_stencil_3dof_40(precision):
     s, xt, x, eta, st = solve_checks(p, 16384, 64, precision)
     log(f"blk3_40 precision {precision}: eta {eta:.2e}, iters {st['refine_iters']}")
     assert eta <= 1e-14
+
+
+@pytest.mark.parametrize("precision", [mfdag.FP64, mfdag.FP32_IR])
+def test_solve_errors_per_rhs_and_fp32_ir_x_against_oracle(precision):
+    """mfd_get_solve_errors: per right-hand side, the backward error and the
+    reference driver's ||b - Ax||_2/||x||_2 equal the host's values (to the
+    rounding of the residual), and the forward error estimate is of the order
+    of the error against the oracle's FP64 x.  The FP32+IR x agrees with the
+    oracle's x within the condition-aware bound of the FP64 parity test.
+    Before a solve, after a new factorization, and for another nrhs the call
+    is refused."""
+    p = P.weighted_laplacian_3d(12, seed=21)
+    s = mfdag.Solver(nb=32, graph=True)
+    s.analyze(p.ccs_col, p.ccs_row, 16, p.perm)
+    assert s.factorize(p.val, precision=precision) == 0
+    with pytest.raises(mfdag.MfdError):
+        s.solve_errors(3)
+    xt, b = P.rhs(p, 3, seed=22)
+    b[:, 1] *= 1e3  # right-hand sides of different scales
+    x = s.solve(b)
+    rep = s.solve_errors(3)
+    with pytest.raises(mfdag.MfdError):
+        s.solve_errors(2)
+    f, st, info = oracle_for(s, p.val)
+    assert info == 0
+    perm = st["perm"] - 1
+    A = p.scipy()
+    kappa = condition_number(p)
+    for k in range(3):
+        eta = P.backward_error(p, x[:, k], b[:, k])
+        r = b[:, k] - A @ x[:, k]
+        res2 = np.linalg.norm(r) / np.linalg.norm(x[:, k])
+        xo = np.empty(p.n)
+        xo[perm] = f.solve(b[perm, k].copy())
+        fwd = np.abs(x[:, k] - xo).max() / np.abs(xo).max()
+        tol = max(1e-12, 4.0 * kappa * max(eta, P.backward_error(p, xo, b[:, k])))
+        log(f"rhs {k}: eta {eta:.2e} reported {rep['eta'][k]:.2e}; res2 {res2:.2e} reported "
+            f"{rep['res2'][k]:.2e}; x vs oracle {fwd:.2e} (tol {tol:.2e}) ferr est {rep['ferr'][k]:.2e}")
+        assert eta <= 1e-14 and rep["eta"][k] <= 1e-14
+        assert rep["eta"][k] <= 4 * eta + 1e-18 and eta <= 4 * rep["eta"][k] + 1e-18
+        assert res2 / 10 <= rep["res2"][k] <= res2 * 10
+        assert fwd <= tol
+        assert rep["ferr"][k] <= tol
+    assert s.factorize(p.val, precision=precision) == 0
+    with pytest.raises(mfdag.MfdError):
+        s.solve_errors(3)
+    s.close()
