@@ -180,6 +180,106 @@ void orc_split_rect(const double *origin, double *left, double *right, int lsize
         for (int j = 1; j <= rsize; ++j) CM(right, rsize, j, i) = CM(origin, lo, lsize + j, i);
 }
 
+/* ------------------------------------------------------- BLAS/LAPACK hooks */
+/* The drivers below call the tile operations through these wrappers.  With
+   no hooks set they are the textbook kernels above (the checker's own
+   arithmetic); orc_set_blas installs a host BLAS/LAPACK (Fortran calling
+   convention, hidden string lengths last) for the CPU comparator, calling
+   the routines the reference calls at factorize_kernel_m.f90:8,15,22,32,
+   border_kernel_m.f90:9-30 and solve_kernel_m.f90:8,15,22,29. */
+typedef void (*potrf_f)(const char *, const int *, double *, const int *, int *, size_t);
+typedef void (*trsm_f)(const char *, const char *, const char *, const char *, const int *, const int *,
+                       const double *, const double *, const int *, double *, const int *, size_t, size_t, size_t,
+                       size_t);
+typedef void (*syrk_f)(const char *, const char *, const int *, const int *, const double *, const double *,
+                       const int *, const double *, double *, const int *, size_t, size_t);
+typedef void (*gemm_f)(const char *, const char *, const int *, const int *, const int *, const double *,
+                       const double *, const int *, const double *, const int *, const double *, double *,
+                       const int *, size_t, size_t);
+typedef void (*trsv_f)(const char *, const char *, const char *, const int *, const double *, const int *, double *,
+                       const int *, size_t, size_t, size_t);
+typedef void (*gemv_f)(const char *, const int *, const int *, const double *, const double *, const int *,
+                       const double *, const int *, const double *, double *, const int *, size_t);
+static struct {
+    potrf_f potrf;
+    trsm_f trsm;
+    syrk_f syrk;
+    gemm_f gemm;
+    trsv_f trsv;
+    gemv_f gemv;
+} blas;
+
+/* All six or none (NULL). */
+int orc_set_blas(void *potrf, void *trsm, void *syrk, void *gemm, void *trsv, void *gemv)
+{
+    int set = potrf && trsm && syrk && gemm && trsv && gemv;
+    blas.potrf = set ? (potrf_f)potrf : 0;
+    blas.trsm = set ? (trsm_f)trsm : 0;
+    blas.syrk = set ? (syrk_f)syrk : 0;
+    blas.gemm = set ? (gemm_f)gemm : 0;
+    blas.trsv = set ? (trsv_f)trsv : 0;
+    blas.gemv = set ? (gemv_f)gemv : 0;
+    return set;
+}
+
+static const double ONE = 1.0, MONE = -1.0;
+static const int INC1 = 1;
+/* dpotrf("U") */
+static void t_potrf(int n, double *a, int *info)
+{
+    *info = 0;
+    if (n <= 0) return;
+    if (blas.potrf) blas.potrf("U", &n, a, &n, info, 1);
+    else orc_dpotrf_u(n, a, info);
+}
+/* dtrsm("L", "U", "T", "N"): B <- U^{-T} B */
+static void t_trsm(int m, int nrhs, const double *u, double *b)
+{
+    if (m <= 0 || nrhs <= 0) return;
+    if (blas.trsm) blas.trsm("L", "U", "T", "N", &m, &nrhs, &ONE, u, &m, b, &m, 1, 1, 1, 1);
+    else orc_dtrsm_lutn(m, nrhs, u, b);
+}
+/* dsyrk("U", "T", alpha = -1, beta = 1) */
+static void t_syrk(int n, int k, const double *a, double *c)
+{
+    if (n <= 0 || k <= 0) return;
+    if (blas.syrk) blas.syrk("U", "T", &n, &k, &MONE, a, &k, &ONE, c, &n, 1, 1);
+    else orc_dsyrk_ut(n, k, a, c);
+}
+/* dgemm("T", "N", alpha = -1, beta = 1) */
+static void t_gemm(int m, int n, int k, const double *a, const double *b, double *c)
+{
+    if (m <= 0 || n <= 0 || k <= 0) return;
+    if (blas.gemm) blas.gemm("T", "N", &m, &n, &k, &MONE, a, &k, b, &k, &ONE, c, &m, 1, 1);
+    else orc_dgemm_tn(m, n, k, a, b, c);
+}
+/* dtrsv("U", "T", "N") and dtrsv("U", "N", "N") */
+static void t_trsv_ut(int n, const double *u, double *x)
+{
+    if (n <= 0) return;
+    if (blas.trsv) blas.trsv("U", "T", "N", &n, u, &n, x, &INC1, 1, 1, 1);
+    else orc_dtrsv_ut(n, u, x);
+}
+static void t_trsv_un(int n, const double *u, double *x)
+{
+    if (n <= 0) return;
+    if (blas.trsv) blas.trsv("U", "N", "N", &n, u, &n, x, &INC1, 1, 1, 1);
+    else orc_dtrsv_un(n, u, x);
+}
+/* dgemv("T" / "N", alpha = -1, beta = 1) */
+static void t_gemv_t(int m, int n, const double *a, const double *x, double *y)
+{
+    if (m <= 0 || n <= 0) return;
+    if (blas.gemv) blas.gemv("T", &m, &n, &MONE, a, &m, x, &INC1, &ONE, y, &INC1, 1);
+    else orc_dgemv_t(m, n, a, x, y);
+}
+static void t_gemv_n(int m, int n, const double *a, const double *x, double *y)
+{
+    if (m <= 0 || n <= 0) return;
+    if (blas.gemv) blas.gemv("N", &m, &n, &MONE, a, &m, x, &INC1, &ONE, y, &INC1, 1);
+    else orc_dgemv_n(m, n, a, x, y);
+}
+
 /* ------------------------------------------------------------- node data */
 /* node_data_m.f90:8-184 */
 typedef struct {
@@ -484,16 +584,16 @@ void orc_assemble(orc_t *h, const i64 *colptr, const int *rows, const double *va
 static int st_factorize(orc_t *h, int s, int j)
 {
     int info;
-    orc_dpotrf_u(matrix_block_size(h, j, s), get_matrix(h, s, j, j), &info);
+    t_potrf(matrix_block_size(h, j, s), get_matrix(h, s, j, j), &info);
     return info;
 }
 static void st_solve(orc_t *h, int s, int i, int j)
 {
-    orc_dtrsm_lutn(matrix_block_size(h, j, s), matrix_block_size(h, i, s), get_matrix(h, s, j, j), get_matrix(h, s, i, j));
+    t_trsm(matrix_block_size(h, j, s), matrix_block_size(h, i, s), get_matrix(h, s, j, j), get_matrix(h, s, i, j));
 }
 static void st_sym_update(orc_t *h, int s, int i, int j)
 {
-    orc_dsyrk_ut(matrix_block_size(h, i, s), matrix_block_size(h, j, s), get_matrix(h, s, i, j), get_matrix(h, s, i, i));
+    t_syrk(matrix_block_size(h, i, s), matrix_block_size(h, j, s), get_matrix(h, s, i, j), get_matrix(h, s, i, i));
 }
 /* factorize_subroutines_m.f90:51-65; the panel width is passed explicitly
    (the reference passes upper_n, which equals it whenever k > i) */
@@ -501,7 +601,7 @@ static void st_update(orc_t *h, int s, int up, int lo, int col)
 {
     int upper_n = matrix_block_size(h, up, s), lower_n = matrix_block_size(h, lo, s);
     int kw = matrix_block_size(h, col, s);
-    orc_dgemm_tn(upper_n, lower_n, kw, get_matrix(h, s, up, col), get_matrix(h, s, lo, col), get_matrix(h, s, lo, up));
+    t_gemm(upper_n, lower_n, kw, get_matrix(h, s, up, col), get_matrix(h, s, lo, col), get_matrix(h, s, lo, up));
 }
 
 /* seq_factorize_m.f90:31-52 */
@@ -533,22 +633,22 @@ static int border_factorize(orc_t *h, int s)
     /* border_potrf */
     double *sup = get_supernode(h, s, j, j);
     int info;
-    orc_dpotrf_u(ss, sup, &info);
+    t_potrf(ss, sup, &info);
     if (info) return (int)(h->first_col[s] - 1) + (j - 1) * h->nb + info;
-    orc_dtrsm_lutn(ss, wsz, sup, sup + (i64)ss * ss);
-    orc_dsyrk_ut(wsz, ss, sup + (i64)ss * ss, get_work(h, s, j, j));
+    t_trsm(ss, wsz, sup, sup + (i64)ss * ss);
+    t_syrk(wsz, ss, sup + (i64)ss * ss, get_work(h, s, j, j));
     /* border_trsm */
     for (int i = j + 1; i <= B; ++i) {
         int nrow = matrix_block_size(h, i, s);
         double *sol = get_supernode(h, s, i, j);
-        orc_dtrsm_lutn(ss, nrow, sup, sol);
-        orc_dgemm_tn(wsz, nrow, ss, sup + (i64)ss * ss, sol, get_work(h, s, i, j));
+        t_trsm(ss, nrow, sup, sol);
+        t_gemm(wsz, nrow, ss, sup + (i64)ss * ss, sol, get_work(h, s, i, j));
     }
     for (int i = j + 1; i <= B; ++i) {
         int nrow = matrix_block_size(h, i, s);
-        orc_dsyrk_ut(nrow, ss, get_supernode(h, s, i, j), get_matrix(h, s, i, i));
+        t_syrk(nrow, ss, get_supernode(h, s, i, j), get_matrix(h, s, i, i));
         for (int k = i + 1; k <= B; ++k)
-            orc_dgemm_tn(matrix_block_size(h, i, s), matrix_block_size(h, k, s), ss, get_supernode(h, s, i, j),
+            t_gemm(matrix_block_size(h, i, s), matrix_block_size(h, k, s), ss, get_supernode(h, s, i, j),
                          get_supernode(h, s, k, j), get_matrix(h, s, k, i));
     }
     return 0;
@@ -591,6 +691,167 @@ int orc_factorize(orc_t *h)
     return 0;
 }
 
+/* ------------------------------------------------------------ task DAG */
+/* The reference's StarPU task graph (starpu_factorize_m.f90:13-100) as
+   OpenMP tasks, for the CPU comparator: one task per tile operation, with
+   the access modes of the reference's codelets on the tiles it touches
+   (factorize_tasks_m.f90:12-15: POTRF RW, TRSM R/RW, SYRK R/RW|COMMUTE,
+   GEMM R/R/RW|COMMUTE; rearrange_tasks_m.f90:12-13: split R/RW/RW;
+   border_tasks_m.f90:12-13: border POTRF RW/RW, border TRSM R/RW/RW;
+   extend_add_tasks_m.f90:12-13: child R, parent RW).  R is depend(in), RW
+   depend(inout) and RW|COMMUTE depend(mutexinoutset), each on the tile's
+   first entry (the tile is the StarPU data handle).  Tasks are submitted in
+   the reference's order (nodes in postorder), so the depend clauses order
+   them as StarPU's sequential consistency does.  The tile operations are
+   the same wrappers as the sequential driver's (host BLAS when hooks are
+   set).  Returns 0 or the first failing global column. */
+static void dag_fail(int *fail, int col)
+{
+#pragma omp critical(orc_dag_fail)
+    if (*fail == 0 || col < *fail) *fail = col;
+}
+
+int orc_factorize_dag(orc_t *h, int nthreads)
+{
+    int fail = 0;
+    int *failp = &fail;
+    const int nb = h->nb;
+#pragma omp parallel num_threads(nthreads)
+#pragma omp single
+    for (int s = 1; s <= h->num_node; ++s) {
+        const int B = num_matrix_block(h, s), ws = work_start(h, s);
+        const int col0 = (int)(h->first_col[s] - 1);
+        for (int j = 1; j <= ws - 1; ++j) {
+            double *d = get_matrix(h, s, j, j);
+            const int nj = matrix_block_size(h, j, s);
+#pragma omp task depend(inout : d[0]) firstprivate(d, nj, j)
+            {
+                int info;
+                t_potrf(nj, d, &info);
+                if (info) dag_fail(failp, col0 + (j - 1) * nb + info);
+            }
+            for (int i = j + 1; i <= B; ++i) {
+                double *a = get_matrix(h, s, i, j);
+                const int ni = matrix_block_size(h, i, s);
+#pragma omp task depend(in : d[0]) depend(inout : a[0]) firstprivate(a, d, ni, nj)
+                t_trsm(nj, ni, d, a);
+            }
+            for (int i = j + 1; i <= B; ++i) {
+                double *a = get_matrix(h, s, i, j), *c = get_matrix(h, s, i, i);
+                const int ni = matrix_block_size(h, i, s);
+#pragma omp task depend(in : a[0]) depend(mutexinoutset : c[0]) firstprivate(a, c, ni, nj)
+                t_syrk(ni, nj, a, c);
+                for (int k = i + 1; k <= B; ++k) {
+                    double *b = get_matrix(h, s, k, j), *u = get_matrix(h, s, k, i);
+                    const int nk = matrix_block_size(h, k, s);
+#pragma omp task depend(in : a[0], b[0]) depend(mutexinoutset : u[0]) firstprivate(a, b, u, ni, nk, nj)
+                    t_gemm(ni, nk, nj, a, b, u);
+                }
+            }
+        }
+        if (exist_border(h, s)) {
+            /* seq_factorize_m.f90:54-79 as tasks */
+            const int j = ws, ss = h->ss[s], wsz = h->wsz[s];
+            double *bd = get_matrix(h, s, j, j), *sup = get_supernode(h, s, j, j), *wk = get_work(h, s, j, j);
+#pragma omp task depend(in : bd[0]) depend(inout : sup[0], wk[0]) firstprivate(bd, sup, wk, ss, wsz)
+            orc_split_tri(bd, sup, wk, ss, wsz);
+            for (int i = j + 1; i <= B; ++i) {
+                double *bi = get_matrix(h, s, i, j), *si = get_supernode(h, s, i, j), *wi = get_work(h, s, i, j);
+                const int ni = matrix_block_size(h, i, s);
+#pragma omp task depend(in : bi[0]) depend(inout : si[0], wi[0]) firstprivate(bi, si, wi, ss, wsz, ni)
+                orc_split_rect(bi, si, wi, ss, wsz, ni);
+            }
+#pragma omp task depend(inout : sup[0], wk[0]) firstprivate(sup, wk, ss, wsz, j)
+            {
+                /* border_potrf, border_kernel_m.f90:9-19 */
+                int info;
+                t_potrf(ss, sup, &info);
+                if (info) dag_fail(failp, col0 + (j - 1) * nb + info);
+                t_trsm(ss, wsz, sup, sup + (i64)ss * ss);
+                t_syrk(wsz, ss, sup + (i64)ss * ss, wk);
+            }
+            for (int i = j + 1; i <= B; ++i) {
+                double *si = get_supernode(h, s, i, j), *wi = get_work(h, s, i, j);
+                const int ni = matrix_block_size(h, i, s);
+#pragma omp task depend(in : sup[0]) depend(inout : si[0], wi[0]) firstprivate(sup, si, wi, ss, wsz, ni)
+                {
+                    /* border_trsm, border_kernel_m.f90:21-30 */
+                    t_trsm(ss, ni, sup, si);
+                    t_gemm(wsz, ni, ss, sup + (i64)ss * ss, si, wi);
+                }
+            }
+            for (int i = j + 1; i <= B; ++i) {
+                double *si = get_supernode(h, s, i, j), *c = get_matrix(h, s, i, i);
+                const int ni = matrix_block_size(h, i, s);
+#pragma omp task depend(in : si[0]) depend(mutexinoutset : c[0]) firstprivate(si, c, ni, ss)
+                t_syrk(ni, ss, si, c);
+                for (int k = i + 1; k <= B; ++k) {
+                    double *sk = get_supernode(h, s, k, j), *u = get_matrix(h, s, k, i);
+                    const int nk = matrix_block_size(h, k, s);
+#pragma omp task depend(in : si[0], sk[0]) depend(mutexinoutset : u[0]) firstprivate(si, sk, u, ni, nk, ss)
+                    t_gemm(ni, nk, ss, si, sk, u);
+                }
+            }
+        }
+        const int p = h->parent[s];
+        if (p > 0) {
+            /* seq_factorize_m.f90:81-98 as tasks (extend_add_subroutines_m.f90:13-59) */
+            const int G = (int)(h->grp_off[s + 1] - h->grp_off[s]);
+            for (int g = 1; g <= G; ++g) {
+                const i64 gj = h->grp_off[s] + g;
+                const int cj = h->grp_child[gj], pj = h->grp_parent[gj];
+                const int ldp = matrix_block_size(h, pj, p), ldc = work_block_size(h, cj, s);
+                const int *lj = group_local(h, s, g), nlj = h->grp_len[gj], oj = block_offset(h, s, g);
+                double *from = get_work(h, s, cj, cj), *to = get_matrix(h, p, pj, pj);
+#pragma omp task depend(in : from[0]) depend(inout : to[0]) firstprivate(from, to, lj, nlj, ldc, ldp, oj)
+                orc_extend_add_tri(from, to, lj, nlj, ldc, ldp, oj);
+                for (int i = g + 1; i <= G; ++i) {
+                    const i64 gi = h->grp_off[s] + i;
+                    const int ci = h->grp_child[gi], pi = h->grp_parent[gi];
+                    const int *li = group_local(h, s, i), nli = h->grp_len[gi], oi = block_offset(h, s, i);
+                    double *f2 = get_work(h, s, ci, cj), *t2 = get_matrix(h, p, pi, pj);
+#pragma omp task depend(in : f2[0]) depend(inout : t2[0]) firstprivate(f2, t2, lj, nlj, li, nli, ldc, ldp, oj, oi)
+                    orc_extend_add_rect(f2, t2, lj, nlj, li, nli, ldc, ldp, oj, oi);
+                }
+            }
+        }
+    }
+    return fail;
+}
+
+/* orc_assemble with the nodes spread over nthreads (the reference's
+   omp parallel do over nodes, zero_setter_m.f90:18, coefficient_setter_m.f90:25). */
+void orc_assemble_par(orc_t *h, const i64 *colptr, const int *rows, const double *vals, int nthreads)
+{
+    int nb = h->nb;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+    for (int s = 1; s <= h->num_node; ++s) {
+        int B = num_matrix_block(h, s);
+        for (int j = 1; j <= B; ++j) {
+            int ncol = matrix_block_size(h, j, s);
+            double *a = get_matrix(h, s, j, j);
+            for (int q = 1; q <= ncol; ++q)
+                for (int r = 1; r <= q; ++r) CM(a, ncol, r, q) = 0.0;
+            for (int i = j + 1; i <= B; ++i) {
+                int nrow = matrix_block_size(h, i, s);
+                a = get_matrix(h, s, i, j);
+                memset(a, 0, sizeof(double) * (size_t)ncol * nrow);
+            }
+        }
+        for (i64 jg = h->first_col[s]; jg < h->first_col[s + 1]; ++jg) {
+            int col = (int)(jg - h->first_col[s] + 1);
+            int bcol = div_ceiling(col, nb), bj = mod2(col, nb);
+            for (i64 p = colptr[jg - 1]; p < colptr[jg]; ++p) {
+                int row = rows[p];
+                int brow = div_ceiling(row, nb), bi = mod2(row, nb);
+                int ld = matrix_block_size(h, bcol, s);
+                double *m = get_matrix(h, s, brow, bcol);
+                CM(m, ld, bj, bi) = vals[p];
+            }
+        }
+    }
+}
+
 /* ---------------------------------------------------------------- solve */
 /* seq_forward_m.f90:14-81 */
 static void forward_sweep(orc_t *h)
@@ -598,9 +859,9 @@ static void forward_sweep(orc_t *h)
     for (int s = 1; s <= h->num_node; ++s) {
         int B = num_matrix_block(h, s), ws = work_start(h, s);
         for (int j = 1; j <= ws - 1; ++j) {
-            orc_dtrsv_ut(matrix_block_size(h, j, s), get_supernode(h, s, j, j), rh_array(h, s, j));
+            t_trsv_ut(matrix_block_size(h, j, s), get_supernode(h, s, j, j), rh_array(h, s, j));
             for (int i = j + 1; i <= B; ++i)
-                orc_dgemv_t(supernode_block_size(h, j, s), matrix_block_size(h, i, s), get_supernode(h, s, i, j),
+                t_gemv_t(supernode_block_size(h, j, s), matrix_block_size(h, i, s), get_supernode(h, s, i, j),
                             rh_array(h, s, j), rh_array(h, s, i));
         }
         if (exist_border(h, s)) {
@@ -611,10 +872,10 @@ static void forward_sweep(orc_t *h)
             for (int k = 0; k < nrow - ncol; ++k) wk[k] = bd[ncol + k];
             /* border_dtrsv_l, border_solve_kernel_m.f90:9-16 */
             double *a = get_supernode(h, s, j, j);
-            orc_dtrsv_ut(ncol, a, sp);
-            orc_dgemv_t(ncol, nrow - ncol, a + (i64)ncol * ncol, sp, wk);
+            t_trsv_ut(ncol, a, sp);
+            t_gemv_t(ncol, nrow - ncol, a + (i64)ncol * ncol, sp, wk);
             for (int i = j + 1; i <= B; ++i)
-                orc_dgemv_t(ncol, matrix_block_size(h, i, s), get_supernode(h, s, i, j), sp, rh_array(h, s, i));
+                t_gemv_t(ncol, matrix_block_size(h, i, s), get_supernode(h, s, i, j), sp, rh_array(h, s, i));
         }
         int p = h->parent[s];
         if (p > 0) {
@@ -652,19 +913,19 @@ static void backward_sweep(orc_t *h)
             int j = ws, ncol = supernode_block_size(h, j, s), nrow = matrix_block_size(h, j, s);
             double *sp = rh_supernode(h, s, j), *wk = rh_work(h, s, j), *bd = rh_border(h, s, j);
             for (int i = j + 1; i <= B; ++i)
-                orc_dgemv_n(ncol, matrix_block_size(h, i, s), get_supernode(h, s, i, j), rh_array(h, s, i), sp);
+                t_gemv_n(ncol, matrix_block_size(h, i, s), get_supernode(h, s, i, j), rh_array(h, s, i), sp);
             double *a = get_supernode(h, s, j, j);
-            orc_dgemv_n(ncol, nrow - ncol, a + (i64)ncol * ncol, wk, sp);
-            orc_dtrsv_un(ncol, a, sp);
+            t_gemv_n(ncol, nrow - ncol, a + (i64)ncol * ncol, wk, sp);
+            t_trsv_un(ncol, a, sp);
             /* join_array, array_rearrange_kernel_m.f90:17-24 */
             for (int k = 0; k < ncol; ++k) bd[k] = sp[k];
             for (int k = 0; k < nrow - ncol; ++k) bd[ncol + k] = wk[k];
         }
         for (int j = ws - 1; j >= 1; --j) {
             for (int i = j + 1; i <= B; ++i)
-                orc_dgemv_n(supernode_block_size(h, j, s), matrix_block_size(h, i, s), get_supernode(h, s, i, j),
+                t_gemv_n(supernode_block_size(h, j, s), matrix_block_size(h, i, s), get_supernode(h, s, i, j),
                             rh_array(h, s, i), rh_array(h, s, j));
-            orc_dtrsv_un(matrix_block_size(h, j, s), get_supernode(h, s, j, j), rh_array(h, s, j));
+            t_trsv_un(matrix_block_size(h, j, s), get_supernode(h, s, j, j), rh_array(h, s, j));
         }
     }
 }

@@ -68,8 +68,53 @@ def lib():
         L.orc_extend_add_tri.argtypes = [dp, dp, ip, i, i, i, i]
         L.orc_split_tri.argtypes = [dp, dp, dp, i, i]
         L.orc_split_rect.argtypes = [dp, dp, dp, i, i, i]
+        L.orc_set_blas.argtypes = [P] * 6
+        L.orc_set_blas.restype = i
+        L.orc_factorize_dag.argtypes = [P, i]
+        L.orc_factorize_dag.restype = i
+        L.orc_assemble_par.argtypes = [P, lp, ip, dp, i]
         _lib = L
     return _lib
+
+
+_blas = None
+
+
+def host_blas():
+    """The host BLAS/LAPACK the CPU comparator calls for its tile operations:
+    scipy's bundled OpenBLAS (LP64), one thread per call (the reference links
+    sequential MKL, camphor2.cmake:2), installed into the oracle's tile
+    wrappers.  Returns a description, or None (the oracle's own textbook
+    kernels stay) when it is not found."""
+    global _blas
+    if _blas is None:
+        import glob
+        try:
+            import scipy
+        except ImportError:
+            return None
+        libs = sorted(glob.glob(os.path.join(os.path.dirname(os.path.dirname(scipy.__file__)), "scipy.libs",
+                                             "libscipy_openblas*.so")))
+        if not libs:
+            return None
+        B = C.CDLL(libs[0])
+        B.scipy_openblas_set_num_threads.argtypes = [C.c_int]
+        B.scipy_openblas_set_num_threads(1)
+        names = ["dpotrf_", "dtrsm_", "dsyrk_", "dgemm_", "dtrsv_", "dgemv_"]
+        ptrs = [C.cast(getattr(B, "scipy_" + n), C.c_void_p) for n in names]
+        if not lib().orc_set_blas(*ptrs):
+            return None
+        cfg = B.scipy_openblas_get_config
+        cfg.restype = C.c_char_p
+        _blas = (B, f"scipy-bundled OpenBLAS ({os.path.basename(libs[0])}; {cfg().decode().strip()}), 1 thread per call")
+    return _blas[1]
+
+
+def textbook_kernels():
+    """Back to the oracle's own tile kernels (after host_blas)."""
+    global _blas
+    lib().orc_set_blas(None, None, None, None, None, None)
+    _blas = None
 
 
 def _i32(a):
@@ -106,6 +151,16 @@ class OracleFactor:
 
     def factorize(self):
         return lib().orc_factorize(self.h)
+
+    def assemble_par(self, colptr, rows, vals, nthreads):
+        """assemble with the nodes spread over nthreads threads."""
+        lib().orc_assemble_par(self.h, np.ascontiguousarray(colptr, np.int64), _i32(rows),
+                               np.ascontiguousarray(vals, np.float64), int(nthreads))
+
+    def factorize_dag(self, nthreads):
+        """The reference's StarPU task graph as OpenMP tasks on nthreads
+        threads (orc_factorize_dag)."""
+        return lib().orc_factorize_dag(self.h, int(nthreads))
 
     def solve(self, bperm):
         x = np.array(bperm, dtype=np.float64)
