@@ -16,7 +16,10 @@ Two ways to choose it:
     problem's front with the most pivot columns -- order n, update width
     K (64 period), update rows w, bytes per entry -- and picks the period with the smallest
     prediction, without a sweep.  The measured tuner is its fallback and the
-    yardstick of its regret (regret below).
+    yardstick of its regret (regret below).  The model covers the FP64 form
+    (MODEL_FORMS): in the FP32+IR form the period also moves the number of
+    refinement iterations (a whole solve sweep each), a jump that these
+    features do not predict, so that form is tuned by measure().
 """
 import math
 
@@ -25,6 +28,7 @@ import numpy as np
 from . import FP64, FP32_IR, MLP_HIDDEN, MLP_INPUTS, MLP_PARAMS, mlp_predict, mlp_train
 
 PERIODS = (1, 2, 3, 4)
+MODEL_FORMS = (FP64,)  # forms the cost model is trained for and chooses in
 AUTO = 0  # per level: 1 for levels whose longest front has < 48 steps, else 2
 DEFAULT_PERIOD = 2
 

@@ -192,7 +192,7 @@ def sweep(cases):
         s = mfdag.Solver(nb=64, graph=True)
         s.analyze(p.ccs_col, p.ccs_row, 16384, p.perm)
         xt, b = P.rhs(p, 1, seed=1)
-        r = tune.record(name, s, p.val, b[:, 0])
+        r = tune.record(name, s, p.val, b[:, 0], precisions=tune.MODEL_FORMS)
         log(name, [(q["precision"], q["period"], round(q["ms"], 3)) for q in r])
         recs += r
         s.close()
