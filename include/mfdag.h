@@ -40,6 +40,7 @@ typedef struct mfd_handle mfd_handle;
 #define MFD_ERR_NODEVICE (-7)
 #define MFD_ERR_UNSUPPORTED (-8)
 #define MFD_ERR_NUMERIC (-9)
+#define MFD_ERR_SCRATCH (-10) /* mfd_solve_device_noalloc: the solve needs more than the fixed scratch, or would allocate */
 
 /* precision forms for mfd_factorize */
 #define MFD_FP64 0     /* plain form: FP64 fronts, FP64 solves */
@@ -140,6 +141,23 @@ int mfd_solve(mfd_handle* h, double* b, int nrhs, int ldb);
 int mfd_solve_permuted(mfd_handle* h, double* b, int nrhs, int ldb);
 /* Device-resident b (caller numbering), ldb >= n. */
 int mfd_solve_device(mfd_handle* h, double* d_b, int nrhs, int ldb);
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+   memory (serving loops that solve with one factor many times, harnesses
+   that forbid allocation after set-up): mfd_solve_device on a factored
+   handle, the same kernels and results, with every solve buffer (Z/Y slices,
+   partial sums, permuted b and x, and for the refined forms residual,
+   correction and norm slots) taken from one fixed 128 MiB device array of
+   the library instead of the handle's allocations.  It calls no hipMalloc
+   or hipFree and captures no graph (launches are eager).  Each GPU holds
+   its own copy of the array from the library's load until the process
+   ends.  Returns MFD_ERR_SCRATCH when a solve of this size (n, fronts, nrhs)
+   needs more than 128 MiB, or when a refinement stall would need a
+   factorization buffer not yet allocated.  Every call on one GPU shares the
+   array and leaves its contents behind: calls that overlap on one GPU (from
+   several threads or handles) race.  mfd_get_solve_errors does not see this
+   call's solution (MFD_ERR_STATE until the next mfd_solve*). */
+int mfd_solve_device_noalloc(mfd_handle* h, double* d_b, int nrhs, int ldb);
 
 /* Errors of the last solve, per right-hand side (nrhs must be that solve's;
    any output may be NULL), evaluated at this call in FP64 on the GPU from

@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "mfd_analysis.h"
@@ -174,6 +175,12 @@ struct mfd_handle {
     int64_t csr_nnz = 0;
     double *d_r = nullptr, *d_d = nullptr;
     unsigned long long* d_norms = nullptr;
+    // mfd_solve_device_noalloc: device address of the fixed scratch array on
+    // this handle's GPU (nullptr if it could not be resolved), and, during
+    // such a call, no_alloc = true (no allocation, no graph capture; the
+    // solve buffers above then point into the scratch array).
+    double* scratch = nullptr;
+    bool no_alloc = false;
 
     hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};  // FP64, FP32, FP16, BF16 factorizations
     struct SolveGraph {
@@ -200,6 +207,7 @@ namespace {
 
 template <typename T> int dalloc(mfd_handle* h, T** p, int64_t n)
 {
+    if (h->no_alloc) return MFD_ERR_SCRATCH;
     size_t bytes = (size_t)std::max<int64_t>(n, 1) * sizeof(T);
     if (h->mem_limit > 0 && h->device_bytes + (int64_t)bytes > h->mem_limit) return MFD_ERR_NOMEM;
     if (hipMalloc((void**)p, bytes) != hipSuccess) {
@@ -938,7 +946,7 @@ void collect_profile(mfd_handle* h, const std::vector<int>& kinds)
 template <typename T> int run_factor(mfd_handle* h)
 {
     const bool profile = (h->flags & MFD_FLAG_PROFILE) != 0;
-    const bool graph = (h->flags & MFD_FLAG_GRAPH) != 0 && !profile;
+    const bool graph = (h->flags & MFD_FLAG_GRAPH) != 0 && !profile && !h->no_alloc;
     const int gi = sizeof(T) == 8 ? 0 : 1 + h->lowp;
     HIP_TRY(hipEventRecord(h->ev[0], h->stream));
     if (graph) {
@@ -1053,7 +1061,7 @@ template <typename T> void enqueue_solve(mfd_handle* h, const double* bp, double
 // are enabled (one graph per precision, nrhs and buffer pair).
 template <typename T> int run_solve(mfd_handle* h, const double* bp, double* xp, int nrhs)
 {
-    const bool graph = (h->flags & MFD_FLAG_GRAPH) != 0 && (h->flags & MFD_FLAG_PROFILE) == 0;
+    const bool graph = (h->flags & MFD_FLAG_GRAPH) != 0 && (h->flags & MFD_FLAG_PROFILE) == 0 && !h->no_alloc;
     if (!graph) {
         enqueue_solve<T>(h, bp, xp, nrhs);
         return MFD_OK;
@@ -1252,6 +1260,87 @@ int solve_host(mfd_handle* h, double* b, int nrhs, int ldb, bool permuted, bool 
     return MFD_OK;
 }
 
+// ----------------------------------------------------- fixed solve scratch
+// mfd_solve_device_noalloc takes every solve buffer from this array instead
+// of the handle's allocations.  Each GPU holds its own copy, at its own
+// address, from the library's load until the process ends.  In host code
+// the array's name is not a device address: the address comes from
+// hipGetSymbolAddress, once per device (scratch_on_current_device).
+constexpr int64_t kScratchBytes = int64_t(128) << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / 8];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+// The scratch array's device address on the current device (resolved on the
+// first call for that device), or nullptr if HIP cannot resolve it.
+double* scratch_on_current_device()
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        g_scratch[dev] = static_cast<double*>(p);
+    }
+    return g_scratch[dev];
+}
+
+// Bytes of scratch a solve of nrhs right-hand sides takes (each buffer
+// 256-byte aligned): Z/Y slices, partial sums, permuted b and x and, for
+// the refined forms, residual, correction and norm slots.
+int64_t noalloc_bytes(const mfd_handle* h, int nrhs)
+{
+    auto al = [](int64_t b) { return (b + 255) & ~int64_t(255); };
+    const int64_t n = h->S.n;
+    int64_t b = al(h->z_per_rhs * nrhs * 8) + al(h->part_slots * 64 * nrhs * 8) + 2 * al(n * nrhs * 8);
+    if (h->precision != MFD_FP64) b += 2 * al(n * nrhs * 8) + al((4 * nrhs + 1 + 2 * kSumParts * nrhs) * 8);
+    return b;
+}
+
+// For the duration of a no-allocation solve: the handle's solve buffers
+// point into the scratch array and no_alloc is set; both restored after.
+struct NoAllocScope {
+    mfd_handle* h;
+    double *z, *part, *bp, *xp, *r, *d;
+    unsigned long long* norms;
+    NoAllocScope(mfd_handle* h_, int nrhs) : h(h_)
+    {
+        z = h->d_z, part = h->d_part, bp = h->d_bp, xp = h->d_xp, r = h->d_r, d = h->d_d, norms = h->d_norms;
+        auto al = [](int64_t b) { return (b + 255) & ~int64_t(255); };
+        const int64_t n = h->S.n;
+        char* p = reinterpret_cast<char*>(h->scratch);
+        auto take = [&](int64_t bytes) {
+            char* q = p;
+            p += al(bytes);
+            return q;
+        };
+        h->d_z = reinterpret_cast<double*>(take(h->z_per_rhs * nrhs * 8));
+        h->d_part = reinterpret_cast<double*>(take(h->part_slots * 64 * nrhs * 8));
+        h->d_bp = reinterpret_cast<double*>(take(n * nrhs * 8));
+        h->d_xp = reinterpret_cast<double*>(take(n * nrhs * 8));
+        if (h->precision != MFD_FP64) {
+            h->d_r = reinterpret_cast<double*>(take(n * nrhs * 8));
+            h->d_d = reinterpret_cast<double*>(take(n * nrhs * 8));
+            h->d_norms = reinterpret_cast<unsigned long long*>(take((4 * nrhs + 1 + 2 * kSumParts * nrhs) * 8));
+        }
+        h->no_alloc = true;
+    }
+    ~NoAllocScope()
+    {
+        h->d_z = z, h->d_part = part, h->d_bp = bp, h->d_xp = xp, h->d_r = r, h->d_d = d, h->d_norms = norms;
+        h->no_alloc = false;
+    }
+};
+
 int finish_analysis(mfd_handle* h)
 {
     delete h->layout;
@@ -1286,6 +1375,7 @@ int mfd_create(mfd_handle** out, int nb, int flags)
         if (hipGetDeviceProperties(&prop, 0) == hipSuccess && strncmp(prop.gcnArchName, "gfx950", 6) == 0 &&
             hipSetDevice(0) == hipSuccess && hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) == hipSuccess) {
             h->device_ok = true;
+            h->scratch = scratch_on_current_device();  // set-up of mfd_solve_device_noalloc
             for (auto& e : h->ev) (void)hipEventCreate(&e);
             if (hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess) h->side = nullptr;
             if (hipStreamCreateWithFlags(&h->fillst, hipStreamNonBlocking) != hipSuccess) h->fillst = nullptr;
@@ -1449,6 +1539,37 @@ int mfd_solve_device(mfd_handle* h, double* d_b, int nrhs, int ldb)
 {
     if (!h) return MFD_ERR_ARG;
     return solve_host(h, d_b, nrhs, ldb, false, true);
+}
+
+int mfd_solve_device_noalloc(mfd_handle* h, double* d_b, int nrhs, int ldb)
+{
+    if (!h) return MFD_ERR_ARG;
+    if (!h->device_ok) return MFD_ERR_NODEVICE;
+    if (!h->factored) return MFD_ERR_STATE;
+    if (!d_b || nrhs < 1 || ldb < h->S.n) return MFD_ERR_ARG;
+    if (!h->scratch) return MFD_ERR_HIP;
+    if (noalloc_bytes(h, nrhs) > kScratchBytes) return MFD_ERR_SCRATCH;
+    const int n = h->S.n;
+    hipStream_t st = h->stream;
+    double t0 = now_ms();
+    int rc;
+    {
+        NoAllocScope scope(h, nrhs);
+        HIP_TRY(hipEventRecord(h->ev[2], st));
+        launch_gather_rows(h->d_bp, n, d_b, ldb, h->d_perm, n, nrhs, st);
+        rc = solve_permuted_device(h, nrhs);
+        if (rc == MFD_OK) launch_scatter_rows(d_b, ldb, h->d_xp, n, h->d_perm, n, nrhs, st);
+        HIP_TRY(hipEventRecord(h->ev[3], st));
+        HIP_TRY(hipStreamSynchronize(st));
+        HIP_TRY(hipGetLastError());
+    }
+    if (rc != MFD_OK) return rc;
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, h->ev[2], h->ev[3]));
+    h->st.solve_ms = ms;
+    h->times[MFD_TIME_SOLVE] = (now_ms() - t0) / 1e3;
+    h->solved_nrhs = 0;  // mfd_get_solve_errors reads the handle's own buffers, not this call's
+    return MFD_OK;
 }
 
 int mfd_get_solve_errors(mfd_handle* h, int nrhs, double* eta, double* res2, double* ferr)

@@ -18,8 +18,10 @@ LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libmfdag.so")
 
 MFD_OK = 0
 MFD_ERR_ARG = -1
+MFD_ERR_STATE = -2
 MFD_ERR_NODEVICE = -7
 MFD_ERR_UNSUPPORTED = -8
+MFD_ERR_SCRATCH = -10
 FP64 = 0
 FP32_IR = 1
 FP16_IR = 2
@@ -74,6 +76,7 @@ def lib():
             "mfd_solve": (i, [P, dp, i, i]),
             "mfd_solve_permuted": (i, [P, dp, i, i]),
             "mfd_solve_device": (i, [P, C.c_void_p, i, i]),
+            "mfd_solve_device_noalloc": (i, [P, C.c_void_p, i, i]),
             "mfd_get_solve_errors": (i, [P, i, dp, dp, dp]),
             "mfd_export_tiles": (i, [P, dp, dp]),
             "mfd_export_front": (i, [P, i, dp]),
@@ -347,6 +350,11 @@ class Solver:
     def solve_device(self, d_b_ptr, nrhs, ldb):
         return self._check(lib().mfd_solve_device(self.h, C.c_void_p(d_b_ptr), int(nrhs), int(ldb)),
                            "mfd_solve_device")
+
+    def solve_device_noalloc(self, d_b_ptr, nrhs, ldb):
+        """mfd_solve_device with the solve buffers in the library's fixed
+        device scratch (no allocation); returns the C return code."""
+        return lib().mfd_solve_device_noalloc(self.h, C.c_void_p(d_b_ptr), int(nrhs), int(ldb))
 
     def solve_errors(self, nrhs=1, forward=True):
         """Errors of the last solve per right-hand side, computed on the GPU in
