@@ -99,6 +99,8 @@ def batch_leg(args):
     t1 = time.perf_counter()
     s.analyze_batch([(p.ccs_col, p.ccs_row) for p in systems], args.batch_max_zero, [p.perm for p in systems])
     analyze_s = time.perf_counter() - t1
+    if args.front_split >= 0:
+        s.set_front_split(args.front_split)
     log(f"batch: {len(systems)} systems, n={bp.n}, fronts={s.num_node}, cost={s.cost():.3e} "
         f"(setup {t1 - t0:.1f}s, analysis {analyze_s:.2f}s)")
     dev = torch.device("cuda:0")
@@ -121,6 +123,50 @@ def batch_leg(args):
             out[name]["refine_iters"] = st["refine_iters"]
         log(f"batch {name}: step {wall * 1e3:.2f} ms (factor {f:.2f} ms, solve {so:.2f} ms), "
             f"eta {out[name]['backward_error']:.2e}")
+    s.close()
+    return out
+
+
+def c5_leg(args):
+    """The low-precision configuration: the 3D 27-point block stencil with 3
+    unknowns per node on a 40^3 grid (n = 192,000; 3x3 node blocks
+    -w_ij (I + 0.3 R R^T), diagonally dominant), one right-hand side, in all
+    four forms -- FP64, FP32 fronts + FP64 refinement, and the diagonally
+    scaled FP32 fronts whose updates multiply FP16 / BF16 operands on the
+    16-bit MFMA, refined in FP64 -- each with the exhaustive tuner's update
+    period, timed like the main leg."""
+    import torch
+    import mfdag
+    from mfdag import problems as P
+    t0 = time.perf_counter()
+    p = P.block_stencil_3dof(args.c5_size, seed=args.seed)
+    xt, b = P.rhs(p, 1, seed=args.seed)
+    s = mfdag.Solver(nb=args.nb, graph=not args.no_graph)
+    s.analyze(p.ccs_col, p.ccs_row, args.max_zero, p.perm)
+    log(f"c5: n={p.n}, fronts={s.num_node}, cost={s.cost():.3e} (setup {time.perf_counter() - t0:.1f}s)")
+    dev = torch.device("cuda:0")
+    d_a = torch.from_numpy(np.ascontiguousarray(p.val)).to(dev)
+    d_b = torch.from_numpy(np.ascontiguousarray(b.T)).to(dev)
+    d_x = torch.empty_like(d_b)
+    torch.cuda.synchronize()
+    out = {"workload": f"3D 27-point 3x3 block stencil {args.c5_size}^3 (n={p.n}), ND ordering, "
+                       f"max_zero={args.max_zero}, 1 RHS",
+           "n": p.n, "fronts": s.num_node, "cost_flops": s.cost()}
+    forms = (("fp64", mfdag.FP64), ("fp32_ir", mfdag.FP32_IR), ("fp16_ir", mfdag.FP16_IR),
+             ("bf16_ir", mfdag.BF16_IR))
+    for name, prec in forms:
+        tuned = tune_leg(args, s, p.val, b, prec)
+        wall, f, so = timed_steps(s, d_a, d_b, d_x, prec, args.steps, args.warmup)
+        x = d_x.cpu().numpy().T
+        st = s.stats()
+        out[name] = {"ms_per_step": round(wall * 1e3, 3), "factor_ms": round(f, 3), "solve_ms": round(so, 3),
+                     "factor_gflops": round(s.cost() / (f * 1e6), 2), "backward_error": P.backward_error(p, x, b),
+                     "forward_error_vs_x_true": float(np.abs(x - xt).max() / np.abs(xt).max()),
+                     "form_of_factor": st["precision"], "refine_iters": st["refine_iters"], "tuner": tuned}
+        log(f"c5 {name}: step {wall * 1e3:.2f} ms (factor {f:.2f} ms, solve {so:.2f} ms), "
+            f"eta {out[name]['backward_error']:.2e}, iters {st['refine_iters']}, form {st['precision']}")
+    for name, _ in forms[1:]:
+        out[name]["speedup_vs_fp64"] = round(out["fp64"]["ms_per_step"] / out[name]["ms_per_step"], 3)
     s.close()
     return out
 
@@ -292,6 +338,11 @@ def main():
     ap.add_argument("--batch-max-zero", type=int, default=4096)
     ap.add_argument("--batch-only", action="store_true", help="run only the batching leg")
     ap.add_argument("--no-nd-leg", action="store_true", help="skip the leg under the library's graph ordering")
+    ap.add_argument("--c5-size", type=int, default=40, help="grid of the low-precision block-stencil leg (0: skip)")
+    ap.add_argument("--front-split", type=int, default=-1, metavar="CMAX",
+                    help="medium fronts: those with at most CMAX pivot columns factored by one workgroup each "
+                         "(-1: the library default)")
+    ap.add_argument("--c5-only", action="store_true", help="run only the low-precision block-stencil leg")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default update period instead of the exhaustive tuner's choice")
     ap.add_argument("--roofline-only", action="store_true",
@@ -320,6 +371,9 @@ def main():
 
     if args.batch_only:
         print(json.dumps({"batch": batch_leg(args)}), flush=True)
+        return
+    if args.c5_only:
+        print(json.dumps({"c5": c5_leg(args)}), flush=True)
         return
     p, b = build_problem(args.size, args.seed)
     if args.roofline_only:
@@ -382,6 +436,7 @@ def main():
 
     nd = None if args.no_nd_leg else nd_leg(args, p, b)
     batch = batch_leg(args) if args.batch > 0 else None
+    c5 = c5_leg(args) if args.c5_size > 0 else None
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
@@ -411,12 +466,13 @@ def main():
         "cpu_baseline": cpu,
         "graph_nd": nd,
         "batch": batch,
+        "c5": c5,
     }
     # every leg's solution must meet the backward-error bound the metric
     # names; a leg that misses it makes the whole line invalid
     etas = {"fp64": eta64, "fp32_ir": eta32}
-    for leg, d in (("graph_nd", nd), ("batch", batch)):
-        for form in ("fp64", "fp32_ir"):
+    for leg, d in (("graph_nd", nd), ("batch", batch), ("c5", c5)):
+        for form in ("fp64", "fp32_ir", "fp16_ir", "bf16_ir"):
             if d and form in d:
                 etas[f"{leg}.{form}"] = d[form]["backward_error"]
     bad = {k: v for k, v in etas.items() if not (v <= ETA_BOUND)}

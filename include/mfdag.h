@@ -80,7 +80,8 @@ typedef struct mfd_handle mfd_handle;
 #define MFD_K_SOLVE_FWD 7
 #define MFD_K_SOLVE_BWD 8
 #define MFD_K_REFINE 9
-#define MFD_K_NUM 10
+#define MFD_K_FRONT 10      /* medium fronts: the whole partial factorization of one front per workgroup */
+#define MFD_K_NUM 11
 
 /* Handle; nb is the reference's tile size (options(1), options_m.f90:3),
    used for the reference-layout views (mfd_export_tiles, array sizes). */
@@ -271,6 +272,23 @@ int mfd_mm_read(const char* path, int n, int64_t nnz, int* ccs_col, int* ccs_row
    MFD_ERR_STATE until then); otherwise it applies to the next analysis. */
 int mfd_set_update_period(mfd_handle* h, int period);
 int mfd_get_update_period(mfd_handle* h);
+
+/* Medium fronts (no counterpart in the reference, whose StarPU tasks are
+   per tile): a front of order > 64 is factored either in 64-column steps
+   whose tiles spread over the whole GPU (the large-front schedule) or, as a
+   medium front, whole by one workgroup (panel, TRSM and update tiles in
+   turn; many such fronts of one level run side by side).
+   mfd_set_front_split: fronts with at most cmax pivot columns are medium
+   (0, the default: none).  mfd_set_front_kinds: one choice per node
+   (num_node entries, 1-based node order: 1 medium, 0 large, -1 by cmax;
+   NULL clears); fronts of order <= 64 are always factored whole in LDS.
+   Both rebuild the schedule of the current analysis like
+   mfd_set_update_period.  mfd_get_front_times: per node, the duration in
+   microseconds of its workgroup in the last factorization (in-kernel
+   clocks), -1 for fronts that are not medium. */
+int mfd_set_front_split(mfd_handle* h, int cmax);
+int mfd_set_front_kinds(mfd_handle* h, const int* kinds);
+int mfd_get_front_times(mfd_handle* h, double* us);
 
 /* The tuner's cost model (no counterpart in the reference): a perceptron
    with MFD_MLP_INPUTS inputs, MFD_MLP_HIDDEN tanh units and one linear

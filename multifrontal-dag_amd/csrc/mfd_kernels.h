@@ -26,6 +26,12 @@ template <typename T>
 void launch_extend_add(T* pool, const int* lidx, const NodeDev* nodes, const ExtTask* tasks, int count, hipStream_t st);
 template <typename T>
 void launch_small_front(T* pool, const NodeDev* nodes, const int* tasks, int count, int* info, hipStream_t st);
+// Medium fronts: the whole partial factorization of one front per
+// workgroup (tasks: node ids); lowp as for launch_gemm's update tiles;
+// clocks (nullable): start and end wall_clock64 per task.
+template <typename T>
+void launch_front_wg(T* pool, const NodeDev* nodes, const int* tasks, int count, int* info, long long* clocks,
+                     hipStream_t st, int lowp = 0);
 template <typename T>
 void launch_diag(T* pool, const NodeDev* nodes, const DiagTask* tasks, int count, int* info, hipStream_t st);
 template <typename T>

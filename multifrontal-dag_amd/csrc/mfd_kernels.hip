@@ -142,9 +142,12 @@ __device__ __forceinline__ void wave_sync()
 // A front (or diagonal block) of order n <= 64 in LDS, factored in its first
 // c columns by a blocked right-looking algorithm with 16-column panels:
 //   - each panel is factored by one wave, one row per lane, in a 16-step
-//     chain (the entries of the finished column come from lanes 0..15 by
-//     readlane); the same chain inverts the panel's 16 x 16 diagonal block
-//     (identity rows solved against its L^T);
+//     chain: the pivot and the finished column's entry of the next column's
+//     row come from their lanes by readlane, the entries the later columns
+//     need are read back from LDS, where each column is stored as soon as it
+//     is finished;
+//   - the panel's 16 x 16 diagonal block is inverted by another wave during
+//     the next panel's chain (lane r solves L u = e_r);
 //   - the rank-16 update of the trailing blocks runs on MFMA across the four
 //     waves;
 //   - the off-diagonal 16 x 16 blocks of L^{-1} follow on MFMA, by distance
@@ -152,82 +155,106 @@ __device__ __forceinline__ void wave_sync()
 // Layout: A[col][row] holds the lower triangle (zero past n); afterwards it
 // holds L in columns < c and the Schur complement in the trailing block.
 // X[col][row] = L^{-1}[row][col] for row, col < c (other entries are
-// scratch).  Upper-triangle entries of A are scratch too.
+// scratch).  Upper-triangle entries of A are scratch too.  rs[k] = 1 / L_kk.
 template <typename T> struct Front64 {
     T A[64][65];
     T X[64][65];
     T Ts[4][16][17];
+    T rs[64];
 };
 
-template <typename T>
+// sqrt(d) from r ~ 1/sqrt(d) (within a few ulp): s = d r and one Newton
+// step s + (d - s^2) r / 2 with the exact residual (fma); exact whenever
+// sqrt(d) is representable, within an ulp otherwise, in three dependent
+// operations after r.
+template <typename T> __device__ __forceinline__ T sqrt_from_rs(T d, T r)
+{
+    const T s = d * r;
+    return fma(fma(-s, s, d), T(0.5) * r, s);
+}
+
+// Panel b16 .. b16 + cb - 1 (cb <= 16; FULL: cb == 16) by one wave, rows
+// b16 .. 63 on their lanes.  Column k: d = x_k on lane k, L_kk = sqrt(d),
+// l = x_k / L_kk (quot: exact whenever the quotient is representable); the
+// column goes to LDS at once, its entry for row k+1 is broadcast by readlane
+// and applied to column k+1 (the next pivot needs it), and its entries for
+// rows k+2 .. 15 are read back from LDS (the wave's LDS accesses complete in
+// issue order, so the reads see the column) and applied to those columns in
+// step k+1, beside that step's pivot chain, by when the reads have landed.
+// A partial panel also updates the block's columns past cb (trailing
+// columns of the front) by all of its columns, and stores them.  Every x_j
+// receives its updates in column order, as in the plain loop.  The first
+// non-positive pivot is recorded.
+template <typename T, bool FULL>
 __device__ __forceinline__ void panel16(Front64<T>& S, int b16, int cb, int n, int* info, int col0)
 {
     const int r = threadIdx.x & 63, row = b16 + r;
     const bool live = row < n;
-    T x[16], y[16];
+    T x[16], v[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
         x[j] = (live && (r >= 16 || j <= r)) ? S.A[b16 + j][row] : T(0);
-        y[j] = j == r ? T(1) : T(0);
+        v[j] = T(0);
     }
-    // Column k: d = x_k on lane k, L_kk = sqrt(d) (correctly rounded),
-    // l = x_k / L_kk (quot: exact whenever the quotient is representable),
-    // y_k /= L_kk as a scaling by rsqrt(d), then x_j -= l l_j, y_j -= y_k l_j
-    // for j > k.  The first non-positive pivot is recorded after the loop.
     int bad = 16;
-    if (cb == 16) {
-        // Column k's update of column k+1 is applied at once (the next pivot
-        // needs it); its updates of columns k+2.. are issued in step k+1,
-        // beside that step's pivot chain.  Every x_j receives its updates in
-        // column order, as in the plain loop.
-        T lp = T(0), yp = T(0);
+    T lp = T(0);
 #pragma unroll
-        for (int k = 0; k < 16; ++k) {
+    for (int k = 0; k < 16; ++k) {
+        if (FULL || k < cb) {  // no early exit: the loop stays unrolled, x and v in registers
             const T d = bcast(x[k], k);
             bad = (!(d > T(0)) && bad == 16) ? k : bad;
-            if (k > 0) {
+            const T rs = rsqrt_nr(d), sd = sqrt_from_rs(d, rs);
+            // column k-1's updates of columns k+1 .. 15 (its entries read in step k-1)
 #pragma unroll
-                for (int j = k + 1; j < 16; ++j) {
-                    const T lj = bcast(lp, j);
-                    x[j] -= lp * lj;
-                    y[j] -= yp * lj;
-                }
-            }
-            const T rs = rsqrt_nr(d), sd = dsqrt(d);
-            const T l = quot(x[k], sd, rs), y0 = y[k] * rs;
+            for (int j = k + 1; j < 16; ++j) x[j] -= lp * v[j];
+            const T l = quot(x[k], sd, rs);
             x[k] = r == k ? sd : l;
-            y[k] = y0;
-            if (k + 1 < 16) {
-                const T lj = bcast(l, k + 1);
-                x[k + 1] -= l * lj;
-                y[k + 1] -= y0 * lj;
-            }
+            if (live) S.A[b16 + k][row] = x[k];
+            if (r == 0) S.rs[b16 + k] = rs;
+            if (k + 1 < 16 && (FULL || k + 1 < cb)) x[k + 1] -= l * bcast(l, k + 1);
             lp = l;
-            yp = y0;
+            __builtin_amdgcn_wave_barrier();  // the reads below stay after the store above
+#pragma unroll
+            for (int j = k + 2; j < 16; ++j) v[j] = S.A[b16 + k][b16 + j];
+            if (!FULL && k + 1 == cb && k + 1 < 16) v[k + 1] = S.A[b16 + k][b16 + k + 1];
         }
-    } else {
+    }
+    if (!FULL) {
+        if (r == 0)
+            for (int k = cb; k < 16; ++k) S.rs[b16 + k] = T(1);  // finite stand-ins for the block's inverse
+        // the last column's updates of the block's columns past cb
 #pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            if (k < cb) {
-                const T d = bcast(x[k], k);
-                bad = (!(d > T(0)) && bad == 16) ? k : bad;
-                const T rs = rsqrt_nr(d), sd = dsqrt(d);
-                const T l = quot(x[k], sd, rs), y0 = y[k] * rs;
-                x[k] = r == k ? sd : l;
-                y[k] = y0;
+        for (int j = 0; j < 16; ++j)
+            if (j >= cb) x[j] -= lp * v[j];
+        if (live) {
 #pragma unroll
-                for (int j = k + 1; j < 16; ++j) {
-                    const T lj = bcast(l, j);
-                    x[j] -= l * lj;
-                    y[j] -= y0 * lj;
-                }
-            }
+            for (int j = 0; j < 16; ++j)
+                if (j >= cb) S.A[b16 + j][row] = x[j];
         }
     }
     if (bad < 16 && r == 0) atomicMin(info, col0 + b16 + bad + 1);
-    if (live) {
+}
+
+// Inverse of panel b16's 16 x 16 diagonal block of L (one wave): lane r < 16
+// solves L u = e_r by forward substitution, u = column r of the inverse,
+// stored as X[b16 + r][b16 + k] = u_k.  For a partial panel the rows and
+// columns past its cb pivots are finite stand-ins (rs = 1 there; the
+// entries of L in them are trailing-block values), which no stored
+// inverse entry uses: the inverse's row k < cb depends only on the rows
+// and columns <= k.
+template <typename T>
+__device__ __forceinline__ void diaginv16(Front64<T>& S, int b16)
+{
+    const int r = threadIdx.x & 63;
+    T y[16];
 #pragma unroll
-        for (int j = 0; j < 16; ++j) S.A[b16 + j][row] = x[j];
+    for (int j = 0; j < 16; ++j) y[j] = j == r ? T(1) : T(0);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const T y0 = y[k] * S.rs[b16 + k];
+        y[k] = y0;
+#pragma unroll
+        for (int j = k + 1; j < 16; ++j) y[j] -= y0 * S.A[b16 + k][b16 + j];
     }
     if (r < 16) {
 #pragma unroll
@@ -295,15 +322,24 @@ __device__ __forceinline__ void factor_dense64_loaded(Front64<T>& S, T* g, int64
 {
     const int tid = threadIdx.x, wv = tid >> 6;
     const int nblk = (c + 15) >> 4, nall = (n + 15) >> 4;
-    // Row i of the inverse's 16 x 16 blocks (X_ij, j < i) needs only the
-    // panels up to i and the rows above it, so row b - 1 is formed by waves
-    // 1.. while wave 0 runs panel b's chain; the last row follows the loop.
+    // Phase b: wave 0 runs panel b's chain; wave 1 inverts panel b-1's
+    // diagonal block; waves 2 and 3 form row b-2 of the inverse's
+    // off-diagonal blocks (it needs the diagonal inverses up to b-2 and the
+    // rows above it).  Then all four waves apply panel b to the trailing
+    // blocks.  The last diagonal inverse and the last two rows follow the
+    // loop.  Only the last panel can be partial.
     for (int b = 0; b < nblk; ++b) {
         const int b16 = 16 * b, cb = min(16, c - b16);
-        if (wv == 0)
-            panel16<T>(S, b16, cb, n, info, col0);
-        else if (b >= 2 && wv <= b - 1)
-            inverse16<T>(S, b - 1, wv - 1, wv);
+        if (wv == 0) {
+            if (cb == 16)
+                panel16<T, true>(S, b16, 16, n, info, col0);
+            else
+                panel16<T, false>(S, b16, cb, n, info, col0);
+        } else if (wv == 1) {
+            if (b >= 1) diaginv16<T>(S, b16 - 16);
+        } else if (wv - 2 < b - 2) {
+            inverse16<T>(S, b - 2, wv - 2, wv);
+        }
         __syncthreads();
         int idx = 0;
         for (int J = b + 1; J < nall; ++J)
@@ -311,7 +347,15 @@ __device__ __forceinline__ void factor_dense64_loaded(Front64<T>& S, T* g, int64
                 if ((idx & 3) == wv) trailing16<T>(S, b16, cb, I, J);
         __syncthreads();
     }
-    if (nblk >= 2 && wv < nblk - 1) inverse16<T>(S, nblk - 1, wv, wv);
+    const int last = nblk - 1;
+    if (last >= 0) {
+        if (wv == 1)
+            diaginv16<T>(S, 16 * last);
+        else if (wv >= 2 && wv - 2 < last - 1)
+            inverse16<T>(S, last - 1, wv - 2, wv);
+        __syncthreads();
+        if (wv < last) inverse16<T>(S, last, wv, wv);
+    }
     __syncthreads();
     for (int e = tid; e < 64 * 64; e += kThreads) {
         const int row = e & 63, col = e >> 6;
@@ -985,6 +1029,84 @@ __global__ __launch_bounds__(kThreads) void k_diag(T* pool, const NodeDev* nodes
     __shared__ Front64<T> S;
     factor_dense64<T>(S, pool + nd.front + t.k0 + (int64_t)t.k0 * nd.n, nd.n, t.kb, t.kb, info, nd.first + t.k0,
                       pool + nd.inv + (int64_t)(t.k0 / kPanel) * kPanel * kPanel);
+}
+
+// ---------------------------------------------------------- medium fronts
+// The whole partial factorization of one front per workgroup, for fronts
+// larger than one 64 x 64 LDS block whose 64-column steps would leave most
+// of the chip idle (many such fronts share a level): panel by panel,
+//   1. the kb x kb diagonal block factored and inverted in LDS
+//      (factor_dense64: DPOTRF, factorize_kernel_m.f90:4-9), its inverse
+//      stored for the solves;
+//   2. the rows below it solved against that inverse, in place, one 64-row
+//      MFMA tile at a time (the panel TRSM, factorize_kernel_m.f90:11-16);
+//   3. the trailing lower triangle -- pivot columns and update matrix --
+//      updated by the panel, one 64 x 64 MFMA tile at a time with K = kb
+//      (DSYRK / DGEMM, factorize_kernel_m.f90:18-33).
+// The workgroup owns its front, so workgroup barriers order the phases; no
+// tile of a phase reads what another tile of it writes.  LP rounds the
+// update tiles' operands to 16 bits (the FP16_IR / BF16_IR forms).  With
+// clocks != nullptr, task i's start and end (wall_clock64, 100 MHz) go to
+// clocks[2 i], clocks[2 i + 1]: the per-front timings of the schedule tuner.
+// One tile of k_front_wg, kept out of line: the medium-front kernel's
+// registers are then those of its diagonal phase (two waves per SIMD, the
+// call's saved registers on the stack) instead of the sum of both phases.
+template <typename T, int LP>
+__device__ __attribute__((noinline)) void front_tile(T* pool, GemmTask g, TileLds<T>& L, T (*Cs)[kTile + 1])
+{
+    gemm_task<T, LP>(pool, g, L, Cs);
+}
+template <typename T, int LP>
+__global__ __launch_bounds__(kThreads) void k_front_wg(T* pool, const NodeDev* nodes, const int* tasks, int* info,
+                                                       long long* clocks)
+{
+    long long t_start = 0;
+    if (clocks) t_start = wall_clock64();
+    const NodeDev nd = nodes[tasks[blockIdx.x]];
+    __shared__ union {
+        Front64<T> D;
+        struct {
+            TileLds<T> L;
+            T Cs[kTile][kTile + 1];
+        } G;
+    } U;
+    const int n = nd.n, c = nd.c;
+    for (int k0 = 0; k0 < c; k0 += kPanel) {
+        const int kb = min(kPanel, c - k0);
+        const int64_t inv = nd.inv + (int64_t)(k0 / kPanel) * kPanel * kPanel;
+        factor_dense64<T>(U.D, pool + nd.front + k0 + (int64_t)k0 * n, n, kb, kb, info, nd.first + k0, pool + inv);
+        __syncthreads();
+        for (int r0 = k0 + kb; r0 < n; r0 += kTile) {
+            GemmTask g{};
+            g.c = g.a = nd.front + r0 + (int64_t)k0 * n;
+            g.ldc = g.lda = n;
+            g.b = inv;  // L^{-T}, read transposed
+            g.ldb = kPanel;
+            g.m = min(kTile, n - r0);
+            g.n = g.k = kb;
+            g.mode = 1;
+            front_tile<T, kNative>(pool, g, U.G.L, U.G.Cs);
+            __syncthreads();
+        }
+        for (int c0 = k0 + kb; c0 < n; c0 += kTile)
+            for (int r0 = c0; r0 < n; r0 += kTile) {
+                GemmTask g{};
+                g.c = nd.front + r0 + (int64_t)c0 * n;
+                g.a = nd.front + r0 + (int64_t)k0 * n;
+                g.b = nd.front + c0 + (int64_t)k0 * n;
+                g.ldc = g.lda = g.ldb = n;
+                g.m = min(kTile, n - r0);
+                g.n = min(kTile, n - c0);
+                g.k = kb;
+                g.tri = r0 == c0;
+                front_tile<T, LP>(pool, g, U.G.L, U.G.Cs);
+                __syncthreads();
+            }
+    }
+    if (clocks && threadIdx.x == 0) {
+        clocks[2 * blockIdx.x] = t_start;
+        clocks[2 * blockIdx.x + 1] = wall_clock64();
+    }
 }
 
 // ------------------------------------------------------------------ solve
@@ -1663,6 +1785,24 @@ void launch_small_front(T* pool, const NodeDev* nodes, const int* tasks, int cou
     if (count > 0) hipLaunchKernelGGL(k_small_front<T>, dim3(count), dim3(kThreads), 0, st, pool, nodes, tasks, info);
 }
 template <typename T>
+void launch_front_wg(T* pool, const NodeDev* nodes, const int* tasks, int count, int* info, long long* clocks,
+                     hipStream_t st, int lowp)
+{
+    if (count <= 0) return;
+    if constexpr (sizeof(T) == 4) {
+        if (lowp == kF16) {
+            hipLaunchKernelGGL((k_front_wg<T, kF16>), dim3(count), dim3(kThreads), 0, st, pool, nodes, tasks, info, clocks);
+            return;
+        }
+        if (lowp == kBF16) {
+            hipLaunchKernelGGL((k_front_wg<T, kBF16>), dim3(count), dim3(kThreads), 0, st, pool, nodes, tasks, info,
+                               clocks);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_front_wg<T, kNative>), dim3(count), dim3(kThreads), 0, st, pool, nodes, tasks, info, clocks);
+}
+template <typename T>
 void launch_diag(T* pool, const NodeDev* nodes, const DiagTask* tasks, int count, int* info, hipStream_t st)
 {
     if (count > 0) hipLaunchKernelGGL(k_diag<T>, dim3(count), dim3(kThreads), 0, st, pool, nodes, tasks, info);
@@ -1777,6 +1917,7 @@ void launch_scatter_rows(double* out, int ld_out, const double* in, int ld_in, c
     template void launch_scatter_a<T>(T*, const double*, const int64_t*, const int64_t*, int64_t, hipStream_t);        \
     template void launch_extend_add<T>(T*, const int*, const NodeDev*, const ExtTask*, int, hipStream_t);              \
     template void launch_small_front<T>(T*, const NodeDev*, const int*, int, int*, hipStream_t);                       \
+    template void launch_front_wg<T>(T*, const NodeDev*, const int*, int, int*, long long*, hipStream_t, int);           \
     template void launch_diag<T>(T*, const NodeDev*, const DiagTask*, int, int*, hipStream_t);                        \
     template void launch_upd_diag<T>(T*, const NodeDev*, const DiagTask*, int, const GemmTask*, int, int*, hipStream_t); \
     template void launch_gemm<T>(T*, const GemmTask*, int, int, const int*, hipStream_t, int);                         \

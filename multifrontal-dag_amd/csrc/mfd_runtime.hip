@@ -101,6 +101,12 @@ double now_ms()
 struct mfd_handle {
     int nb = 0, flags = 0;
     int upd_period = kUpdPeriod;
+    // Medium fronts (order > 64) factored whole by one workgroup each
+    // (k_front_wg): those with at most fuse_cmax pivot columns, unless
+    // front_kind (one entry per node, when set) says otherwise: 1 = one
+    // workgroup, 0 = 64-column steps over the chip, -1 = by fuse_cmax.
+    int fuse_cmax = 0;
+    std::vector<int> front_kind;
     bool device_ok = false;
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;  // large in-panel updates beside the diagonal chain
@@ -132,6 +138,8 @@ struct mfd_handle {
     std::vector<NodeDev> nodes;
     std::vector<ExtTask> ext;
     std::vector<int> small;
+    std::vector<int> fused;             // medium fronts factored whole by one workgroup
+    std::vector<int> fused_pos;         // node -> position in fused, or -1
     std::vector<DiagTask> diag;         // diagonal-block factorizations of large fronts
     std::vector<int2> fill;             // (node, first column) blocks of the pool's lower triangles
     std::vector<GemmTask> gemm;
@@ -153,6 +161,8 @@ struct mfd_handle {
     int* d_lidx = nullptr;
     ExtTask* d_ext = nullptr;
     int* d_small = nullptr;
+    int* d_fused = nullptr;
+    long long* d_clocks = nullptr;  // start/end clock of each medium front's workgroup, in fused's order
     DiagTask* d_diag = nullptr;
     int2* d_fill = nullptr;
     GemmTask* d_gemm = nullptr;
@@ -247,6 +257,8 @@ void free_device(mfd_handle* h)
     h->d_lidx = nullptr;
     h->d_ext = nullptr;
     h->d_small = nullptr;
+    h->d_fused = nullptr;
+    h->d_clocks = nullptr;
     h->d_diag = nullptr;
     h->d_fill = nullptr;
     h->d_gemm = nullptr;
@@ -497,6 +509,22 @@ void build_plan(mfd_handle* h)
 
     h->ext.clear();
     h->small.clear();
+    h->fused.clear();
+    h->fused_pos.assign(m, -1);
+    // front classes: small (n <= 64, whole in LDS), medium (one workgroup
+    // per front), large (64-column steps over the chip)
+    enum { kClsSmall = 0, kClsMedium = 1, kClsLarge = 2 };
+    std::vector<int> cls(m, kClsLarge);
+    for (int s = 0; s < m; ++s) {
+        const NodeDev& d = h->nodes[s];
+        if (d.n <= kSmallFront) {
+            cls[s] = kClsSmall;
+            continue;
+        }
+        const int k = (int)h->front_kind.size() == m ? h->front_kind[s] : -1;
+        const bool medium = k >= 0 ? k == 1 : d.c <= h->fuse_cmax;
+        cls[s] = medium ? kClsMedium : kClsLarge;
+    }
     h->diag.clear();
     h->gemm.clear();
     h->fseq.clear();
@@ -520,7 +548,7 @@ void build_plan(mfd_handle* h)
                 const auto& chs = S.children[lv[k]];
                 if (chs.size() <= r) continue;
                 int ch = chs[r] - 1, nt = (S.w[ch] + kTile - 1) / kTile;
-                if (h->nodes[ch].n > kSmallFront) continue;
+                if (cls[ch] == kClsLarge) continue;
                 for (int ti = 0; ti < nt; ++ti)
                     for (int tj = 0; tj <= ti; ++tj) h->ext.push_back(ExtTask{ch, ti, tj, 0});
             }
@@ -530,11 +558,18 @@ void build_plan(mfd_handle* h)
         int64_t first = (int64_t)h->small.size();
         std::vector<int> large;
         for (int k = 0; k < cnt; ++k) {
-            const NodeDev& d = h->nodes[lv[k]];
-            if (d.n <= kSmallFront) h->small.push_back(lv[k]);
-            else large.push_back(lv[k]);
+            if (cls[lv[k]] == kClsSmall) h->small.push_back(lv[k]);
+            else if (cls[lv[k]] == kClsLarge) large.push_back(lv[k]);
         }
         push(MFD_K_SMALL_FRONT, first, (int64_t)h->small.size());
+        // medium fronts: one workgroup each, the whole partial factorization
+        first = (int64_t)h->fused.size();
+        for (int k = 0; k < cnt; ++k)
+            if (cls[lv[k]] == kClsMedium) {
+                h->fused_pos[lv[k]] = (int)h->fused.size();
+                h->fused.push_back(lv[k]);
+            }
+        push(MFD_K_FRONT, first, (int64_t)h->fused.size());
         // large fronts: the first diagonal block, then one launch per
         // 64-column step t (k_upd_diag) holding
         //  * the TRSM of panel t for the rows after block t+1, each tile
@@ -741,6 +776,8 @@ int setup_device(mfd_handle* h)
     MFD_TRY(upload(h, &h->d_lidx, S.lidx));
     MFD_TRY(upload(h, &h->d_ext, h->ext));
     MFD_TRY(upload(h, &h->d_small, h->small));
+    MFD_TRY(upload(h, &h->d_fused, h->fused));
+    MFD_TRY(dalloc(h, &h->d_clocks, 2 * (int64_t)h->fused.size()));
     MFD_TRY(upload(h, &h->d_diag, h->diag));
     MFD_TRY(upload(h, &h->d_fill, h->fill));
     while (h->fill_ev.size() < h->group_level.size()) {
@@ -834,6 +871,10 @@ template <typename T> void enqueue_launch(mfd_handle* h, const Launch& l, hipStr
     switch (l.kind) {
     case MFD_K_EXTEND_ADD: launch_extend_add<T>(pool, h->d_lidx, h->d_nodes, h->d_ext + l.first, l.count, st); break;
     case MFD_K_SMALL_FRONT: launch_small_front<T>(pool, h->d_nodes, h->d_small + l.first, l.count, h->d_info, st); break;
+    case MFD_K_FRONT:
+        launch_front_wg<T>(pool, h->d_nodes, h->d_fused + l.first, l.count, h->d_info, h->d_clocks + 2 * l.first, st,
+                           sizeof(T) == 4 ? h->lowp : 0);
+        break;
     case MFD_K_POTRF:
         if (l.fused)
             launch_upd_diag<T>(pool, h->d_nodes, h->d_diag + l.dfirst, l.dcount, h->d_gemm + l.first, l.count, h->d_info,
@@ -1343,6 +1384,7 @@ struct NoAllocScope {
 
 int finish_analysis(mfd_handle* h)
 {
+    h->front_kind.clear();  // per-node choices belong to the previous structure
     delete h->layout;
     h->layout = new RefLayout(h->S.c, h->S.w, h->nb);
     h->analyzed = true;
@@ -1476,13 +1518,12 @@ int mfd_set_structure(mfd_handle* h, int num_node, const int* c, const int* w, c
     return finish_analysis(h);
 }
 
-int mfd_set_update_period(mfd_handle* h, int period)
+namespace {
+// A schedule setting changed: rebuild the schedule of the current analysis
+// (its factor is gone).
+int replan(mfd_handle* h)
 {
-    if (!h || period < 0 || period > kMaxPeriod) return MFD_ERR_ARG;
-    if (period == h->upd_period) return MFD_OK;
-    h->upd_period = period;
     if (!h->analyzed) return MFD_OK;
-    // the schedule of the current analysis is rebuilt; its factor is gone
     h->factored = false;
     if (!h->device_ok) return MFD_OK;
     int rc = setup_device(h);
@@ -1492,8 +1533,61 @@ int mfd_set_update_period(mfd_handle* h, int period)
     }
     return rc;
 }
+}  // namespace
+
+int mfd_set_update_period(mfd_handle* h, int period)
+{
+    if (!h || period < 0 || period > kMaxPeriod) return MFD_ERR_ARG;
+    if (period == h->upd_period) return MFD_OK;
+    h->upd_period = period;
+    return replan(h);
+}
 
 int mfd_get_update_period(mfd_handle* h) { return h ? h->upd_period : MFD_ERR_ARG; }
+
+
+int mfd_set_front_split(mfd_handle* h, int cmax)
+{
+    if (!h || cmax < 0) return MFD_ERR_ARG;
+    if (cmax == h->fuse_cmax) return MFD_OK;
+    h->fuse_cmax = cmax;
+    return replan(h);
+}
+
+int mfd_set_front_kinds(mfd_handle* h, const int* kinds)
+{
+    if (!h) return MFD_ERR_ARG;
+    if (!kinds) {
+        if (h->front_kind.empty()) return MFD_OK;
+        h->front_kind.clear();
+        return replan(h);
+    }
+    if (!h->analyzed) return MFD_ERR_STATE;
+    const int m = h->S.num_node;
+    for (int s = 0; s < m; ++s)
+        if (kinds[s] < -1 || kinds[s] > 1) return MFD_ERR_ARG;
+    std::vector<int> k(kinds, kinds + m);
+    if (k == h->front_kind) return MFD_OK;
+    h->front_kind = std::move(k);
+    return replan(h);
+}
+
+int mfd_get_front_times(mfd_handle* h, double* us)
+{
+    if (!h || !us) return MFD_ERR_ARG;
+    if (!h->analyzed) return MFD_ERR_STATE;
+    if (!h->device_ok) return MFD_ERR_NODEVICE;
+    const int m = h->S.num_node;
+    for (int s = 0; s < m; ++s) us[s] = -1.0;
+    if (h->fused.empty()) return MFD_OK;
+    int khz = 0;
+    HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0));
+    if (khz <= 0) return MFD_ERR_UNSUPPORTED;
+    std::vector<long long> c(2 * h->fused.size());
+    HIP_TRY(hipMemcpy(c.data(), h->d_clocks, c.size() * sizeof(long long), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < h->fused.size(); ++i) us[h->fused[i]] = (double)(c[2 * i + 1] - c[2 * i]) * 1e3 / khz;
+    return MFD_OK;
+}
 
 int mfd_factorize(mfd_handle* h, const double* a, int precision)
 {
@@ -1718,13 +1812,22 @@ int mfd_get_kernel_work(mfd_handle* h, int k, int s, double* flops, double* byte
         b = s * (double)h->pool_elems + nnz * (8 + 8 + 8 + s);
         break;
     }
-    case MFD_K_EXTEND_ADD:  // small children (large ones add in their Schur launch)
-        for (const auto& d : h->nodes)
-            if (d.parent >= 0 && d.n <= kSmallFront) {
+    case MFD_K_EXTEND_ADD:  // small and medium children (large ones add in their Schur launch)
+        for (size_t q = 0; q < h->nodes.size(); ++q) {
+            const NodeDev& d = h->nodes[q];
+            if (d.parent >= 0 && (d.n <= kSmallFront || h->fused_pos[q] >= 0)) {
                 double e = tri(d.w);
                 f += e;
                 b += e * 3 * s + 4.0 * d.w;
             }
+        }
+        break;
+    case MFD_K_FRONT:  // the cost formula's flops of each medium front; its lower triangle read and written
+        for (int s0 : h->fused) {
+            const NodeDev& d = h->nodes[s0];
+            for (int64_t m = d.w + 1; m <= d.n; ++m) f += (double)m * m;
+            b += 2.0 * s * tri(d.n);
+        }
         break;
     case MFD_K_SMALL_FRONT:
         for (int s0 : h->small) {

@@ -30,7 +30,7 @@ FLAG_GRAPH = 1
 FLAG_PROFILE = 2
 FLAG_ORDER_ND = 4
 KERNEL_CLASSES = ["assemble", "extend_add", "small_front", "potrf", "trsm", "update", "schur", "solve_fwd",
-                  "solve_bwd", "refine"]
+                  "solve_bwd", "refine", "front"]
 
 _lib = None
 
@@ -95,6 +95,9 @@ def lib():
             "mfd_mm_read": (i, [C.c_char_p, i, i64, ip, ip, dp]),
             "mfd_set_update_period": (i, [P, i]),
             "mfd_get_update_period": (i, [P]),
+            "mfd_set_front_split": (i, [P, i]),
+            "mfd_set_front_kinds": (i, [P, ip]),
+            "mfd_get_front_times": (i, [P, dp]),
             "mfd_mlp_predict": (i, [i, dp, dp, dp]),
             "mfd_mlp_gradient": (i, [i, dp, dp, dp, dp, dp]),
             "mfd_mlp_train": (i, [i, dp, dp, dp, i, d, dp]),
@@ -250,6 +253,27 @@ class Solver:
     @update_period.setter
     def update_period(self, period):
         self._check(lib().mfd_set_update_period(self.h, int(period)), "mfd_set_update_period")
+
+    def set_front_split(self, cmax):
+        """Fronts of order > 64 with at most cmax pivot columns are factored
+        whole by one workgroup each (0: none)."""
+        self._check(lib().mfd_set_front_split(self.h, int(cmax)), "mfd_set_front_split")
+
+    def set_front_kinds(self, kinds):
+        """Per node (0-based order): 1 one workgroup, 0 64-column steps,
+        -1 by the split; None clears."""
+        if kinds is None:
+            self._check(lib().mfd_set_front_kinds(self.h, None), "mfd_set_front_kinds")
+            return
+        k = _i32(kinds)
+        self._check(lib().mfd_set_front_kinds(self.h, _ptr(k, C.c_int)), "mfd_set_front_kinds")
+
+    def front_times(self):
+        """Per node: microseconds of its workgroup in the last factorization
+        (medium fronts; -1 for the others)."""
+        out = np.empty(self.num_node, dtype=np.float64)
+        self._check(lib().mfd_get_front_times(self.h, _ptr(out, C.c_double)), "mfd_get_front_times")
+        return out
 
     # -------------------------------------------------------------- analysis
     def analyze(self, ccs_col, ccs_row, max_zero=0, perm=None):
