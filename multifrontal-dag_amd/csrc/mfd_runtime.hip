@@ -123,7 +123,13 @@ struct mfd_handle {
     std::vector<int64_t> scat_gptr;
     hipError_t last_hip = hipSuccess;
     int64_t mem_limit = 0, device_bytes = 0, front_bytes = 0;
-    std::vector<void*> allocs;
+    // Device buffers in use (pointer, bytes) and released ones kept for
+    // reuse.  A buffer that a captured hipGraph's kernels used is not given
+    // back to the device by hipFree after the graph is destroyed (measured:
+    // every schedule rebuild lost the front pool), so buffers are freed only
+    // by mfd_destroy and otherwise reused by later allocations of the handle.
+    // device_bytes counts both.
+    std::vector<std::pair<void*, int64_t>> allocs, spare;
 
     bool analyzed = false, factored = false, from_parts = false;
     int precision = MFD_FP64;
@@ -219,13 +225,25 @@ template <typename T> int dalloc(mfd_handle* h, T** p, int64_t n)
 {
     if (h->no_alloc) return MFD_ERR_SCRATCH;
     size_t bytes = (size_t)std::max<int64_t>(n, 1) * sizeof(T);
+    // the smallest released buffer that holds it and is at most twice its size
+    int best = -1;
+    for (size_t i = 0; i < h->spare.size(); ++i) {
+        const int64_t b = h->spare[i].second;
+        if (b >= (int64_t)bytes && b <= 2 * (int64_t)bytes && (best < 0 || b < h->spare[best].second)) best = (int)i;
+    }
+    if (best >= 0) {
+        *p = static_cast<T*>(h->spare[best].first);
+        h->allocs.push_back(h->spare[best]);
+        h->spare.erase(h->spare.begin() + best);
+        return MFD_OK;
+    }
     if (h->mem_limit > 0 && h->device_bytes + (int64_t)bytes > h->mem_limit) return MFD_ERR_NOMEM;
     if (hipMalloc((void**)p, bytes) != hipSuccess) {
         (void)hipGetLastError();
         return MFD_ERR_NOMEM;
     }
     h->device_bytes += (int64_t)bytes;
-    h->allocs.push_back((void*)*p);
+    h->allocs.push_back({(void*)*p, (int64_t)bytes});
     return MFD_OK;
 }
 
@@ -250,9 +268,9 @@ void free_device(mfd_handle* h)
             g = nullptr;
         }
     drop_solve_graphs(h);
-    for (void* p : h->allocs) (void)hipFree(p);
+    // every call synchronizes before it returns, so no kernel uses them now
+    h->spare.insert(h->spare.end(), h->allocs.begin(), h->allocs.end());
     h->allocs.clear();
-    h->device_bytes = 0;
     h->d_nodes = nullptr;
     h->d_lidx = nullptr;
     h->d_ext = nullptr;
@@ -1436,6 +1454,9 @@ void mfd_destroy(mfd_handle* h)
     if (h->device_ok) {
         (void)hipStreamSynchronize(h->stream);
         free_device(h);
+        for (auto& b : h->spare) (void)hipFree(b.first);
+        h->spare.clear();
+        h->device_bytes = 0;
         for (auto e : h->ev)
             if (e) (void)hipEventDestroy(e);
         if (h->fork) (void)hipEventDestroy(h->fork);
