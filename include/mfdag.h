@@ -290,6 +290,20 @@ int mfd_set_front_split(mfd_handle* h, int cmax);
 int mfd_set_front_kinds(mfd_handle* h, const int* kinds);
 int mfd_get_front_times(mfd_handle* h, double* us);
 
+/* Static check of the factorization schedule (no counterpart in the
+   reference, whose StarPU runtime orders tasks by their data accesses):
+   the launches of the current analysis on their streams (main, zeroing and
+   assembly, large in-panel updates) with their event edges as a DAG, and
+   every task's accesses to front entries, stored diagonal-block inverses
+   and parent entries.  *conflicts = the number of task pairs that touch a
+   common entry, one of them writing, without an order: in one launch, or in
+   two launches no path of the DAG orders.  Variant 0 checks the schedule as
+   built; variant 1 a model of a schedule the library does not use (each
+   front's Schur tiles on a fourth stream forked after the launch with the
+   front's last diagonal task).  report (nullable) receives the first few
+   pairs.  Runs on the host; works without a GPU. */
+int mfd_check_schedule(mfd_handle* h, int variant, int64_t* conflicts, char* report, int report_len);
+
 /* The tuner's cost model (no counterpart in the reference): a perceptron
    with MFD_MLP_INPUTS inputs, MFD_MLP_HIDDEN tanh units and one linear
    output, trained and evaluated by HIP kernels in FP64.  params

@@ -19,6 +19,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <set>
+#include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "mfd_analysis.h"
@@ -917,8 +920,14 @@ template <typename T> void enqueue_launch(mfd_handle* h, const Launch& l, hipStr
 // The factorization schedule: reset the pivot flag, zero the front pool,
 // scatter A, then every level's launches.  No hipMemsetAsync, so the
 // sequence can be captured into a graph.
-template <typename T> void enqueue_factor(mfd_handle* h, bool profile)
+// Returns the first error of the event and stream-wait calls that order the
+// streams (a failed one would leave two launches unordered).
+template <typename T> hipError_t enqueue_factor(mfd_handle* h, bool profile)
 {
+    hipError_t err = hipSuccess;
+    auto ok = [&](hipError_t e) {
+        if (e != hipSuccess && err == hipSuccess) err = e;
+    };
     hipStream_t st = h->stream;
     T* pool = pool_of<T>(h);
     auto mark = [&](int k) {
@@ -938,8 +947,8 @@ template <typename T> void enqueue_factor(mfd_handle* h, bool profile)
     const bool split_fill = !profile && h->fillst && (int)h->fill_ev.size() >= G;
     hipStream_t fst = split_fill ? h->fillst : st;
     if (split_fill) {
-        (void)hipEventRecord(h->fork, st);
-        (void)hipStreamWaitEvent(fst, h->fork, 0);
+        ok(hipEventRecord(h->fork, st));
+        ok(hipStreamWaitEvent(fst, h->fork, 0));
     }
     const bool scaled = sizeof(T) == 4 && h->lowp != 0;
     if (scaled) launch_diag_scale(h->d_dscale, h->d_a, h->d_diag_src, h->S.n, fst);
@@ -951,7 +960,7 @@ template <typename T> void enqueue_factor(mfd_handle* h, bool profile)
                                        fst);
         else
             launch_scatter_a<T>(pool, h->d_a, h->d_dst + e0, h->d_src + e0, ne, fst);
-        if (split_fill) (void)hipEventRecord(h->fill_ev[g], fst);
+        if (split_fill) ok(hipEventRecord(h->fill_ev[g], fst));
     }
     mark(0);
     // Side launches fork from the main stream's current position and run
@@ -962,27 +971,28 @@ template <typename T> void enqueue_factor(mfd_handle* h, bool profile)
     for (size_t i = 0; i < h->fseq.size(); ++i) {
         const Launch& l = h->fseq[i];
         for (; gnext < G && h->level_launch[h->group_level[gnext]] <= (int)i; ++gnext)
-            (void)hipStreamWaitEvent(st, h->fill_ev[gnext], 0);
+            ok(hipStreamWaitEvent(st, h->fill_ev[gnext], 0));
         if (two && l.side) {
-            (void)hipEventRecord(h->fork, st);
-            (void)hipStreamWaitEvent(h->side, h->fork, 0);
+            ok(hipEventRecord(h->fork, st));
+            ok(hipStreamWaitEvent(h->side, h->fork, 0));
             enqueue_launch<T>(h, l, h->side);
-            (void)hipEventRecord(h->sev[l.side_id & 1], h->side);
+            ok(hipEventRecord(h->sev[l.side_id & 1], h->side));
             last_side = l.side_id;
             continue;
         }
         if (two && l.kind != MFD_K_POTRF && l.kind != MFD_K_TRSM && last_side > joined) {
-            (void)hipStreamWaitEvent(st, h->sev[last_side & 1], 0);
+            ok(hipStreamWaitEvent(st, h->sev[last_side & 1], 0));
             joined = last_side;
         } else if (two && l.wait_id > joined) {
-            (void)hipStreamWaitEvent(st, h->sev[l.wait_id & 1], 0);
+            ok(hipStreamWaitEvent(st, h->sev[l.wait_id & 1], 0));
             joined = l.wait_id;
         }
         enqueue_launch<T>(h, l, st);
         mark(l.kind);
     }
-    if (two && last_side > joined) (void)hipStreamWaitEvent(st, h->sev[last_side & 1], 0);
-    for (; gnext < G; ++gnext) (void)hipStreamWaitEvent(st, h->fill_ev[gnext], 0);
+    if (two && last_side > joined) ok(hipStreamWaitEvent(st, h->sev[last_side & 1], 0));
+    for (; gnext < G; ++gnext) ok(hipStreamWaitEvent(st, h->fill_ev[gnext], 0));
+    return err;
 }
 
 void collect_profile(mfd_handle* h, const std::vector<int>& kinds)
@@ -1012,8 +1022,12 @@ template <typename T> int run_factor(mfd_handle* h)
         if (!h->graph[gi]) {
             hipGraph_t g;
             HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-            enqueue_factor<T>(h, false);
+            const hipError_t qe = enqueue_factor<T>(h, false);
             HIP_TRY(hipStreamEndCapture(h->stream, &g));
+            if (qe != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                HIP_TRY(qe);
+            }
             HIP_TRY(hipGraphInstantiate(&h->graph[gi], g, nullptr, nullptr, 0));
             (void)hipGraphDestroy(g);
         }
@@ -1021,7 +1035,7 @@ template <typename T> int run_factor(mfd_handle* h)
     } else {
         if (profile)
             for (int k = 0; k < MFD_K_NUM; ++k) h->ktime[k] = 0, h->kcount[k] = 0;
-        enqueue_factor<T>(h, profile);
+        HIP_TRY(enqueue_factor<T>(h, profile));
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(h->ev[1], h->stream));
@@ -1418,6 +1432,336 @@ int finish_analysis(mfd_handle* h)
 
 }  // namespace
 
+// ------------------------------------------------ static schedule check
+// The factorization schedule as enqueue_factor issues it on the graph
+// path -- main stream, fill stream (zeroing and assembly by level groups),
+// side stream (large in-panel updates) and their event edges -- turned into
+// a DAG of launches, and every task's accesses: rectangles of fronts, the
+// inverses of their 64 x 64 diagonal blocks, and the parent entries an
+// extend-add adds into (the bounding rectangle of the local_index images).
+// The check: no two tasks of one launch touch one entry where either
+// writes, and every such pair in two launches is ordered by the DAG.
+// Variant 1 models a schedule that is not built: the Schur tiles of the
+// fronts whose chain ends before their level's last step issued on a fourth
+// stream forked after the launch with the front's last diagonal task and
+// joined at the level's end.
+namespace {
+
+struct Access {
+    int region;  // front s, or num_node + s for its diagonal-block inverses (rows = block index)
+    int r0, r1, c0, c1;  // [r0, r1) x [c0, c1)
+    int node;            // DAG node of the launch
+    int64_t task;        // global task id (unique across launches)
+    bool write;
+    int kind;            // MFD_K_* of the launch, -1 zeroing/assembly, -2 a Schur tile of variant 1's fourth stream
+};
+
+struct ScheduleCheck {
+    const mfd_handle* h;
+    int m = 0;
+    std::vector<std::vector<int>> pred;  // DAG: predecessors of each node
+    std::vector<Access> acc;
+    std::vector<std::pair<int64_t, int>> starts;  // pool offset -> region (fronts, then inverses)
+    int64_t next_task = 0;
+    int kind = 0;  // kind recorded with the accesses being added
+
+    int add_node() {
+        pred.emplace_back();
+        return (int)pred.size() - 1;
+    }
+    void edge(int a, int b) {
+        if (a >= 0 && b >= 0 && a != b) pred[b].push_back(a);
+    }
+    void rect(int region, int r0, int r1, int c0, int c1, int node, int64_t task, bool write) {
+        if (r1 > r0 && c1 > c0) acc.push_back(Access{region, r0, r1, c0, c1, node, task, write, kind});
+    }
+    // region and (row, col) of a pool offset
+    void locate(int64_t off, int& region, int& row, int& col) const {
+        auto it = std::upper_bound(starts.begin(), starts.end(), std::make_pair(off, INT_MAX));
+        --it;
+        region = it->second;
+        const int64_t rel = off - it->first;
+        if (region < m) {
+            const int n = h->nodes[region].n;
+            row = (int)(rel % n);
+            col = (int)(rel / n);
+        } else {
+            row = (int)(rel / (kPanel * kPanel));
+            col = 0;
+        }
+    }
+    void front_all(int s, int node, int64_t task) {
+        const NodeDev& d = h->nodes[s];
+        rect(s, 0, d.n, 0, d.n, node, task, true);
+        rect(m + s, 0, (d.c + kPanel - 1) / kPanel, 0, 1, node, task, true);
+    }
+    // parent entries of child ch's update rows [i0, i1) x columns [j0, j1)
+    void parent_rect(int ch, int i0, int i1, int j0, int j1, int node, int64_t task) {
+        const NodeDev& d = h->nodes[ch];
+        if (d.parent < 0 || i1 <= i0 || j1 <= j0) return;
+        const int* li = h->S.lidx.data() + d.lidx;
+        rect(d.parent, li[i0] - 1, li[i1 - 1], li[j0] - 1, li[j1 - 1], node, task, true);
+    }
+    void gemm(const GemmTask& g, int node, int64_t task) {
+        int rc, r, c;
+        locate(g.c, rc, r, c);
+        rect(rc, r, r + g.m, c, c + g.n, node, task, true);
+        int ra, ar, ac;
+        locate(g.a, ra, ar, ac);
+        rect(ra, ar, ar + g.m, ac, ac + g.k, node, task, false);
+        int rb, br, bc;
+        locate(g.b, rb, br, bc);
+        if (rb >= m)
+            rect(rb, br, br + 1, 0, 1, node, task, false);  // a stored inverse block
+        else
+            rect(rb, br, br + g.n, bc, bc + g.k, node, task, false);
+        if (g.mode == 2) {
+            int rd, dr, dc;
+            locate(g.d, rd, dr, dc);
+            rect(rd, dr, dr + 1, 0, 1, node, task, false);
+        }
+        if (g.mode == 3) {
+            const NodeDev& d = h->nodes[rc];
+            parent_rect(rc, r - d.c, r - d.c + g.m, c - d.c, c - d.c + g.n, node, task);
+        }
+    }
+    void diag(const DiagTask& t, int node, int64_t task) {
+        const int s = t.node, k0 = t.k0, kb = t.kb, kp = t.kprev, kpp = t.kpp;
+        rect(s, k0, k0 + kb, k0, k0 + kb, node, task, true);
+        rect(m + s, k0 / kPanel, k0 / kPanel + 1, 0, 1, node, task, true);
+        if (kp > 0) {
+            rect(s, k0, k0 + kb, k0 - kp, k0, node, task, true);
+            rect(m + s, (k0 - kp) / kPanel, (k0 - kp) / kPanel + 1, 0, 1, node, task, false);
+        }
+        if (kpp > 0) {
+            rect(s, k0, k0 + kb, k0 - kp - kpp, k0 - kp, node, task, false);
+            rect(s, k0 - kp, k0, k0 - kp - kpp, k0 - kp, node, task, false);
+        }
+    }
+    void launch(const Launch& l, int node) {
+        kind = l.kind;
+        switch (l.kind) {
+        case MFD_K_EXTEND_ADD:
+            for (int64_t i = l.first; i < l.first + l.count; ++i) {
+                const ExtTask& t = h->ext[i];
+                const NodeDev& d = h->nodes[t.child];
+                const int64_t task = next_task++;
+                const int i0 = t.ti * kTile, i1 = std::min(d.w, i0 + kTile);
+                const int j0 = t.tj * kTile, j1 = std::min(d.w, j0 + kTile);
+                rect(t.child, d.c + i0, d.c + i1, d.c + j0, d.c + j1, node, task, false);
+                parent_rect(t.child, i0, i1, j0, j1, node, task);
+            }
+            break;
+        case MFD_K_SMALL_FRONT:
+            for (int64_t i = l.first; i < l.first + l.count; ++i) front_all(h->small[i], node, next_task++);
+            break;
+        case MFD_K_FRONT:
+            for (int64_t i = l.first; i < l.first + l.count; ++i) front_all(h->fused[i], node, next_task++);
+            break;
+        case MFD_K_POTRF: {
+            const int64_t d0 = l.fused ? l.dfirst : l.first, dn = l.fused ? l.dcount : l.count;
+            for (int64_t i = d0; i < d0 + dn; ++i) diag(h->diag[i], node, next_task++);
+            if (l.fused)
+                for (int64_t i = l.first; i < l.first + l.count; ++i) gemm(h->gemm[i], node, next_task++);
+            break;
+        }
+        default:
+            for (int64_t i = l.first; i < l.first + l.count; ++i) {
+                const GemmTask& g = h->gemm[i];
+                gemm(g, node, next_task++);
+            }
+            break;
+        }
+    }
+};
+
+// Reachability over the DAG (nodes in enqueue order: predecessors first).
+struct Reach {
+    int n, words;
+    std::vector<uint64_t> bits;
+    explicit Reach(const std::vector<std::vector<int>>& pred) : n((int)pred.size()), words(((int)pred.size() + 63) / 64) {
+        bits.assign((size_t)n * words, 0);
+        for (int v = 0; v < n; ++v)
+            for (int u : pred[v]) {
+                uint64_t* bv = &bits[(size_t)v * words];
+                const uint64_t* bu = &bits[(size_t)u * words];
+                for (int w = 0; w < words; ++w) bv[w] |= bu[w];
+                bv[u / 64] |= uint64_t(1) << (u % 64);
+            }
+    }
+    bool before(int a, int b) const { return (bits[(size_t)b * words + a / 64] >> (a % 64)) & 1; }
+};
+
+int64_t check_schedule(mfd_handle* h, int variant, std::string& report)
+{
+    ScheduleCheck C;
+    C.h = h;
+    C.m = (int)h->nodes.size();
+    const int m = C.m;
+    for (int s = 0; s < m; ++s) C.starts.push_back({h->nodes[s].front, s});
+    for (int s = 0; s < m; ++s)
+        if (h->nodes[s].c > 0) C.starts.push_back({h->nodes[s].inv, m + s});
+    std::sort(C.starts.begin(), C.starts.end());
+    // streams as enqueue_factor issues them (graph path: fill and side streams present)
+    const int init = C.add_node();
+    int main_last = init;
+    const int G = (int)h->group_level.size();
+    std::vector<int> fill_node(G);
+    int fill_last = init;  // the fill stream forks after the info reset
+    for (int g = 0; g < G; ++g) {
+        const int v = C.add_node();
+        C.edge(fill_last, v);
+        fill_last = v;
+        fill_node[g] = v;
+        C.kind = -1;
+        for (int s = 0; s < m; ++s)
+            if (h->front_group[s] == g) C.rect(s, 0, h->nodes[s].n, 0, h->nodes[s].n, v, C.next_task++, true);
+    }
+    // variants 1 and 2: per large front, the launch the fourth stream forks
+    // after -- the one with its last diagonal task (1), or the last one with
+    // any of its step tasks, TRSM and in-panel update tiles included (2)
+    std::vector<int> last_diag_launch(m, -1);
+    if (variant >= 1)
+        for (size_t i = 0; i < h->fseq.size(); ++i) {
+            const Launch& l = h->fseq[i];
+            if (l.kind == MFD_K_POTRF) {
+                const int64_t d0 = l.fused ? l.dfirst : l.first, dn = l.fused ? l.dcount : l.count;
+                for (int64_t j = d0; j < d0 + dn; ++j) last_diag_launch[h->diag[j].node] = (int)i;
+            }
+            if (variant == 2 && (l.kind == MFD_K_TRSM || l.kind == MFD_K_UPDATE || (l.kind == MFD_K_POTRF && l.fused)))
+                for (int64_t j = l.first; j < l.first + l.count; ++j) {
+                    int rg, r, c;
+                    C.locate(h->gemm[j].c, rg, r, c);
+                    if (rg < m) last_diag_launch[rg] = (int)i;
+                }
+        }
+    std::vector<int> side_rec(2, -1);  // side launch last recorded on sev[0/1]
+    int side_last = -1, last_side = -1, joined = -1, gnext = 0;
+    std::vector<int> launch_node(h->fseq.size(), -1), early_nodes;
+    int cur_level = -1;
+    for (size_t i = 0; i < h->fseq.size(); ++i) {
+        const Launch& l = h->fseq[i];
+        int lev = -1;
+        for (int L = 0; L < h->num_levels; ++L)
+            if (h->level_launch[L] <= (int)i) lev = L;
+        if (lev != cur_level) {
+            // variant 1: the fourth stream joins before the next level starts
+            if (!early_nodes.empty()) {
+                const int j = C.add_node();
+                C.edge(main_last, j);
+                for (int e : early_nodes) C.edge(e, j);
+                main_last = j;
+                early_nodes.clear();
+            }
+            cur_level = lev;
+        }
+        std::vector<int> waits;
+        for (; gnext < G && h->level_launch[h->group_level[gnext]] <= (int)i; ++gnext) waits.push_back(fill_node[gnext]);
+        if (l.side) {
+            const int v = C.add_node();
+            C.edge(main_last, v);  // fork from the main stream's position
+            for (int w : waits) C.edge(w, v);
+            C.edge(side_last, v);
+            side_last = v;
+            side_rec[l.side_id & 1] = v;
+            last_side = l.side_id;
+            launch_node[i] = v;
+            C.launch(l, v);
+            // the waits were recorded on the main stream
+            if (!waits.empty()) {
+                const int j = C.add_node();
+                C.edge(main_last, j);
+                for (int w : waits) C.edge(w, j);
+                main_last = j;
+            }
+            continue;
+        }
+        if (l.kind != MFD_K_POTRF && l.kind != MFD_K_TRSM && last_side > joined) {
+            waits.push_back(side_rec[last_side & 1]);
+            joined = last_side;
+        } else if (l.wait_id > joined) {
+            waits.push_back(side_rec[l.wait_id & 1]);
+            joined = l.wait_id;
+        }
+        const int v = C.add_node();
+        C.edge(main_last, v);
+        for (int w : waits) C.edge(w, v);
+        main_last = v;
+        launch_node[i] = v;
+        if (variant >= 1 && l.kind == MFD_K_SCHUR) {
+            // the Schur tiles of fronts whose chain ended early go to the fourth stream
+            int steps_end = -1;
+            for (size_t k = 0; k < i; ++k)
+                if (h->fseq[k].kind == MFD_K_POTRF && (int)k >= h->level_launch[lev]) steps_end = (int)k;
+            std::vector<int> early;
+            for (int64_t t = l.first; t < l.first + l.count; ++t) {
+                int rg, r, c;
+                C.locate(h->gemm[t].c, rg, r, c);
+                if (rg < m && last_diag_launch[rg] >= 0 && last_diag_launch[rg] < steps_end &&
+                    std::find(early.begin(), early.end(), rg) == early.end())
+                    early.push_back(rg);
+            }
+            C.kind = -2;
+            for (int s : early) {
+                const int e = C.add_node();
+                C.edge(launch_node[last_diag_launch[s]], e);
+                if (!early_nodes.empty()) C.edge(early_nodes.back(), e);  // one stream: in order
+                early_nodes.push_back(e);
+                for (int64_t t = l.first; t < l.first + l.count; ++t) {
+                    int rg, r, c;
+                    C.locate(h->gemm[t].c, rg, r, c);
+                    if (rg == s) C.gemm(h->gemm[t], e, C.next_task++);
+                }
+            }
+            // the rest stays on the main stream
+            C.kind = MFD_K_SCHUR;
+            for (int64_t t = l.first; t < l.first + l.count; ++t) {
+                int rg, r, c;
+                C.locate(h->gemm[t].c, rg, r, c);
+                if (std::find(early.begin(), early.end(), rg) == early.end()) C.gemm(h->gemm[t], v, C.next_task++);
+            }
+            continue;
+        }
+        C.launch(l, v);
+    }
+    Reach R(C.pred);
+    // conflicts: accesses bucketed by region and 64 x 64 cell
+    std::unordered_map<int64_t, std::vector<int>> cells;
+    for (size_t a = 0; a < C.acc.size(); ++a) {
+        const Access& x = C.acc[a];
+        for (int I = x.r0 / 64; I <= (x.r1 - 1) / 64; ++I)
+            for (int J = x.c0 / 64; J <= (x.c1 - 1) / 64; ++J)
+                cells[((int64_t)x.region << 40) | ((int64_t)I << 20) | J].push_back((int)a);
+    }
+    int64_t bad = 0;
+    std::set<std::pair<int64_t, int64_t>> seen;
+    char line[256];
+    for (const auto& kv : cells) {
+        const auto& v = kv.second;
+        for (size_t p = 0; p < v.size(); ++p)
+            for (size_t q = p + 1; q < v.size(); ++q) {
+                const Access &x = C.acc[v[p]], &y = C.acc[v[q]];
+                if (x.task == y.task || !(x.write || y.write)) continue;
+                if (x.r1 <= y.r0 || y.r1 <= x.r0 || x.c1 <= y.c0 || y.c1 <= x.c0) continue;
+                const bool ordered = x.node != y.node && (R.before(x.node, y.node) || R.before(y.node, x.node));
+                if (ordered) continue;
+                const auto key = std::make_pair(std::min(x.task, y.task), std::max(x.task, y.task));
+                if (!seen.insert(key).second) continue;
+                ++bad;
+                if (bad <= 8) {
+                    snprintf(line, sizeof line,
+                             "%s: region %d, kinds %d (%s) / %d (%s), rows [%d,%d) / [%d,%d), cols [%d,%d) / [%d,%d)\n",
+                             x.node == y.node ? "same launch" : "unordered", x.region, x.kind, x.write ? "w" : "r",
+                             y.kind, y.write ? "w" : "r", x.r0, x.r1, y.r0, y.r1, x.c0, x.c1, y.c0, y.c1);
+                    report += line;
+                }
+            }
+    }
+    return bad;
+}
+
+}  // namespace
+
 // ================================================================== C-ABI
 extern "C" {
 
@@ -1566,6 +1910,21 @@ int mfd_set_update_period(mfd_handle* h, int period)
 
 int mfd_get_update_period(mfd_handle* h) { return h ? h->upd_period : MFD_ERR_ARG; }
 
+
+int mfd_check_schedule(mfd_handle* h, int variant, int64_t* conflicts, char* report, int report_len)
+{
+    if (!h || !conflicts || variant < 0 || variant > 2) return MFD_ERR_ARG;
+    if (!h->analyzed) return MFD_ERR_STATE;
+    if (!h->device_ok) build_plan(h);  // a handle without a GPU keeps no schedule
+    std::string rep;
+    *conflicts = check_schedule(h, variant, rep);
+    if (report && report_len > 0) {
+        const size_t n = std::min(rep.size(), (size_t)report_len - 1);
+        memcpy(report, rep.data(), n);
+        report[n] = 0;
+    }
+    return MFD_OK;
+}
 
 int mfd_set_front_split(mfd_handle* h, int cmax)
 {

@@ -98,6 +98,7 @@ def lib():
             "mfd_set_front_split": (i, [P, i]),
             "mfd_set_front_kinds": (i, [P, ip]),
             "mfd_get_front_times": (i, [P, dp]),
+            "mfd_check_schedule": (i, [P, i, lp, C.c_char_p, i]),
             "mfd_mlp_predict": (i, [i, dp, dp, dp]),
             "mfd_mlp_gradient": (i, [i, dp, dp, dp, dp, dp]),
             "mfd_mlp_train": (i, [i, dp, dp, dp, i, d, dp]),
@@ -267,6 +268,14 @@ class Solver:
             return
         k = _i32(kinds)
         self._check(lib().mfd_set_front_kinds(self.h, _ptr(k, C.c_int)), "mfd_set_front_kinds")
+
+    def check_schedule(self, variant=0):
+        """Static check of the factorization schedule: (number of unordered
+        conflicting task pairs, report of the first few)."""
+        n = C.c_int64()
+        buf = C.create_string_buffer(4096)
+        self._check(lib().mfd_check_schedule(self.h, int(variant), C.byref(n), buf, len(buf)), "mfd_check_schedule")
+        return n.value, buf.value.decode()
 
     def front_times(self):
         """Per node: microseconds of its workgroup in the last factorization
