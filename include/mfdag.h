@@ -290,6 +290,14 @@ int mfd_set_front_split(mfd_handle* h, int cmax);
 int mfd_set_front_kinds(mfd_handle* h, const int* kinds);
 int mfd_get_front_times(mfd_handle* h, double* us);
 
+/* Tile shape of the Schur updates (no counterpart in the reference, whose
+   tiles are nb x nb): fronts with at least min_rows update rows update
+   their Schur complement in 128 x 128 tiles, the others in 64 x 64 ones;
+   0 (the default): 64 x 64 only.  Rebuilds the schedule like
+   mfd_set_update_period. */
+int mfd_set_tile_bound(mfd_handle* h, int min_rows);
+int mfd_get_tile_bound(mfd_handle* h);
+
 /* Static check of the factorization schedule (no counterpart in the
    reference, whose StarPU runtime orders tasks by their data accesses):
    the launches of the current analysis on their streams (main, zeroing and

@@ -43,7 +43,9 @@ void launch_upd_diag(T* pool, const NodeDev* nodes, const DiagTask* diags, int n
 // with FP32 accumulation; 0 the front's own precision.
 template <typename T>
 void launch_gemm(T* pool, const GemmTask* tasks, int count, int kind, const int* lidx, hipStream_t st, int lowp = 0);
-template <typename T> void launch_gemm128(T* pool, const GemmTask* tasks, int count, hipStream_t st);
+// 128 x 128 tiles (m, n <= 128) of modes 0 and 3; lowp as for launch_gemm.
+template <typename T>
+void launch_gemm_wide(T* pool, const GemmTask* tasks, int count, const int* lidx, hipStream_t st, int lowp = 0);
 void launch_fwd_init(double* zpool, const double* b, int ldb, int nrhs, const NodeDev* nodes, const int2* tasks,
                      int count, hipStream_t st);
 void launch_fwd_add(double* zpool, int nrhs, const NodeDev* nodes, const int* lidx, const int2* tasks, int count,

@@ -926,15 +926,20 @@ __global__ __launch_bounds__(kThreads) void k_upd_diag(T* pool, const NodeDev* n
                              pool + nd.inv + (int64_t)(t.k0 / kPanel) * kPanel * kPanel);
 }
 
-// C -= A B^T on a 128 x 128 tile (GemmTask with m, n <= 128, mode 0): the
-// DSYRK/DGEMM updates of large fronts.  Each of the four waves owns a
-// 64 x 64 quarter as 4 x 4 MFMA 16x16x4 blocks, so every LDS operand read
-// feeds four MFMAs.  K is staged through LDS in chunks of 16, the next
-// chunk's global loads in flight during the current chunk's MFMAs.  A
-// diagonal tile (tri) skips the quarter above the diagonal.  The result is
-// written back one quarter at a time through LDS, coalesced along columns.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_gemm128(T* pool, const GemmTask* tasks)
+// C -= A B^T on a 128 x 128 tile (GemmTask with m, n <= 128, mode 0 or 3):
+// the Schur update of fronts with many update rows (and their extend-add,
+// mode 3) and large in-panel updates.  Each of the four waves owns a
+// 64 x 64 quarter as 4 x 4 MFMA 16x16x4 blocks, so every operand read from
+// LDS feeds four MFMAs (half the LDS reads and half the L2 traffic per flop
+// of the 64 x 64 tile).  K is staged through LDS in chunks of 16, double
+// buffered: chunk j+1's global loads are issued before chunk j's MFMAs and
+// stored to the other buffer after them, one barrier per chunk.  A diagonal
+// tile (tri) skips the quarter above the diagonal.  The result goes back one
+// quarter at a time through LDS (coalesced along columns); mode 3 also adds
+// each new entry into the parent front at the rows and columns local_index
+// gives (one launch holds at most one child of each parent).
+template <typename T, int LP = kNative>
+__global__ __launch_bounds__(kThreads, 2) void k_gemm_wide(T* pool, const GemmTask* tasks, const int* lidx)
 {
     typedef typename Mfma<T>::acc_t acc_t;
     constexpr int TB = 128, LD = TB + 4;
@@ -942,8 +947,8 @@ __global__ __launch_bounds__(kThreads) void k_gemm128(T* pool, const GemmTask* t
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, li = lane & 15;
     __shared__ union {
         struct {
-            T As[16][LD];
-            T Bs[16][LD];
+            T As[2][16][LD];
+            T Bs[2][16][LD];
         } k;
         T Cs[64][65];
     } u;
@@ -966,53 +971,95 @@ __global__ __launch_bounds__(kThreads) void k_gemm128(T* pool, const GemmTask* t
             rb[p] = (lr < t.n && kk < t.k) ? B[lr + (int64_t)kk * t.ldb] : T(0);
         }
     };
-    load(0);
-    for (int k0 = 0; k0 < t.k; k0 += 16) {
+    auto store = [&](int buf) {
 #pragma unroll
         for (int p = 0; p < 8; ++p) {
-            u.k.As[lk + 2 * p][lr] = ra[p];
-            u.k.Bs[lk + 2 * p][lr] = rb[p];
+            u.k.As[buf][lk + 2 * p][lr] = ra[p];
+            u.k.Bs[buf][lk + 2 * p][lr] = rb[p];
         }
-        __syncthreads();
-        if (k0 + 16 < t.k) load(k0 + 16);
+    };
+    load(0);
+    store(0);
+    __syncthreads();
+    const int nch = (t.k + 15) / 16;
+    for (int j = 0; j < nch; ++j) {
+        const int buf = j & 1;
+        if (j + 1 < nch) load(16 * (j + 1));
         if (!skip) {
+            if constexpr (LP == kNative) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int kr = 4 * q + (lane >> 4);
-                T a[4], b[4];
+                for (int q = 0; q < 4; ++q) {
+                    const int kr = 4 * q + (lane >> 4);
+                    T a[4], b[4];
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    a[i] = u.k.As[kr][wr + 16 * i + li];
-                    b[i] = u.k.Bs[kr][wc + 16 * i + li];
+                    for (int i = 0; i < 4; ++i) {
+                        a[i] = u.k.As[buf][kr][wr + 16 * i + li];
+                        b[i] = u.k.Bs[buf][kr][wc + 16 * i + li];
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int jj = 0; jj < 4; ++jj) acc[i][jj] = Mfma<T>::mma(a[i], b[jj], acc[i][jj]);
                 }
+            } else {
+                typedef Mfma16<LP> M16;
+                const int kr = 4 * (lane >> 4);
+                typename M16::op_t a[4], b[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::mma(a[i], b[j], acc[i][j]);
+                    for (int e = 0; e < 4; ++e) {
+                        a[i][e] = M16::cvt(u.k.As[buf][kr + e][wr + 16 * i + li]);
+                        b[i][e] = M16::cvt(u.k.Bs[buf][kr + e][wc + 16 * i + li]);
+                    }
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int jj = 0; jj < 4; ++jj) acc[i][jj] = M16::mma(a[i], b[jj], acc[i][jj]);
             }
         }
+        if (j + 1 < nch) store(buf ^ 1);
         __syncthreads();
     }
     T* C = pool + t.c;
+    const bool ea = t.mode == 3;
     for (int w = 0; w < 4; ++w) {
         const int qr = (w & 1) * 64, qc = (w >> 1) * 64;
-        if (t.tri && qc > qr) continue;
-        if (qr >= t.m || qc >= t.n) continue;
+        if ((t.tri && qc > qr) || qr >= t.m || qc >= t.n) continue;  // uniform
         if (wv == w) {
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
+                for (int jj = 0; jj < 4; ++jj)
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) u.Cs[16 * j + li][16 * i + Mfma<T>::row(lane, v)] = acc[i][j][v];
+                    for (int v = 0; v < 4; ++v) u.Cs[16 * jj + li][16 * i + Mfma<T>::row(lane, v)] = acc[i][jj][v];
         }
         __syncthreads();
         const int row = tid & 63, gr = qr + row;
-        if (gr < t.m)
-            for (int col = tid >> 6; col < 64; col += 4) {
-                const int gc = qc + col;
-                if (gc < t.n && (!t.tri || gr >= gc)) C[gr + (int64_t)gc * t.ldc] -= u.Cs[col][row];
+        if (gr < t.m) {
+            T* P = ea ? pool + t.p + (lidx[t.li + gr] - 1) : C;
+            for (int c0 = tid >> 6; c0 < 64; c0 += 16) {
+                T cv[4], pv[4];
+                int64_t po[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int gc = min(qc + c0 + 4 * q, t.n - 1);
+                    cv[q] = C[gr + (int64_t)gc * t.ldc];
+                    po[q] = ea ? (int64_t)(lidx[t.lj + gc] - 1) * t.np : 0;
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) pv[q] = ea ? P[po[q]] : T(0);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int col = c0 + 4 * q, gc = qc + col;
+                    if (gc < t.n && (!t.tri || gr >= gc)) {
+                        const T v = cv[q] - u.Cs[col][row];
+                        C[gr + (int64_t)gc * t.ldc] = v;
+                        if (ea) P[po[q]] = pv[q] + v;
+                    }
+                }
             }
+        }
         __syncthreads();
     }
 }
@@ -1861,9 +1908,21 @@ void launch_scale_rows(double* out, const double* in, const double* s, int n, in
         hipLaunchKernelGGL(k_scale_rows, dim3((unsigned)((m + kThreads - 1) / kThreads)), dim3(kThreads), 0, st, out, in,
                            s, n, nrhs);
 }
-template <typename T> void launch_gemm128(T* pool, const GemmTask* tasks, int count, hipStream_t st)
+template <typename T>
+void launch_gemm_wide(T* pool, const GemmTask* tasks, int count, const int* lidx, hipStream_t st, int lowp)
 {
-    if (count > 0) hipLaunchKernelGGL(k_gemm128<T>, dim3(count), dim3(kThreads), 0, st, pool, tasks);
+    if (count <= 0) return;
+    if constexpr (sizeof(T) == 4) {
+        if (lowp == kF16) {
+            hipLaunchKernelGGL((k_gemm_wide<T, kF16>), dim3(count), dim3(kThreads), 0, st, pool, tasks, lidx);
+            return;
+        }
+        if (lowp == kBF16) {
+            hipLaunchKernelGGL((k_gemm_wide<T, kBF16>), dim3(count), dim3(kThreads), 0, st, pool, tasks, lidx);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_gemm_wide<T, kNative>), dim3(count), dim3(kThreads), 0, st, pool, tasks, lidx);
 }
 void launch_fwd_init(double* zpool, const double* b, int ldb, int nrhs, const NodeDev* nodes, const int2* tasks,
                      int count, hipStream_t st)
@@ -1921,7 +1980,7 @@ void launch_scatter_rows(double* out, int ld_out, const double* in, int ld_in, c
     template void launch_diag<T>(T*, const NodeDev*, const DiagTask*, int, int*, hipStream_t);                        \
     template void launch_upd_diag<T>(T*, const NodeDev*, const DiagTask*, int, const GemmTask*, int, int*, hipStream_t); \
     template void launch_gemm<T>(T*, const GemmTask*, int, int, const int*, hipStream_t, int);                         \
-    template void launch_gemm128<T>(T*, const GemmTask*, int, hipStream_t);                                            \
+    template void launch_gemm_wide<T>(T*, const GemmTask*, int, const int*, hipStream_t, int);                         \
     template void launch_scatter_a_scaled<T>(T*, const double*, const int64_t*, const int64_t*, const int2*,           \
                                              const double*, int64_t, hipStream_t);                                    \
     template void launch_fwd_step<T>(const T*, double*, int, const NodeDev*, const SolveTask*, int, hipStream_t);       \

@@ -37,7 +37,7 @@ struct Launch {
     int kind;       // MFD_K_*
     int64_t first;  // first task in the kind's task array
     int count;
-    int wide = 0;   // GEMM launches: 1 = 128 x 128 tiles (k_gemm128)
+    int wide = 0;   // GEMM launches: 1 = 128 x 128 tiles (k_gemm_wide)
     // MFD_K_POTRF launches: fused = 1 runs the step's update tiles
     // [first, first + count) of the GEMM tasks beside the diagonal tasks
     // [dfirst, dfirst + dcount) (k_upd_diag); otherwise [first, first +
@@ -75,9 +75,14 @@ constexpr int kUpdPeriod = 2;
 constexpr int kMaxPeriod = 8;
 constexpr int kAutoSteps = 48;  // 3072 columns
 
-// Update and Schur GEMMs: 128 x 128 tiles (k_gemm128) or 64 x 64 (k_gemm).
-constexpr bool kUseWide = false;
-constexpr int kWide = kUseWide ? 128 : kTile;
+// Schur updates of fronts with at least wide_min_rows update rows use
+// 128 x 128 tiles (k_gemm_wide), the others 64 x 64 (k_gemm); set per handle
+// by mfd_set_tile_bound, 0 (64 x 64 only) by default: on the 64^3 workload
+// every bound tried (256, 512, 1024) was slower (factor 24.1-24.7 against
+// 21.5 ms), the 128 x 128 launches of its few large fronts holding fewer
+// tiles than the GPU has CUs.
+constexpr int kWideTile = 128;
+constexpr int kWideMinRows = 0;
 
 double now_ms()
 {
@@ -110,6 +115,7 @@ struct mfd_handle {
     // workgroup, 0 = 64-column steps over the chip, -1 = by fuse_cmax.
     int fuse_cmax = 0;
     std::vector<int> front_kind;
+    int wide_min_rows = kWideMinRows;  // Schur tiles of 128 x 128 from this many update rows (0: never)
     bool device_ok = false;
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;  // large in-panel updates beside the diagonal chain
@@ -713,35 +719,38 @@ void build_plan(mfd_handle* h)
         // no two tiles of a launch add into one parent entry
         int maxrank = 0;
         for (int s : large) maxrank = std::max(maxrank, rank[s]);
-        for (int r = 0; r <= maxrank; ++r) {
-            int64_t f4 = (int64_t)h->gemm.size();
-            for (int s : large) {
-                if (rank[s] != r) continue;
-                const NodeDev& d = h->nodes[s];
-                const NodeDev* pa = d.parent >= 0 ? &h->nodes[d.parent] : nullptr;
-                for (int c0 = d.c; c0 < d.n; c0 += kWide)
-                    for (int r0 = c0; r0 < d.n; r0 += kWide) {
-                        GemmTask g{};
-                        g.c = d.front + r0 + (int64_t)c0 * d.n;
-                        g.a = d.front + r0;
-                        g.b = d.front + c0;
-                        g.ldc = g.lda = g.ldb = d.n;
-                        g.m = std::min(kWide, d.n - r0);
-                        g.n = std::min(kWide, d.n - c0);
-                        g.k = d.c;
-                        g.tri = r0 == c0;
-                        if (pa) {
-                            g.mode = 3;
-                            g.p = pa->front;
-                            g.np = pa->n;
-                            g.li = d.lidx + (r0 - d.c);
-                            g.lj = d.lidx + (c0 - d.c);
+        for (int r = 0; r <= maxrank; ++r)
+            for (int wide = 1; wide >= 0; --wide) {  // the 128 x 128 launch, then the 64 x 64 one
+                const int tile = wide ? kWideTile : kTile;
+                int64_t f4 = (int64_t)h->gemm.size();
+                for (int s : large) {
+                    if (rank[s] != r) continue;
+                    const NodeDev& d = h->nodes[s];
+                    if ((h->wide_min_rows > 0 && d.w >= h->wide_min_rows) != (wide == 1)) continue;
+                    const NodeDev* pa = d.parent >= 0 ? &h->nodes[d.parent] : nullptr;
+                    for (int c0 = d.c; c0 < d.n; c0 += tile)
+                        for (int r0 = c0; r0 < d.n; r0 += tile) {
+                            GemmTask g{};
+                            g.c = d.front + r0 + (int64_t)c0 * d.n;
+                            g.a = d.front + r0;
+                            g.b = d.front + c0;
+                            g.ldc = g.lda = g.ldb = d.n;
+                            g.m = std::min(tile, d.n - r0);
+                            g.n = std::min(tile, d.n - c0);
+                            g.k = d.c;
+                            g.tri = r0 == c0;
+                            if (pa) {
+                                g.mode = 3;
+                                g.p = pa->front;
+                                g.np = pa->n;
+                                g.li = d.lidx + (r0 - d.c);
+                                g.lj = d.lidx + (c0 - d.c);
+                            }
+                            h->gemm.push_back(g);
                         }
-                        h->gemm.push_back(g);
-                    }
+                }
+                push(MFD_K_SCHUR, f4, (int64_t)h->gemm.size(), wide);
             }
-            push(MFD_K_SCHUR, f4, (int64_t)h->gemm.size(), kUseWide);
-        }
 
     }
     build_solve_plan(h);
@@ -907,7 +916,7 @@ template <typename T> void enqueue_launch(mfd_handle* h, const Launch& l, hipStr
     case MFD_K_UPDATE:
     case MFD_K_SCHUR:
         if (l.wide)
-            launch_gemm128<T>(pool, h->d_gemm + l.first, l.count, st);
+            launch_gemm_wide<T>(pool, h->d_gemm + l.first, l.count, h->d_lidx, st, sizeof(T) == 4 ? h->lowp : 0);
         else
             launch_gemm<T>(pool, h->d_gemm + l.first, l.count,
                            l.kind == MFD_K_TRSM ? 0 : l.kind == MFD_K_UPDATE ? 1 : 2, h->d_lidx, st,
@@ -1925,6 +1934,15 @@ int mfd_check_schedule(mfd_handle* h, int variant, int64_t* conflicts, char* rep
     }
     return MFD_OK;
 }
+
+int mfd_set_tile_bound(mfd_handle* h, int min_rows)
+{
+    if (!h || min_rows < 0) return MFD_ERR_ARG;
+    if (min_rows == h->wide_min_rows) return MFD_OK;
+    h->wide_min_rows = min_rows;
+    return replan(h);
+}
+int mfd_get_tile_bound(mfd_handle* h) { return h ? h->wide_min_rows : MFD_ERR_ARG; }
 
 int mfd_set_front_split(mfd_handle* h, int cmax)
 {

@@ -96,6 +96,8 @@ def lib():
             "mfd_set_update_period": (i, [P, i]),
             "mfd_get_update_period": (i, [P]),
             "mfd_set_front_split": (i, [P, i]),
+            "mfd_set_tile_bound": (i, [P, i]),
+            "mfd_get_tile_bound": (i, [P]),
             "mfd_set_front_kinds": (i, [P, ip]),
             "mfd_get_front_times": (i, [P, dp]),
             "mfd_check_schedule": (i, [P, i, lp, C.c_char_p, i]),
@@ -254,6 +256,15 @@ class Solver:
     @update_period.setter
     def update_period(self, period):
         self._check(lib().mfd_set_update_period(self.h, int(period)), "mfd_set_update_period")
+
+    @property
+    def tile_bound(self):
+        """Update rows from which a front's Schur tiles are 128 x 128 (0: never)."""
+        return lib().mfd_get_tile_bound(self.h)
+
+    @tile_bound.setter
+    def tile_bound(self, min_rows):
+        self._check(lib().mfd_set_tile_bound(self.h, int(min_rows)), "mfd_set_tile_bound")
 
     def set_front_split(self, cmax):
         """Fronts of order > 64 with at most cmax pivot columns are factored

@@ -139,3 +139,20 @@ def test_medium_batch_replays_and_matches_oracle():
     assert all(np.array_equal(e.front(k + 1), ref[k]) for k in range(s.num_node))
     s.close()
     e.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,make,max_zero,nb", CASES, ids=[c[0] for c in CASES])
+def test_wide_schur_tiles_against_oracle(name, make, max_zero, nb):
+    """Every Schur update (with its extend-add) in 128 x 128 tiles."""
+    p = make()
+    s = mfdag.Solver(nb=nb, graph=True)
+    s.analyze(p.ccs_col, p.ccs_row, max_zero, p.perm)
+    s.tile_bound = 1
+    assert s.tile_bound == 1
+    _check_fp64(s, p, name + " wide")
+    xt, b = P.rhs(p, 1, seed=6)
+    for prec in (mfdag.FP32_IR, mfdag.FP16_IR):
+        assert s.factorize(p.val, prec) == 0
+        assert P.backward_error(p, s.solve(b.copy()), b) <= 1e-14
+    s.close()
