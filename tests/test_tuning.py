@@ -246,3 +246,71 @@ def test_tuner_cost_model_regret():
     assert mean_gain >= 1.0
     assert mean_regret <= 0.5 * worst_regret
     assert mean_regret <= blind_regret
+
+
+# ----------------------------------------------- per-front medium/large choice
+def _open(make, max_zero, profile=False):
+    p = make()
+    s = mfdag.Solver(nb=64, graph=not profile, profile=profile)
+    if isinstance(p, list):  # a batch of systems
+        bp = P.batch_problem(p)
+        s.analyze_batch([(q.ccs_col, q.ccs_row) for q in p], max_zero, [q.perm for q in p])
+        return s, bp.val, p
+    s.analyze(p.ccs_col, p.ccs_row, max_zero, p.perm)
+    return s, p.val, p
+
+
+FRONT_TRAIN = [("wlap3d_24", lambda: P.weighted_laplacian_3d(24, seed=24), 16384),
+               ("lap2d_200", lambda: P.laplacian_2d(200), 4096),
+               ("st27_16", lambda: P.stencil27_3d(16, seed=16), 4096),
+               ("blk3_12", lambda: P.block_stencil_3dof(12, seed=12), 4096),
+               ("batch_96", lambda: P.batch_anisotropic(96, seed=5), 4096)]
+FRONT_HELD = [("lap2d_400", lambda: P.laplacian_2d(400), 4096),
+              ("wlap3d_40", lambda: P.weighted_laplacian_3d(40, seed=40), 16384),
+              ("batch_192", lambda: P.batch_anisotropic(192, seed=9), 4096)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [mfdag.FP64, mfdag.FP32_IR])
+def test_front_policy_regret(prec):
+    """The per-front medium/large choice (tune.choose_kinds: the perceptron
+    trained on the medium kernel's per-front clocks, the step and Schur
+    launches by constants fitted to profiled runs) against the exhaustive
+    tuner over global pivot bounds, on problems outside the training set.
+    Controls: every front medium (the blind extreme) does far worse, and the
+    perceptron fitted to targets shuffled across fronts does not fit."""
+    eb = 8 if prec == mfdag.FP64 else 4
+    recs, crecs = [], []
+    for name, make, mz in FRONT_TRAIN:
+        s, a, p = _open(make, mz)
+        recs.append(tune.front_records(s, a, prec))
+        s.close()
+        if isinstance(p, list):
+            crecs.append(tune.chain_record(None, None, mz, [q.perm for q in p], a, prec,
+                                           batch=[(q.ccs_col, q.ccs_row) for q in p]))
+        else:
+            crecs.append(tune.chain_record(p.ccs_col, p.ccs_row, mz, p.perm, a, prec))
+    tau, t0, rate = tune.chain_constants(crecs)
+    model = tune.FrontModel(seed=0).fit(recs)
+    max_order = int(max(r["n"].max() for r in recs))
+    log(f"prec {prec}: {sum(len(r['us']) for r in recs)} front records, loss {model.loss:.4f}; "
+        f"step round {tau:.1f} us, Schur {t0:.1f} us/launch + {rate / 1e6:.1f} TFLOP/s")
+    shuffled = [dict(r, us=np.random.default_rng(3).permutation(r["us"])) for r in recs]
+    blind = tune.FrontModel(seed=0).fit(shuffled)
+    assert blind.loss > 3 * model.loss
+    regrets, extreme = [], []
+    for name, make, mz in FRONT_HELD:
+        s, a, p = _open(make, mz)
+        t = tune.measure_split(s, a, prec)
+        kinds = tune.choose_kinds(s.structure(), model, tau, t0, rate, eb, max_order)
+        tk = tune.measure_kinds(s, a, prec, kinds)
+        best = min(t.values())
+        regrets.append(tk / best - 1.0)
+        extreme.append(t[max(t)] / best - 1.0)
+        log(f"{name}: bounds {({k: round(v, 2) for k, v in t.items()})}, policy {tk:.2f} ms "
+            f"({int(kinds.sum())} of {s.num_node} fronts medium), regret {100 * regrets[-1]:.1f}%")
+        s.close()
+    log(f"mean regret {100 * np.mean(regrets):.1f}%, every front medium {100 * np.mean(extreme):.0f}%")
+    assert np.mean(regrets) <= 0.03
+    assert max(regrets) <= 0.10
+    assert np.mean(extreme) >= 10 * max(np.mean(regrets), 0.01)
