@@ -341,6 +341,9 @@ def main():
     ap.add_argument("--c5-size", type=int, default=40, help="grid of the low-precision block-stencil leg (0: skip)")
     ap.add_argument("--tile-bound", type=int, default=-1, metavar="ROWS",
                     help="Schur tiles of 128 x 128 for fronts with at least ROWS update rows (-1: the library default)")
+    ap.add_argument("--solve-block", type=int, default=-1, metavar="COLS",
+                    help="forward solve over 256-column superblocks for fronts with at least COLS pivot columns "
+                         "(0: never; -1: the library default)")
     ap.add_argument("--front-split", type=int, default=-1, metavar="CMAX",
                     help="medium fronts: those with at most CMAX pivot columns factored by one workgroup each "
                          "(-1: the library default)")
@@ -388,6 +391,8 @@ def main():
     s.analyze(p.ccs_col, p.ccs_row, args.max_zero, p.perm)
     if args.tile_bound >= 0:
         s.tile_bound = args.tile_bound
+    if args.solve_block >= 0:
+        s.solve_block = args.solve_block
     analyze_s = time.perf_counter() - t0
     cost = s.cost()
     log(f"analysis {analyze_s:.2f}s: fronts={s.num_node} cost={cost:.3e} nnz(L)={s.nonzero():.3e}")

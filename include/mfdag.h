@@ -298,6 +298,17 @@ int mfd_get_front_times(mfd_handle* h, double* us);
 int mfd_set_tile_bound(mfd_handle* h, int min_rows);
 int mfd_get_tile_bound(mfd_handle* h);
 
+/* Block width of the forward solve (no counterpart in the reference, whose
+   forward sweep is one DTRSV and one DGEMV per front,
+   seq_solve/seq_forward_m.f90:14-81): fronts with at least min_c pivot
+   columns step over 256-column superblocks through explicit inverses of
+   their 256 x 256 diagonal blocks (formed at the end of each
+   factorization), the others over 64-column blocks; 0: 64 columns only.
+   Default 512.  Rebuilds the schedule like mfd_set_update_period (the
+   factorization is lost). */
+int mfd_set_solve_block(mfd_handle* h, int min_c);
+int mfd_get_solve_block(mfd_handle* h);
+
 /* Static check of the factorization schedule (no counterpart in the
    reference, whose StarPU runtime orders tasks by their data accesses):
    the launches of the current analysis on their streams (main, zeroing and

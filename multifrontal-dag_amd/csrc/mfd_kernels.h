@@ -53,6 +53,14 @@ void launch_fwd_add(double* zpool, int nrhs, const NodeDev* nodes, const int* li
 template <typename T>
 void launch_fwd_step(const T* pool, double* zpool, int nrhs, const NodeDev* nodes, const SolveTask* tasks, int count,
                      hipStream_t st);
+// 64 x 64 blocks of the superblock matrices (X by block distance, then W)
+template <typename T>
+void launch_superinv(const T* pool, T* sinv, const NodeDev* nodes, const SuperInvTask* tasks, int count,
+                     hipStream_t st);
+// forward sweep step over 256-column superblocks
+template <typename T>
+void launch_fwd_super(const T* pool, const T* sinv, double* zpool, int nrhs, const NodeDev* nodes,
+                      const SuperTask* tasks, int count, hipStream_t st);
 void launch_bwd_gather(double* zpool, int nrhs, const NodeDev* nodes, const int* lidx, const int2* tasks, int count,
                        hipStream_t st);
 template <typename T>

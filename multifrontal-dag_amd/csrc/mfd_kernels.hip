@@ -1415,6 +1415,179 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(const T* pool, double* zp
     }
 }
 
+// ------------------------------------------------------------ superblocks
+// The two 256 x 256 matrices of superblock u (u >= 1 for W), column-major,
+// X at xoff and W at xoff + 256^2:
+//   X = inverse of the superblock's diagonal block of L (zero past its
+//       width sb), by 64 x 64 blocks (i, j), j <= i: X_ii is the stored
+//       64 x 64 inverse (j = i; the row's blocks above the diagonal cleared,
+//       the whole row past sb) and X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj;
+//   W = X L[B_u, B_{u-1}] (j >= 4: block (i, j - 4)), the coupling of the
+//       superblock to the previous one, so that y_u = X z_u - W y_{u-1}.
+// One task per block, launched by block distance i - j (then the W blocks).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_superinv(const T* pool, T* sinv, const NodeDev* nodes,
+                                                       const SuperInvTask* tasks)
+{
+    const SuperInvTask t = tasks[blockIdx.x];
+    const NodeDev nd = nodes[t.node];
+    T* X = sinv + t.xoff;
+    const int n = nd.n, c0 = kSuper * t.u, tid = threadIdx.x;
+    const int i = t.i, j = t.j, b0 = c0 + 64 * i;  // first global pivot column of block row i
+    const int kbi = max(0, min(64, nd.c - b0));     // pivot rows of block row i
+    if (i == j) {
+        const bool live = kbi > 0;
+        const T* inv = pool + nd.inv + (int64_t)(live ? b0 / 64 : 0) * 4096;  // inv[r + 64 q] = L^{-1}(q, r)
+        for (int e = tid; e < 4096; e += kThreads) {
+            const int r = e & 63, q = e >> 6;  // block row q, column r
+            X[(64 * i + q) + kSuper * (64 * i + r)] = (r < kbi && q < kbi) ? inv[r + 64 * q] : T(0);
+            for (int jj = 0; jj < 4; ++jj)
+                if (jj > i || (!live && jj < i)) X[(64 * i + q) + kSuper * (64 * jj + r)] = T(0);
+        }
+        return;
+    }
+    __shared__ T As[64][65], Bs[64][65];  // As[m][r] = A(r, m), Bs[col][m] = B(m, col)
+    const int r = tid & 63, cq = tid >> 6;
+    T acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = T(0);
+    const bool wblk = j >= 4;
+    const int cp = c0 - kSuper;  // previous superblock's first column
+    for (int k = wblk ? 0 : j; k < (wblk ? i + 1 : i); ++k) {
+        const int kbk = min(64, nd.c - (c0 + 64 * k));
+        for (int e = tid; e < 4096; e += kThreads) {
+            const int rr = e & 63, m = e >> 6;
+            if (wblk) {
+                // A = X_ik, B = L[B_u block k, B_{u-1} block j - 4]
+                As[m][rr] = X[(64 * i + rr) + kSuper * (64 * k + m)];
+                Bs[m][rr] = rr < kbk ? pool[nd.front + (c0 + 64 * k + rr) + (int64_t)(cp + 64 * (j - 4) + m) * n] : T(0);
+            } else {
+                // A = L_ik, B = X_kj
+                As[m][rr] = rr < kbi ? pool[nd.front + (b0 + rr) + (int64_t)(c0 + 64 * k + m) * n] : T(0);
+                Bs[m][rr] = X[(64 * k + rr) + kSuper * (64 * j + m)];
+            }
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int m = 0; m < 64; ++m) {
+            const T a = As[m][r];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) acc[q] += a * Bs[16 * cq + q][m];
+        }
+        __syncthreads();
+    }
+    if (wblk) {
+        T* W = X + kSuper * kSuper;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) W[(64 * i + r) + kSuper * (64 * (j - 4) + 16 * cq + q)] = acc[q];
+        return;
+    }
+    // X_ij = -X_ii T1: As[m][rr] = X_ii(rr, m), Bs[col][m] = T1(m, col)
+    for (int e = tid; e < 4096; e += kThreads) {
+        const int rr = e & 63, m = e >> 6;
+        As[m][rr] = X[(64 * i + rr) + kSuper * (64 * i + m)];
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) Bs[16 * cq + q][r] = acc[q];
+    __syncthreads();
+    T out[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) out[q] = T(0);
+#pragma unroll 4
+    for (int m = 0; m < 64; ++m) {
+        const T a = As[m][r];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) out[q] += a * Bs[16 * cq + q][m];
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) X[(64 * i + r) + kSuper * (64 * j + 16 * cq + q)] = -out[q];
+}
+
+// Forward sweep over superblocks, one launch per superblock step u, every
+// task 16 rows wide (thread: row tid % 16, columns 16 g .. 16 g + 15 of
+// each 256-column operand, g = tid / 16; the 16 partial sums per row summed
+// through LDS):
+//   - top task (kind 1, rows row0 .. row0 + 15 of superblock u):
+//     y_u = X z_u - W y_{u-1}, written to Y (z_u: Z at the superblock's
+//     rows, already holding every superblock before u - 1);
+//   - chunk task (kind 0, front rows row0 .. row0 + 15, all after
+//     superblock u): Z[rows] -= L[rows, B_{u-1}] y_{u-1}.
+// Every matrix load of a task is issued before its first wait.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_fwd_super(const T* pool, const T* sinv, double* zpool, int nrhs,
+                                                        const NodeDev* nodes, const SuperTask* tasks)
+{
+    const SuperTask t = tasks[blockIdx.x];
+    const NodeDev nd = nodes[t.node];
+    const int n = nd.n, c = nd.c, c0 = kSuper * t.u, cp = c0 - kSuper, tid = threadIdx.x;
+    const int i = tid & 15, g = tid >> 4;
+    double* Z = zpool + nd.z * nrhs;
+    double* Y = Z + (int64_t)n * nrhs;
+    __shared__ double zs[kSuper][kRhsGroup + 1], ys[kSuper][kRhsGroup + 1];
+    __shared__ double red[16][16][kRhsGroup + 1];
+    const int pb = t.pb, sb = t.sb;
+    const bool top = t.kind == 1;
+    T a[16], w[16];
+    if (top) {
+        const T* X = sinv + t.xoff;
+        const int r = t.row0 + i;
+        const bool lower = 16 * g <= t.row0 + 15;  // X is lower triangular
+#pragma unroll
+        for (int q = 0; q < 16; ++q) a[q] = lower ? X[r + kSuper * (16 * g + q)] : T(0);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) w[q] = pb > 0 ? X[kSuper * kSuper + r + kSuper * (16 * g + q)] : T(0);
+    } else {
+        const T* Lr = pool + nd.front + min(t.row0 + i, n - 1) + (int64_t)cp * n;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) w[q] = Lr[(int64_t)min(16 * g + q, pb - 1) * n];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) a[q] = T(0);
+    }
+    for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
+        const int nr = min(kRhsGroup, nrhs - r0);
+        for (int e = tid; e < kSuper * kRhsGroup; e += kThreads) {
+            const int m = e & (kSuper - 1), rr = e / kSuper;
+            ys[m][rr] = (m < pb && rr < nr) ? Y[cp + m + (int64_t)(r0 + rr) * c] : 0.0;
+            if (top) zs[m][rr] = (m < sb && rr < nr) ? Z[c0 + m + (int64_t)(r0 + rr) * n] : 0.0;
+        }
+        __syncthreads();
+        double acc[kRhsGroup];
+#pragma unroll
+        for (int rr = 0; rr < kRhsGroup; ++rr) acc[rr] = 0.0;
+        if (top) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const double av = (double)a[q];
+#pragma unroll
+                for (int rr = 0; rr < kRhsGroup; ++rr) acc[rr] += av * zs[16 * g + q][rr];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const double wv = 16 * g + q < pb ? (double)w[q] : 0.0;
+#pragma unroll
+            for (int rr = 0; rr < kRhsGroup; ++rr) acc[rr] -= wv * ys[16 * g + q][rr];
+        }
+#pragma unroll
+        for (int rr = 0; rr < kRhsGroup; ++rr) red[g][i][rr] = acc[rr];
+        __syncthreads();
+        if (tid < 16 * kRhsGroup) {
+            const int i2 = tid & 15, rr = tid >> 4;
+            double v = 0.0;
+#pragma unroll
+            for (int gg = 0; gg < 16; ++gg) v += red[gg][i2][rr];
+            if (rr < nr) {
+                if (top) {
+                    if (t.row0 + i2 < sb) Y[c0 + t.row0 + i2 + (int64_t)(r0 + rr) * c] = v;
+                } else if (t.row0 + i2 < n) {
+                    Z[t.row0 + i2 + (int64_t)(r0 + rr) * n] += v;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Backward, first launch of a level, task (front, first row): 256 rows of
 // the update part of Z take the parent's solution at the parent-front
 // positions -- pivot positions from the parent's Y, update positions from
@@ -1940,6 +2113,19 @@ void launch_fwd_step(const T* pool, double* zpool, int nrhs, const NodeDev* node
 {
     if (count > 0) hipLaunchKernelGGL(k_fwd_step<T>, dim3(count), dim3(kThreads), 0, st, pool, zpool, nrhs, nodes, tasks);
 }
+template <typename T>
+void launch_superinv(const T* pool, T* sinv, const NodeDev* nodes, const SuperInvTask* tasks, int count,
+                     hipStream_t st)
+{
+    if (count > 0) hipLaunchKernelGGL(k_superinv<T>, dim3(count), dim3(kThreads), 0, st, pool, sinv, nodes, tasks);
+}
+template <typename T>
+void launch_fwd_super(const T* pool, const T* sinv, double* zpool, int nrhs, const NodeDev* nodes,
+                      const SuperTask* tasks, int count, hipStream_t st)
+{
+    if (count > 0)
+        hipLaunchKernelGGL(k_fwd_super<T>, dim3(count), dim3(kThreads), 0, st, pool, sinv, zpool, nrhs, nodes, tasks);
+}
 void launch_bwd_gather(double* zpool, int nrhs, const NodeDev* nodes, const int* lidx, const int2* tasks, int count,
                        hipStream_t st)
 {
@@ -1984,6 +2170,9 @@ void launch_scatter_rows(double* out, int ld_out, const double* in, int ld_in, c
     template void launch_scatter_a_scaled<T>(T*, const double*, const int64_t*, const int64_t*, const int2*,           \
                                              const double*, int64_t, hipStream_t);                                    \
     template void launch_fwd_step<T>(const T*, double*, int, const NodeDev*, const SolveTask*, int, hipStream_t);       \
+    template void launch_superinv<T>(const T*, T*, const NodeDev*, const SuperInvTask*, int, hipStream_t);            \
+    template void launch_fwd_super<T>(const T*, const T*, double*, int, const NodeDev*, const SuperTask*, int,         \
+                                      hipStream_t);                                                                    \
     template void launch_bwd_step<T>(const T*, double*, double*, double*, int, int, const NodeDev*, const SolveTask*,  \
                                      int, hipStream_t);
 MFD_INSTANTIATE(double)

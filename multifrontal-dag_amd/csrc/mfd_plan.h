@@ -58,6 +58,30 @@ struct SolveTask {
     int slot, nchunk, pad0, pad;
 };
 
+// Superblocks: the forward solve of a front with many pivot columns steps
+// over 256-column superblocks u with two 256 x 256 matrices per superblock
+// (column-major, in a separate array at xoff): X, the inverse of the
+// superblock's diagonal block (zero past its width), and at xoff + 256^2
+// W = X L[B_u, B_{u-1}] (u >= 1).
+constexpr int kSuper = 256;
+// Forward task of superblock step u, 16 rows: kind 1, rows row0 .. row0 +
+// 15 of superblock u (width sb; pb the width of superblock u-1, 0 for
+// u = 0); kind 0, front rows row0 .. row0 + 15 after superblock u, to which
+// superblock u-1 (width pb) is applied.
+struct SuperTask {
+    int node, u, row0, sb;
+    int pb, kind;
+    int64_t xoff;
+};
+// One 64 x 64 block (i, j) of superblock u's matrices: j = i copies the
+// stored 64 x 64 inverse (and clears the row's blocks above the diagonal,
+// or the whole row past the superblock's width); j < i forms
+// X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj; j >= 4 forms block (i, j - 4) of W.
+struct SuperInvTask {
+    int node, u, i, j;
+    int64_t xoff;
+};
+
 // mode 0: C (m x n tile, ldc) -= A (m x k, lda) * B (n x k, ldb)^T;
 // mode 1: C = A B^T with B stored transposed (B[j][k] at b + j * ldb + k);
 // A may be C itself: the panel TRSM X = P L^{-T}, with b the stored L^{-T}

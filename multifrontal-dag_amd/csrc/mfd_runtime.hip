@@ -84,6 +84,12 @@ constexpr int kAutoSteps = 48;  // 3072 columns
 constexpr int kWideTile = 128;
 constexpr int kWideMinRows = 0;
 
+// Fronts with at least super_min pivot columns solve forward over
+// 256-column superblocks (k_fwd_super: a quarter of the dependent launches
+// of the 64-column steps), with their superblock inverses formed at the end
+// of the factorization; set per handle by mfd_set_solve_block (0: never).
+constexpr int kSuperMin = 512;
+
 double now_ms()
 {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -116,6 +122,7 @@ struct mfd_handle {
     int fuse_cmax = 0;
     std::vector<int> front_kind;
     int wide_min_rows = kWideMinRows;  // Schur tiles of 128 x 128 from this many update rows (0: never)
+    int super_min = kSuperMin;         // forward superblocks from this many pivot columns (0: never)
     bool device_ok = false;
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;  // large in-panel updates beside the diagonal chain
@@ -167,6 +174,17 @@ struct mfd_handle {
     std::vector<Launch> fwd_seq, bwd_seq;
     int64_t part_slots = 0;
     SolveTask *d_fstep = nullptr, *d_bstep = nullptr;
+    // superblock forward steps (kind kFX) and the blocks of the superblock
+    // matrices: X by block distance i - j ([xinv_ptr[d], xinv_ptr[d + 1]),
+    // d = 0..3), then W ([xinv_ptr[4], xinv_ptr[5]))
+    std::vector<SuperTask> xstep;
+    std::vector<SuperInvTask> xinv;
+    int xinv_ptr[6] = {};
+    int64_t sinv_elems = 0;
+    SuperTask* d_xstep = nullptr;
+    SuperInvTask* d_xinv = nullptr;
+    double* d_sinv64 = nullptr;
+    float* d_sinv32 = nullptr;
     double* d_part = nullptr;
     int64_t pool_elems = 0, z_per_rhs = 0;
     int num_levels = 0;
@@ -291,6 +309,10 @@ void free_device(mfd_handle* h)
     h->d_gemm = nullptr;
     h->d_level_nodes = h->d_child_ptr = h->d_child_list = nullptr;
     h->d_fstep = h->d_bstep = nullptr;
+    h->d_xstep = nullptr;
+    h->d_xinv = nullptr;
+    h->d_sinv64 = nullptr;
+    h->d_sinv32 = nullptr;
     h->d_fi = h->d_fa = h->d_bg = nullptr;
     h->d_part = nullptr;
     h->d_dst = h->d_src = nullptr;
@@ -313,12 +335,15 @@ void free_device(mfd_handle* h)
 int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 // ------------------------------------------------------------- schedule
-enum SolveKind { kFI = 100, kFA, kFS, kBG, kBS };
+enum SolveKind { kFI = 100, kFA, kFS, kBG, kBS, kFX };
 
 // Solve schedule.  Forward, per level: a gather launch, then launches
 // t = 0 .. steps: the block task of block t (which first applies block t - 1
 // to block t) and the row chunks applying block t - 1 to the rows after
-// block t.
+// block t.  Fronts with at least super_min pivot columns step over
+// superblocks of 256 columns instead (kind kFX, launched beside step t's
+// kFS launch): the superblock task of superblock t and the row chunks
+// applying superblock t - 1 to the rows after it.
 // Backward, per level: a gather launch, then launches t = steps .. 0: the
 // block task of block t (for fronts that have it) and the row chunks forming
 // the partial sums block t - 1 needs from the rows after block t (launch
@@ -334,15 +359,55 @@ void build_solve_plan(mfd_handle* h)
     h->fwd_seq.clear();
     h->bwd_seq.clear();
     h->part_slots = 0;
+    h->xstep.clear();
+    h->xinv.clear();
+    h->sinv_elems = 0;
     auto push = [](std::vector<Launch>& seq, int kind, int64_t first, int64_t end) {
         if (end > first) seq.push_back(Launch{kind, first, (int)(end - first)});
     };
+    auto super = [&](const NodeDev& d) { return h->super_min > 0 && d.c >= h->super_min; };
     auto steps_of = [&](int L) {
         int steps = 0;
         for (int k = h->level_ptr[L]; k < h->level_ptr[L + 1]; ++k)
             steps = std::max(steps, (h->nodes[h->level_nodes[k]].c + 63) / 64);
         return steps;
     };
+    auto fsteps_of = [&](int L) {
+        int steps = 0;
+        for (int k = h->level_ptr[L]; k < h->level_ptr[L + 1]; ++k) {
+            const NodeDev& d = h->nodes[h->level_nodes[k]];
+            steps = std::max(steps, (d.c + (super(d) ? kSuper : 64) - 1) / (super(d) ? kSuper : 64));
+        }
+        return steps;
+    };
+    // superblock matrices (X and W, 2 x 256^2 entries per superblock): four
+    // launches of X blocks by block distance, then one of W blocks
+    const int64_t per_super = 2 * (int64_t)kSuper * kSuper;
+    std::vector<int64_t> xoff(h->nodes.size(), -1);
+    for (size_t s = 0; s < h->nodes.size(); ++s)
+        if (super(h->nodes[s])) {
+            xoff[s] = h->sinv_elems;
+            h->sinv_elems += (int64_t)((h->nodes[s].c + kSuper - 1) / kSuper) * per_super;
+        }
+    for (int dist = 0; dist <= 4; ++dist) {
+        h->xinv_ptr[dist] = (int)h->xinv.size();
+        for (size_t s = 0; s < h->nodes.size(); ++s) {
+            if (xoff[s] < 0) continue;
+            const int c = h->nodes[s].c;
+            for (int u = 0; u * kSuper < c; ++u) {
+                const int64_t xo = xoff[s] + u * per_super;
+                if (dist == 4) {
+                    if (u > 0)
+                        for (int i = 0; i < 4; ++i)
+                            for (int j = 0; j < 4; ++j) h->xinv.push_back(SuperInvTask{(int)s, u, i, 4 + j, xo});
+                    continue;
+                }
+                for (int i = dist; i < 4; ++i)
+                    if (dist == 0 || u * kSuper + 64 * i < c) h->xinv.push_back(SuperInvTask{(int)s, u, i, i - dist, xo});
+            }
+        }
+    }
+    h->xinv_ptr[5] = (int)h->xinv.size();
     for (int L = 0; L < h->num_levels; ++L) {
         const int b0 = h->level_ptr[L], b1 = h->level_ptr[L + 1];
         {
@@ -369,12 +434,28 @@ void build_solve_plan(mfd_handle* h)
                 push(h->fwd_seq, kFA, f, (int64_t)h->fa.size());
             }
         }
-        const int steps = steps_of(L);
+        const int steps = fsteps_of(L);
         for (int t = 0; t <= steps; ++t) {
-            int64_t f0 = (int64_t)h->fstep.size();
+            int64_t f0 = (int64_t)h->fstep.size(), x0 = (int64_t)h->xstep.size();
             for (int k = b0; k < b1; ++k) {
                 const int s = h->level_nodes[k];
                 const NodeDev& d = h->nodes[s];
+                if (super(d)) {
+                    const int U = (d.c + kSuper - 1) / kSuper;
+                    const int64_t xo = xoff[s] + t * per_super;
+                    if (t <= U - 1) {
+                        const int sb = std::min(kSuper, d.c - kSuper * t);
+                        for (int r0 = 0; r0 < sb; r0 += 16)
+                            h->xstep.push_back(SuperTask{s, t, r0, sb, t > 0 ? kSuper : 0, 1, xo});
+                    }
+                    if (t >= 1 && t - 1 <= U - 1) {
+                        // superblock t - 1 applied to the rows after superblock t
+                        const int pb = std::min(kSuper, d.c - kSuper * (t - 1));
+                        for (int r0 = std::min(kSuper * (t + 1), d.c); r0 < d.n; r0 += 16)
+                            h->xstep.push_back(SuperTask{s, t, r0, 0, pb, 0, xo});
+                    }
+                    continue;
+                }
                 const int T = (d.c + 63) / 64;
                 // a front of one block with few update rows: one task solves
                 // the block and applies it to those rows
@@ -391,6 +472,7 @@ void build_solve_plan(mfd_handle* h)
                 }
             }
             push(h->fwd_seq, kFS, f0, (int64_t)h->fstep.size());
+            push(h->fwd_seq, kFX, x0, (int64_t)h->xstep.size());
         }
     }
     // chunks of the partial sums of block j of front s: rows [min(64 (j + 2), c), n)
@@ -821,6 +903,9 @@ int setup_device(mfd_handle* h)
     MFD_TRY(upload(h, &h->d_child_list, h->child_list));
     MFD_TRY(upload(h, &h->d_fstep, h->fstep));
     MFD_TRY(upload(h, &h->d_bstep, h->bstep));
+    MFD_TRY(upload(h, &h->d_xstep, h->xstep));
+    MFD_TRY(upload(h, &h->d_xinv, h->xinv));
+    if (h->sinv_elems > 0) MFD_TRY(dalloc(h, &h->d_sinv64, h->sinv_elems));
     MFD_TRY(upload(h, &h->d_fi, h->fi));
     MFD_TRY(upload(h, &h->d_fa, h->fa));
     MFD_TRY(upload(h, &h->d_bg, h->bg));
@@ -894,6 +979,9 @@ int ensure_rhs(mfd_handle* h, int nrhs)
 template <typename T> T* pool_of(mfd_handle* h);
 template <> double* pool_of<double>(mfd_handle* h) { return h->d_pool64; }
 template <> float* pool_of<float>(mfd_handle* h) { return h->d_pool32; }
+template <typename T> T* sinv_of(mfd_handle* h);
+template <> double* sinv_of<double>(mfd_handle* h) { return h->d_sinv64; }
+template <> float* sinv_of<float>(mfd_handle* h) { return h->d_sinv32; }
 
 template <typename T> void enqueue_launch(mfd_handle* h, const Launch& l, hipStream_t st)
 {
@@ -1001,6 +1089,10 @@ template <typename T> hipError_t enqueue_factor(mfd_handle* h, bool profile)
     }
     if (two && last_side > joined) ok(hipStreamWaitEvent(st, h->sev[last_side & 1], 0));
     for (; gnext < G; ++gnext) ok(hipStreamWaitEvent(st, h->fill_ev[gnext], 0));
+    // superblock inverses of the forward solve, after every front is final
+    for (int dist = 0; dist <= 4; ++dist)
+        launch_superinv<T>(pool, sinv_of<T>(h), h->d_nodes, h->d_xinv + h->xinv_ptr[dist],
+                           h->xinv_ptr[dist + 1] - h->xinv_ptr[dist], st);
     return err;
 }
 
@@ -1072,6 +1164,7 @@ int factorize_common(mfd_handle* h, int precision)
     int rc = MFD_OK;
     if (precision == MFD_FP16_IR || precision == MFD_BF16_IR) {
         if (!h->d_pool32) MFD_TRY(dalloc(h, &h->d_pool32, h->pool_elems));
+        if (!h->d_sinv32 && h->sinv_elems > 0) MFD_TRY(dalloc(h, &h->d_sinv32, h->sinv_elems));
         h->lowp = precision == MFD_FP16_IR ? 1 : 2;
         rc = run_factor<float>(h);
         h->lowp = 0;
@@ -1080,6 +1173,7 @@ int factorize_common(mfd_handle* h, int precision)
     }
     if (precision == MFD_FP32_IR) {
         if (!h->d_pool32) MFD_TRY(dalloc(h, &h->d_pool32, h->pool_elems));
+        if (!h->d_sinv32 && h->sinv_elems > 0) MFD_TRY(dalloc(h, &h->d_sinv32, h->sinv_elems));
         rc = run_factor<float>(h);
         // a pivot not positive in FP32: the FP64 refactorization the
         // refinement falls back to when it stalls; only an FP64 failure
@@ -1112,6 +1206,9 @@ template <typename T> void enqueue_solve(mfd_handle* h, const double* bp, double
         case kFI: launch_fwd_init(h->d_z, bp, n, nrhs, h->d_nodes, h->d_fi + l.first, l.count, st); break;
         case kFA: launch_fwd_add(h->d_z, nrhs, h->d_nodes, h->d_lidx, h->d_fa + l.first, l.count, st); break;
         case kFS: launch_fwd_step<T>(pool, h->d_z, nrhs, h->d_nodes, h->d_fstep + l.first, l.count, st); break;
+        case kFX:
+            launch_fwd_super<T>(pool, sinv_of<T>(h), h->d_z, nrhs, h->d_nodes, h->d_xstep + l.first, l.count, st);
+            break;
         default: break;
         }
     }
@@ -1943,6 +2040,15 @@ int mfd_set_tile_bound(mfd_handle* h, int min_rows)
     return replan(h);
 }
 int mfd_get_tile_bound(mfd_handle* h) { return h ? h->wide_min_rows : MFD_ERR_ARG; }
+
+int mfd_set_solve_block(mfd_handle* h, int min_c)
+{
+    if (!h || min_c < 0) return MFD_ERR_ARG;
+    if (min_c == h->super_min) return MFD_OK;
+    h->super_min = min_c;
+    return replan(h);
+}
+int mfd_get_solve_block(mfd_handle* h) { return h ? h->super_min : MFD_ERR_ARG; }
 
 int mfd_set_front_split(mfd_handle* h, int cmax)
 {

@@ -98,6 +98,8 @@ def lib():
             "mfd_set_front_split": (i, [P, i]),
             "mfd_set_tile_bound": (i, [P, i]),
             "mfd_get_tile_bound": (i, [P]),
+            "mfd_set_solve_block": (i, [P, i]),
+            "mfd_get_solve_block": (i, [P]),
             "mfd_set_front_kinds": (i, [P, ip]),
             "mfd_get_front_times": (i, [P, dp]),
             "mfd_check_schedule": (i, [P, i, lp, C.c_char_p, i]),
@@ -265,6 +267,16 @@ class Solver:
     @tile_bound.setter
     def tile_bound(self, min_rows):
         self._check(lib().mfd_set_tile_bound(self.h, int(min_rows)), "mfd_set_tile_bound")
+
+    @property
+    def solve_block(self):
+        """Pivot columns from which a front's forward solve steps over
+        256-column superblocks (0: 64-column blocks only)."""
+        return lib().mfd_get_solve_block(self.h)
+
+    @solve_block.setter
+    def solve_block(self, min_c):
+        self._check(lib().mfd_set_solve_block(self.h, int(min_c)), "mfd_set_solve_block")
 
     def set_front_split(self, cmax):
         """Fronts of order > 64 with at most cmax pivot columns are factored
