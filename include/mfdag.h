@@ -298,11 +298,12 @@ int mfd_get_front_times(mfd_handle* h, double* us);
 int mfd_set_tile_bound(mfd_handle* h, int min_rows);
 int mfd_get_tile_bound(mfd_handle* h);
 
-/* Block width of the forward solve (no counterpart in the reference, whose
-   forward sweep is one DTRSV and one DGEMV per front,
-   seq_solve/seq_forward_m.f90:14-81): fronts with at least min_c pivot
-   columns step over 256-column superblocks through explicit inverses of
-   their 256 x 256 diagonal blocks (formed at the end of each
+/* Block width of the solves (no counterpart in the reference, whose sweeps
+   are one DTRSV and one DGEMV per front, seq_solve/seq_forward_m.f90:14-81,
+   seq_solve/seq_backward_m.f90:14-131): fronts with at least min_c pivot
+   columns step forward and backward over 256-column superblocks through
+   explicit inverses of their 256 x 256 diagonal blocks and the products
+   coupling neighbouring superblocks (formed at the end of each
    factorization), the others over 64-column blocks; 0: 64 columns only.
    Default 512.  Rebuilds the schedule like mfd_set_update_period (the
    factorization is lost). */

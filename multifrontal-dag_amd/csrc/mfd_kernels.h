@@ -53,14 +53,20 @@ void launch_fwd_add(double* zpool, int nrhs, const NodeDev* nodes, const int* li
 template <typename T>
 void launch_fwd_step(const T* pool, double* zpool, int nrhs, const NodeDev* nodes, const SolveTask* tasks, int count,
                      hipStream_t st);
-// 64 x 64 blocks of the superblock matrices (X by block distance, then W)
+// 64 x 64 blocks of the superblock matrices: X by block distance (wv =
+// false), then the W and V blocks (wv = true, MFMA tile products)
 template <typename T>
-void launch_superinv(const T* pool, T* sinv, const NodeDev* nodes, const SuperInvTask* tasks, int count,
+void launch_superinv(const T* pool, T* sinv, const NodeDev* nodes, const SuperInvTask* tasks, int count, bool wv,
                      hipStream_t st);
 // forward sweep step over 256-column superblocks
 template <typename T>
 void launch_fwd_super(const T* pool, const T* sinv, double* zpool, int nrhs, const NodeDev* nodes,
                       const SuperTask* tasks, int count, hipStream_t st);
+// backward top tasks over 256-column superblocks (the partial sums of the
+// rows after superblock u+1 come from k_bwd_step chunk tasks)
+template <typename T>
+void launch_bwd_super(const T* pool, const T* sinv, double* zpool, const double* part, double* x, int ldx, int nrhs,
+                      const NodeDev* nodes, const SuperTask* tasks, int count, hipStream_t st);
 void launch_bwd_gather(double* zpool, int nrhs, const NodeDev* nodes, const int* lidx, const int2* tasks, int count,
                        hipStream_t st);
 template <typename T>

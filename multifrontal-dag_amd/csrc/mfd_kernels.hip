@@ -73,8 +73,6 @@ template <> struct Mfma16<kBF16> {
     static __device__ f4 mma(op_t a, op_t b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
 };
 
-__device__ inline double dsqrt(double x) { return sqrt(x); }
-__device__ inline float dsqrt(float x) { return sqrtf(x); }
 
 // 1 / sqrt(d): the hardware estimate (v_rsq) refined by Newton-Raphson
 // steps r <- r + (r / 2) (1 - d r^2), each squaring the relative error (two
@@ -1416,15 +1414,18 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(const T* pool, double* zp
 }
 
 // ------------------------------------------------------------ superblocks
-// The two 256 x 256 matrices of superblock u (u >= 1 for W), column-major,
-// X at xoff and W at xoff + 256^2:
+// The three 256 x 256 matrices of superblock u, column-major, X at xoff, W
+// at xoff + 256^2 (u >= 1), V at xoff + 2 256^2 (superblock u+1 not empty):
 //   X = inverse of the superblock's diagonal block of L (zero past its
 //       width sb), by 64 x 64 blocks (i, j), j <= i: X_ii is the stored
 //       64 x 64 inverse (j = i; the row's blocks above the diagonal cleared,
 //       the whole row past sb) and X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj;
-//   W = X L[B_u, B_{u-1}] (j >= 4: block (i, j - 4)), the coupling of the
-//       superblock to the previous one, so that y_u = X z_u - W y_{u-1}.
-// One task per block, launched by block distance i - j (then the W blocks).
+//   W = X L[B_u, B_{u-1}] (4 <= j < 8: block (i, j - 4)), the coupling of
+//       the superblock to the previous one, so that y_u = X z_u - W y_{u-1};
+//   V = L[B_{u+1}, B_u] X (j >= 8: block (i, j - 8), rows of superblock
+//       u+1), so that x_u = X^T (y_u - q_u) - V^T x_{u+1}.
+// One task per block, launched by block distance i - j; the W and V blocks
+// follow in one launch of k_superwv.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_superinv(const T* pool, T* sinv, const NodeDev* nodes,
                                                        const SuperInvTask* tasks)
@@ -1451,21 +1452,12 @@ __global__ __launch_bounds__(kThreads) void k_superinv(const T* pool, T* sinv, c
     T acc[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) acc[q] = T(0);
-    const bool wblk = j >= 4;
-    const int cp = c0 - kSuper;  // previous superblock's first column
-    for (int k = wblk ? 0 : j; k < (wblk ? i + 1 : i); ++k) {
-        const int kbk = min(64, nd.c - (c0 + 64 * k));
+    for (int k = j; k < i; ++k) {
         for (int e = tid; e < 4096; e += kThreads) {
             const int rr = e & 63, m = e >> 6;
-            if (wblk) {
-                // A = X_ik, B = L[B_u block k, B_{u-1} block j - 4]
-                As[m][rr] = X[(64 * i + rr) + kSuper * (64 * k + m)];
-                Bs[m][rr] = rr < kbk ? pool[nd.front + (c0 + 64 * k + rr) + (int64_t)(cp + 64 * (j - 4) + m) * n] : T(0);
-            } else {
-                // A = L_ik, B = X_kj
-                As[m][rr] = rr < kbi ? pool[nd.front + (b0 + rr) + (int64_t)(c0 + 64 * k + m) * n] : T(0);
-                Bs[m][rr] = X[(64 * k + rr) + kSuper * (64 * j + m)];
-            }
+            // A = L_ik, B = X_kj
+            As[m][rr] = rr < kbi ? pool[nd.front + (b0 + rr) + (int64_t)(c0 + 64 * k + m) * n] : T(0);
+            Bs[m][rr] = X[(64 * k + rr) + kSuper * (64 * j + m)];
         }
         __syncthreads();
 #pragma unroll 4
@@ -1475,12 +1467,6 @@ __global__ __launch_bounds__(kThreads) void k_superinv(const T* pool, T* sinv, c
             for (int q = 0; q < 16; ++q) acc[q] += a * Bs[16 * cq + q][m];
         }
         __syncthreads();
-    }
-    if (wblk) {
-        T* W = X + kSuper * kSuper;
-#pragma unroll
-        for (int q = 0; q < 16; ++q) W[(64 * i + r) + kSuper * (64 * (j - 4) + 16 * cq + q)] = acc[q];
-        return;
     }
     // X_ij = -X_ii T1: As[m][rr] = X_ii(rr, m), Bs[col][m] = T1(m, col)
     for (int e = tid; e < 4096; e += kThreads) {
@@ -1501,6 +1487,42 @@ __global__ __launch_bounds__(kThreads) void k_superinv(const T* pool, T* sinv, c
     }
 #pragma unroll
     for (int q = 0; q < 16; ++q) X[(64 * i + r) + kSuper * (64 * j + 16 * cq + q)] = -out[q];
+}
+
+// The W and V blocks of the superblock matrices (tasks with j >= 4, see
+// k_superinv) as single MFMA tile products with K up to 256:
+//   W block (i, j - 4) = X[block row i, :] L[B_u, B_{u-1} block j - 4]
+//       (X is zero right of its diagonal block, so K = min(64 (i + 1), sb));
+//   V block (i, j - 8) = L[B_{u+1} block i, B_u from block j - 8]
+//       X[B_u from block j - 8, block j - 8] (X is zero above it).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_superwv(const T* pool, T* sinv, const NodeDev* nodes,
+                                                      const SuperInvTask* tasks)
+{
+    const SuperInvTask t = tasks[blockIdx.x];
+    const NodeDev nd = nodes[t.node];
+    T* X = sinv + t.xoff;
+    const int n = nd.n, c0 = kSuper * t.u, sb = min(kSuper, nd.c - c0), i = t.i;
+    const T* F = pool + nd.front;
+    __shared__ union {
+        TileLds<T> L;
+        T Cs[kTile][kTile + 1];
+    } u;
+    T* dst;
+    if (t.j >= 8) {
+        const int jv = t.j - 8, rv = c0 + kSuper + 64 * i;
+        const int m = max(0, min(64, nd.c - rv));
+        tile_product<T>(F + rv + (int64_t)(c0 + 64 * jv) * n, n, X + kSuper * 64 * jv + 64 * jv, kSuper, true, m, 64,
+                        kSuper - 64 * jv, u.L, u.Cs);
+        dst = X + 2 * kSuper * kSuper + 64 * i + kSuper * 64 * jv;
+    } else {
+        const int jw = t.j - 4, cp = c0 - kSuper;
+        tile_product<T>(X + 64 * i, kSuper, F + c0 + (int64_t)(cp + 64 * jw) * n, n, true, 64, 64, min(64 * (i + 1), sb),
+                        u.L, u.Cs);
+        dst = X + kSuper * kSuper + 64 * i + kSuper * 64 * jw;
+    }
+    const int row = threadIdx.x & 63;
+    for (int col = threadIdx.x >> 6; col < 64; col += 4) dst[row + kSuper * col] = u.Cs[col][row];
 }
 
 // Forward sweep over superblocks, one launch per superblock step u, every
@@ -1588,6 +1610,96 @@ __global__ __launch_bounds__(kThreads) void k_fwd_super(const T* pool, const T* 
     }
 }
 
+// Backward sweep over superblocks, top tasks (kind 2), 16 rows of
+// superblock u each, in the launch of backward step u (thread: row
+// tid % 16, entries 16 g .. 16 g + 15 of each 256-long operand, g = tid /
+// 16; the 16 partial sums per row summed through LDS):
+//   x_u = X^T (y_u - q_u) - V^T x_{u+1},
+// y_u from Y, q_u summed in a fixed order from the partial-sum slots that
+// the previous launch's chunk tasks wrote (the rows after superblock u+1),
+// x_{u+1} from Z (the pivot part of x of these fronts lives in Z, so that
+// no task of a launch reads what another writes); x_u goes to Z and to x.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_bwd_super(const T* pool, const T* sinv, double* zpool, const double* part,
+                                                        double* x, int ldx, int nrhs, const NodeDev* nodes,
+                                                        const SuperTask* tasks)
+{
+    const SuperTask t = tasks[blockIdx.x];
+    const NodeDev nd = nodes[t.node];
+    const int n = nd.n, c = nd.c, c0 = kSuper * t.u, cn = c0 + kSuper, tid = threadIdx.x;
+    const int i = tid & 15, g = tid >> 4;
+    double* Z = zpool + nd.z * nrhs;
+    const double* Y = Z + (int64_t)n * nrhs;
+    __shared__ double zs[kSuper][kRhsGroup + 1], xs[kSuper][kRhsGroup + 1];
+    __shared__ double red[16][16][kRhsGroup + 1];
+    const int pb = t.pb, sb = t.sb;
+    const T* X = sinv + t.xoff;
+    const T* V = X + 2 * kSuper * kSuper;
+    const int col = t.row0 + i;  // column of X and V: the row of x_u
+    T a[16], v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) a[q] = X[(16 * g + q) + kSuper * col];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = pb > 0 ? V[(16 * g + q) + kSuper * col] : T(0);
+    static_assert(kSuper == kThreads, "one superblock entry per thread");
+    const int m = tid;  // the entry of y_u, q_u and x_{u+1} this thread stages
+    const double* pp = part + (int64_t)(t.slot + t.nchunk * (m >> 6)) * 64 * nrhs + (m & 63);
+    const int64_t cs = (int64_t)64 * nrhs;  // stride between chunks
+    for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
+        const int nr = min(kRhsGroup, nrhs - r0);
+        // the chunks' partial sums two at a time, every load of a pair (all
+        // right-hand sides of the group) issued before the subtractions
+        double zv[kRhsGroup], xv[kRhsGroup];
+#pragma unroll
+        for (int rr = 0; rr < kRhsGroup; ++rr) {
+            zv[rr] = (rr < nr && m < sb) ? Y[c0 + m + (int64_t)(r0 + rr) * c] : 0.0;
+            xv[rr] = (rr < nr && m < pb) ? Z[cn + m + (int64_t)(r0 + rr) * n] : 0.0;
+        }
+        if (m < sb)
+            for (int ch = 0; ch < t.nchunk; ch += 2) {
+                const bool two = ch + 1 < t.nchunk;
+                double p0[kRhsGroup], p1[kRhsGroup];
+#pragma unroll
+                for (int rr = 0; rr < kRhsGroup; ++rr) {
+                    const double* q = pp + (int64_t)(r0 + rr) * 64 + ch * cs;
+                    p0[rr] = rr < nr ? q[0] : 0.0;
+                    p1[rr] = (rr < nr && two) ? q[cs] : 0.0;
+                }
+#pragma unroll
+                for (int rr = 0; rr < kRhsGroup; ++rr) zv[rr] = (zv[rr] - p0[rr]) - p1[rr];
+            }
+#pragma unroll
+        for (int rr = 0; rr < kRhsGroup; ++rr) {
+            zs[m][rr] = zv[rr];
+            xs[m][rr] = xv[rr];
+        }
+        __syncthreads();
+        double acc[kRhsGroup];
+#pragma unroll
+        for (int rr = 0; rr < kRhsGroup; ++rr) acc[rr] = 0.0;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const double av = (double)a[q], vv = (double)v[q];
+#pragma unroll
+            for (int rr = 0; rr < kRhsGroup; ++rr) acc[rr] += av * zs[16 * g + q][rr] - vv * xs[16 * g + q][rr];
+        }
+#pragma unroll
+        for (int rr = 0; rr < kRhsGroup; ++rr) red[g][i][rr] = acc[rr];
+        __syncthreads();
+        if (tid < 16 * kRhsGroup) {
+            const int i2 = tid & 15, rr = tid >> 4, row = t.row0 + i2;
+            double s = 0.0;
+#pragma unroll
+            for (int gg = 0; gg < 16; ++gg) s += red[gg][i2][rr];
+            if (rr < nr && row < sb) {
+                Z[c0 + row + (int64_t)(r0 + rr) * n] = s;
+                x[(int64_t)(nd.first + c0 + row) + (int64_t)(r0 + rr) * ldx] = s;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Backward, first launch of a level, task (front, first row): 256 rows of
 // the update part of Z take the parent's solution at the parent-front
 // positions -- pivot positions from the parent's Y, update positions from
@@ -1605,7 +1717,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_gather(double* zpool, int nrhs
     const double* Yp = Zp + (int64_t)pa.n * nrhs;
     const int p = lidx[nd.lidx + i] - 1;
     for (int r = 0; r < nrhs; ++r)
-        Z[(nd.c + i) + (int64_t)r * nd.n] = p < pa.c ? Yp[p + (int64_t)r * pa.c] : Zp[p + (int64_t)r * pa.n];
+        Z[(nd.c + i) + (int64_t)r * nd.n] = (p < pa.c && !pa.xz) ? Yp[p + (int64_t)r * pa.c] : Zp[p + (int64_t)r * pa.n];
 }
 
 // Backward sweep, one launch per block step t (DGEMV "N" + DTRSV "U","N",
@@ -1666,7 +1778,8 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zp
             const int nr = min(kRhsGroup, nrhs - r0);
 #pragma unroll
             for (int r = 0; r < kRhsGroup; ++r)
-                u.c.Xs[tid][r] = (live && r < nr) ? (row < c ? Y[row + (int64_t)(r0 + r) * c] : Z[row + (int64_t)(r0 + r) * n])
+                u.c.Xs[tid][r] = (live && r < nr) ? ((row < c && !nd.xz) ? Y[row + (int64_t)(r0 + r) * c]
+                                                                          : Z[row + (int64_t)(r0 + r) * n])
                                               : 0.0;
             acc_t acc = {0, 0, 0, 0};
             const int rc = lane & 15;
@@ -1717,7 +1830,8 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zp
             const int nr = min(kRhsGroup, nrhs - r0);
 #pragma unroll
             for (int r = 0; r < kRhsGroup; ++r)
-                u.c.Xs[tid][r] = (live && r < nr) ? (row < c ? Y[row + (int64_t)(r0 + r) * c] : Z[row + (int64_t)(r0 + r) * n])
+                u.c.Xs[tid][r] = (live && r < nr) ? ((row < c && !nd.xz) ? Y[row + (int64_t)(r0 + r) * c]
+                                                                          : Z[row + (int64_t)(r0 + r) * n])
                                               : 0.0;
 #pragma unroll
             for (int j = 0; j < 16; ++j) Lc[j][tid] = (double)l[j];
@@ -2114,10 +2228,14 @@ void launch_fwd_step(const T* pool, double* zpool, int nrhs, const NodeDev* node
     if (count > 0) hipLaunchKernelGGL(k_fwd_step<T>, dim3(count), dim3(kThreads), 0, st, pool, zpool, nrhs, nodes, tasks);
 }
 template <typename T>
-void launch_superinv(const T* pool, T* sinv, const NodeDev* nodes, const SuperInvTask* tasks, int count,
+void launch_superinv(const T* pool, T* sinv, const NodeDev* nodes, const SuperInvTask* tasks, int count, bool wv,
                      hipStream_t st)
 {
-    if (count > 0) hipLaunchKernelGGL(k_superinv<T>, dim3(count), dim3(kThreads), 0, st, pool, sinv, nodes, tasks);
+    if (count <= 0) return;
+    if (wv)
+        hipLaunchKernelGGL(k_superwv<T>, dim3(count), dim3(kThreads), 0, st, pool, sinv, nodes, tasks);
+    else
+        hipLaunchKernelGGL(k_superinv<T>, dim3(count), dim3(kThreads), 0, st, pool, sinv, nodes, tasks);
 }
 template <typename T>
 void launch_fwd_super(const T* pool, const T* sinv, double* zpool, int nrhs, const NodeDev* nodes,
@@ -2125,6 +2243,14 @@ void launch_fwd_super(const T* pool, const T* sinv, double* zpool, int nrhs, con
 {
     if (count > 0)
         hipLaunchKernelGGL(k_fwd_super<T>, dim3(count), dim3(kThreads), 0, st, pool, sinv, zpool, nrhs, nodes, tasks);
+}
+template <typename T>
+void launch_bwd_super(const T* pool, const T* sinv, double* zpool, const double* part, double* x, int ldx, int nrhs,
+                      const NodeDev* nodes, const SuperTask* tasks, int count, hipStream_t st)
+{
+    if (count > 0)
+        hipLaunchKernelGGL(k_bwd_super<T>, dim3(count), dim3(kThreads), 0, st, pool, sinv, zpool, part, x, ldx, nrhs,
+                           nodes, tasks);
 }
 void launch_bwd_gather(double* zpool, int nrhs, const NodeDev* nodes, const int* lidx, const int2* tasks, int count,
                        hipStream_t st)
@@ -2170,9 +2296,11 @@ void launch_scatter_rows(double* out, int ld_out, const double* in, int ld_in, c
     template void launch_scatter_a_scaled<T>(T*, const double*, const int64_t*, const int64_t*, const int2*,           \
                                              const double*, int64_t, hipStream_t);                                    \
     template void launch_fwd_step<T>(const T*, double*, int, const NodeDev*, const SolveTask*, int, hipStream_t);       \
-    template void launch_superinv<T>(const T*, T*, const NodeDev*, const SuperInvTask*, int, hipStream_t);            \
+    template void launch_superinv<T>(const T*, T*, const NodeDev*, const SuperInvTask*, int, bool, hipStream_t);      \
     template void launch_fwd_super<T>(const T*, const T*, double*, int, const NodeDev*, const SuperTask*, int,         \
                                       hipStream_t);                                                                    \
+    template void launch_bwd_super<T>(const T*, const T*, double*, const double*, double*, int, int, const NodeDev*,    \
+                                      const SuperTask*, int, hipStream_t);                                             \
     template void launch_bwd_step<T>(const T*, double*, double*, double*, int, int, const NodeDev*, const SolveTask*,  \
                                      int, hipStream_t);
 MFD_INSTANTIATE(double)

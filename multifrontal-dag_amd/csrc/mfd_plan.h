@@ -23,7 +23,9 @@ struct NodeDev {
     int n, c, w;
     int parent;      // 0-based parent node, -1 for a root
     int first;       // 0-based first global (postordered) column
-    int pad[3];
+    int xz;          // 1: the backward solve keeps the pivot part of x in Z
+                     // (fronts solved over superblocks), 0: in Y
+    int pad[2];
 };
 
 // Extend-add of one 64 x 64 tile (ti >= tj) of child's update matrix.
@@ -58,25 +60,33 @@ struct SolveTask {
     int slot, nchunk, pad0, pad;
 };
 
-// Superblocks: the forward solve of a front with many pivot columns steps
-// over 256-column superblocks u with two 256 x 256 matrices per superblock
-// (column-major, in a separate array at xoff): X, the inverse of the
-// superblock's diagonal block (zero past its width), and at xoff + 256^2
-// W = X L[B_u, B_{u-1}] (u >= 1).
+// Superblocks: the solves of a front with many pivot columns step over
+// 256-column superblocks u (columns B_u) with three 256 x 256 matrices per
+// superblock (column-major, in a separate array at xoff): X, the inverse of
+// the superblock's diagonal block (zero past its width); at xoff + 256^2
+// W = X L[B_u, B_{u-1}] (u >= 1, forward); at xoff + 2 256^2
+// V = L[B_{u+1}, B_u] X (B_{u+1} not empty, backward).
 constexpr int kSuper = 256;
+constexpr int64_t kSuperMats = 3 * (int64_t)kSuper * kSuper;  // entries per superblock
 // Forward task of superblock step u, 16 rows: kind 1, rows row0 .. row0 +
 // 15 of superblock u (width sb; pb the width of superblock u-1, 0 for
 // u = 0); kind 0, front rows row0 .. row0 + 15 after superblock u, to which
 // superblock u-1 (width pb) is applied.
+// Backward task (kind 2), rows row0 .. row0 + 15 of superblock u (width
+// sb; pb the width of superblock u+1, 0 if none): x_u = X^T (y_u - q) -
+// V^T x_{u+1}, with q summed from the partial-sum slots slot + nchunk j +
+// ch (64 columns of block j of the superblock, chunk ch < nchunk).
 struct SuperTask {
     int node, u, row0, sb;
     int pb, kind;
+    int slot, nchunk;
     int64_t xoff;
 };
 // One 64 x 64 block (i, j) of superblock u's matrices: j = i copies the
 // stored 64 x 64 inverse (and clears the row's blocks above the diagonal,
 // or the whole row past the superblock's width); j < i forms
-// X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj; j >= 4 forms block (i, j - 4) of W.
+// X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj; 4 <= j < 8 forms block (i, j - 4)
+// of W; j >= 8 block (i, j - 8) of V.
 struct SuperInvTask {
     int node, u, i, j;
     int64_t xoff;

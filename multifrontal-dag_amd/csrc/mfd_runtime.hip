@@ -174,14 +174,15 @@ struct mfd_handle {
     std::vector<Launch> fwd_seq, bwd_seq;
     int64_t part_slots = 0;
     SolveTask *d_fstep = nullptr, *d_bstep = nullptr;
-    // superblock forward steps (kind kFX) and the blocks of the superblock
-    // matrices: X by block distance i - j ([xinv_ptr[d], xinv_ptr[d + 1]),
-    // d = 0..3), then W ([xinv_ptr[4], xinv_ptr[5]))
-    std::vector<SuperTask> xstep;
+    // superblock forward steps (kind kFX), backward top tasks (kind kBX)
+    // and the blocks of the superblock matrices: X by block distance i - j
+    // ([xinv_ptr[d], xinv_ptr[d + 1]), d = 0..3), then W and V
+    // ([xinv_ptr[4], xinv_ptr[5]))
+    std::vector<SuperTask> xstep, xbstep;
     std::vector<SuperInvTask> xinv;
     int xinv_ptr[6] = {};
     int64_t sinv_elems = 0;
-    SuperTask* d_xstep = nullptr;
+    SuperTask *d_xstep = nullptr, *d_xbstep = nullptr;
     SuperInvTask* d_xinv = nullptr;
     double* d_sinv64 = nullptr;
     float* d_sinv32 = nullptr;
@@ -309,7 +310,7 @@ void free_device(mfd_handle* h)
     h->d_gemm = nullptr;
     h->d_level_nodes = h->d_child_ptr = h->d_child_list = nullptr;
     h->d_fstep = h->d_bstep = nullptr;
-    h->d_xstep = nullptr;
+    h->d_xstep = h->d_xbstep = nullptr;
     h->d_xinv = nullptr;
     h->d_sinv64 = nullptr;
     h->d_sinv32 = nullptr;
@@ -335,7 +336,7 @@ void free_device(mfd_handle* h)
 int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 // ------------------------------------------------------------- schedule
-enum SolveKind { kFI = 100, kFA, kFS, kBG, kBS, kFX };
+enum SolveKind { kFI = 100, kFA, kFS, kBG, kBS, kFX, kBX };
 
 // Solve schedule.  Forward, per level: a gather launch, then launches
 // t = 0 .. steps: the block task of block t (which first applies block t - 1
@@ -348,7 +349,11 @@ enum SolveKind { kFI = 100, kFA, kFS, kBG, kBS, kFX };
 // block task of block t (for fronts that have it) and the row chunks forming
 // the partial sums block t - 1 needs from the rows after block t (launch
 // t = steps only has chunks).  Partials of block j live in bank j % 2, so a
-// launch never overwrites the partials its block tasks read.
+// launch never overwrites the partials its block tasks read.  Superblock
+// fronts step 256 columns at a time here too: the top tasks of superblock t
+// (kind kBX, after step t's kBS launch) and, in the kBS launch, chunk tasks
+// forming the partial sums of superblock t - 1's four 64-column blocks from
+// the rows after superblock t; their pivot part of x lives in Z (NodeDev::xz).
 void build_solve_plan(mfd_handle* h)
 {
     h->fstep.clear();
@@ -360,18 +365,15 @@ void build_solve_plan(mfd_handle* h)
     h->bwd_seq.clear();
     h->part_slots = 0;
     h->xstep.clear();
+    h->xbstep.clear();
     h->xinv.clear();
     h->sinv_elems = 0;
     auto push = [](std::vector<Launch>& seq, int kind, int64_t first, int64_t end) {
         if (end > first) seq.push_back(Launch{kind, first, (int)(end - first)});
     };
     auto super = [&](const NodeDev& d) { return h->super_min > 0 && d.c >= h->super_min; };
-    auto steps_of = [&](int L) {
-        int steps = 0;
-        for (int k = h->level_ptr[L]; k < h->level_ptr[L + 1]; ++k)
-            steps = std::max(steps, (h->nodes[h->level_nodes[k]].c + 63) / 64);
-        return steps;
-    };
+    // steps of a level when superblock fronts step 256 columns at a time
+    // (forward and backward alike)
     auto fsteps_of = [&](int L) {
         int steps = 0;
         for (int k = h->level_ptr[L]; k < h->level_ptr[L + 1]; ++k) {
@@ -380,9 +382,11 @@ void build_solve_plan(mfd_handle* h)
         }
         return steps;
     };
-    // superblock matrices (X and W, 2 x 256^2 entries per superblock): four
-    // launches of X blocks by block distance, then one of W blocks
-    const int64_t per_super = 2 * (int64_t)kSuper * kSuper;
+    for (auto& d : h->nodes) d.xz = super(d) ? 1 : 0;
+    // superblock matrices (X, W and V, 3 x 256^2 entries per superblock):
+    // four launches of X blocks by block distance, then one of W and V
+    // blocks
+    const int64_t per_super = kSuperMats;
     std::vector<int64_t> xoff(h->nodes.size(), -1);
     for (size_t s = 0; s < h->nodes.size(); ++s)
         if (super(h->nodes[s])) {
@@ -400,6 +404,9 @@ void build_solve_plan(mfd_handle* h)
                     if (u > 0)
                         for (int i = 0; i < 4; ++i)
                             for (int j = 0; j < 4; ++j) h->xinv.push_back(SuperInvTask{(int)s, u, i, 4 + j, xo});
+                    if ((u + 1) * kSuper < c)
+                        for (int i = 0; i < 4; ++i)
+                            for (int j = 0; j < 4; ++j) h->xinv.push_back(SuperInvTask{(int)s, u, i, 8 + j, xo});
                     continue;
                 }
                 for (int i = dist; i < 4; ++i)
@@ -446,13 +453,13 @@ void build_solve_plan(mfd_handle* h)
                     if (t <= U - 1) {
                         const int sb = std::min(kSuper, d.c - kSuper * t);
                         for (int r0 = 0; r0 < sb; r0 += 16)
-                            h->xstep.push_back(SuperTask{s, t, r0, sb, t > 0 ? kSuper : 0, 1, xo});
+                            h->xstep.push_back(SuperTask{s, t, r0, sb, t > 0 ? kSuper : 0, 1, 0, 0, xo});
                     }
                     if (t >= 1 && t - 1 <= U - 1) {
                         // superblock t - 1 applied to the rows after superblock t
                         const int pb = std::min(kSuper, d.c - kSuper * (t - 1));
                         for (int r0 = std::min(kSuper * (t + 1), d.c); r0 < d.n; r0 += 16)
-                            h->xstep.push_back(SuperTask{s, t, r0, 0, pb, 0, xo});
+                            h->xstep.push_back(SuperTask{s, t, r0, 0, pb, 0, 0, 0, xo});
                     }
                     continue;
                 }
@@ -480,16 +487,27 @@ void build_solve_plan(mfd_handle* h)
         const int start = std::min(64 * (j + 2), d.c);
         return std::make_pair(start, (d.n - start + 255) / 256);
     };
-    auto blocks = [&](const NodeDev& d) { return (d.c + 63) / 64; };
+    // superblock fronts: the partial sums of superblock w, rows
+    // [min(256 (w + 2), c), n), for each of its (at most four) 64-column
+    // blocks
+    auto super_chunk_rows = [&](const NodeDev& d, int w) {
+        const int start = std::min(kSuper * (w + 2), d.c);
+        return std::make_pair(start, (d.n - start + 255) / 256);
+    };
+    auto super_blocks = [&](const NodeDev& d, int w) { return std::min(4, (d.c - kSuper * w + 63) / 64); };
+    auto blocks = [&](const NodeDev& d) { return super(d) ? (d.c + kSuper - 1) / kSuper : (d.c + 63) / 64; };
+    // slots written by the chunk tasks of backward step t of front d
+    auto step_slots = [&](const NodeDev& d, int t) -> int64_t {
+        if (t < 1 || t - 1 >= blocks(d)) return 0;
+        if (super(d)) return (int64_t)super_blocks(d, t - 1) * super_chunk_rows(d, t - 1).second;
+        return std::max(0, chunk_rows(d, t - 1).second);
+    };
     int64_t bank = 1;  // slots per bank: the most chunks any launch has
     for (int L = 0; L < h->num_levels; ++L) {
         const int b0 = h->level_ptr[L], b1 = h->level_ptr[L + 1];
-        for (int t = steps_of(L); t >= 1; --t) {
+        for (int t = fsteps_of(L); t >= 1; --t) {
             int64_t cnt = 0;
-            for (int k = b0; k < b1; ++k) {
-                const NodeDev& d = h->nodes[h->level_nodes[k]];
-                if (t - 1 < blocks(d)) cnt += std::max(0, chunk_rows(d, t - 1).second);
-            }
+            for (int k = b0; k < b1; ++k) cnt += step_slots(h->nodes[h->level_nodes[k]], t);
             bank = std::max(bank, cnt);
         }
     }
@@ -505,21 +523,44 @@ void build_solve_plan(mfd_handle* h)
             }
             push(h->bwd_seq, kBG, f, (int64_t)h->bg.size());
         }
-        for (int t = steps_of(L); t >= 0; --t) {
-            int64_t f0 = (int64_t)h->bstep.size();
+        for (int t = fsteps_of(L); t >= 0; --t) {
+            int64_t f0 = (int64_t)h->bstep.size(), x0 = (int64_t)h->xbstep.size();
             int idx = 0;
             // few chunks (latency-bound steps): four 16-column tasks per
             // chunk; many: one task per chunk
             int64_t nch = 0;
-            for (int k = b0; k < b1; ++k) {
-                const NodeDev& d = h->nodes[h->level_nodes[k]];
-                if (t >= 1 && t - 1 <= blocks(d) - 1) nch += chunk_rows(d, t - 1).second;
-            }
+            for (int k = b0; k < b1; ++k) nch += step_slots(h->nodes[h->level_nodes[k]], t);
             const bool split = nch <= kSplitChunks;
             for (int k = b0; k < b1; ++k) {
                 const int s = h->level_nodes[k];
                 const NodeDev& d = h->nodes[s];
                 const int T = blocks(d);
+                if (super(d)) {
+                    const int64_t xo = xoff[s] + (int64_t)t * per_super;
+                    if (t <= T - 1) {
+                        // superblock t from the partial sums of the previous
+                        // launch and x of superblock t + 1
+                        const int sb = std::min(kSuper, d.c - kSuper * t);
+                        const int pb = t + 1 <= T - 1 ? std::min(kSuper, d.c - kSuper * (t + 1)) : 0;
+                        for (int r0 = 0; r0 < sb; r0 += 16)
+                            h->xbstep.push_back(SuperTask{s, t, r0, sb, pb, 2, qfirst[s], qcount[s], xo});
+                    }
+                    if (t >= 1 && t - 1 <= T - 1) {
+                        // partial sums of superblock t - 1 over the rows after superblock t
+                        const int w = t - 1;
+                        const auto rc = super_chunk_rows(d, w);
+                        qfirst[s] = (int)((w % 2) * bank + idx);
+                        qcount[s] = rc.second;
+                        for (int j = 0; j < super_blocks(d, w); ++j) {
+                            const int k0 = kSuper * w + 64 * j, kb = std::min(64, d.c - k0);
+                            for (int q = 0; q < rc.second; ++q, ++idx)
+                                for (int cq = 0; cq < (split ? 4 : 1); ++cq)
+                                    h->bstep.push_back(SolveTask{s, k0, kb, rc.first + 256 * q,
+                                                                 (int)((w % 2) * bank + idx), 0, split ? cq : -1, 0});
+                        }
+                    }
+                    continue;
+                }
                 if (t <= T - 1) {
                     const int k0 = 64 * t, kb = std::min(64, d.c - k0);
                     const int kn = t + 1 <= T - 1 ? std::min(64, d.c - k0 - 64) : 0;
@@ -537,6 +578,7 @@ void build_solve_plan(mfd_handle* h)
                 }
             }
             push(h->bwd_seq, kBS, f0, (int64_t)h->bstep.size());
+            push(h->bwd_seq, kBX, x0, (int64_t)h->xbstep.size());
         }
     }
     h->part_slots = 2 * bank;
@@ -904,6 +946,7 @@ int setup_device(mfd_handle* h)
     MFD_TRY(upload(h, &h->d_fstep, h->fstep));
     MFD_TRY(upload(h, &h->d_bstep, h->bstep));
     MFD_TRY(upload(h, &h->d_xstep, h->xstep));
+    MFD_TRY(upload(h, &h->d_xbstep, h->xbstep));
     MFD_TRY(upload(h, &h->d_xinv, h->xinv));
     if (h->sinv_elems > 0) MFD_TRY(dalloc(h, &h->d_sinv64, h->sinv_elems));
     MFD_TRY(upload(h, &h->d_fi, h->fi));
@@ -1092,7 +1135,7 @@ template <typename T> hipError_t enqueue_factor(mfd_handle* h, bool profile)
     // superblock inverses of the forward solve, after every front is final
     for (int dist = 0; dist <= 4; ++dist)
         launch_superinv<T>(pool, sinv_of<T>(h), h->d_nodes, h->d_xinv + h->xinv_ptr[dist],
-                           h->xinv_ptr[dist + 1] - h->xinv_ptr[dist], st);
+                           h->xinv_ptr[dist + 1] - h->xinv_ptr[dist], dist == 4, st);
     return err;
 }
 
@@ -1218,6 +1261,10 @@ template <typename T> void enqueue_solve(mfd_handle* h, const double* bp, double
         case kBG: launch_bwd_gather(h->d_z, nrhs, h->d_nodes, h->d_lidx, h->d_bg + l.first, l.count, st); break;
         case kBS:
             launch_bwd_step<T>(pool, h->d_z, h->d_part, xp, n, nrhs, h->d_nodes, h->d_bstep + l.first, l.count, st);
+            break;
+        case kBX:
+            launch_bwd_super<T>(pool, sinv_of<T>(h), h->d_z, h->d_part, xp, n, nrhs, h->d_nodes, h->d_xbstep + l.first,
+                                l.count, st);
             break;
         default: break;
         }

@@ -1,8 +1,8 @@
-"""Forward solve over 256-column superblocks (k_superinv, k_fwd_super),
-chosen per front by mfd_set_solve_block.  No counterpart in the reference
-(its forward sweep is one DTRSV and one DGEMV per front,
-seq_solve/seq_forward_m.f90:14-81); the solution it produces is the
-reference's.
+"""Forward and backward solves over 256-column superblocks (k_superinv,
+k_fwd_super, k_bwd_super), chosen per front by mfd_set_solve_block.  No
+counterpart in the reference (its sweeps are one DTRSV and one DGEMV per
+front, seq_solve/seq_forward_m.f90:14-81, seq_solve/seq_backward_m.f90:
+14-131); the solution it produces is the reference's.
 
 CPU: the setting's argument checks (no compute without a GPU).
 GPU: with a bound low enough that fronts of one partial superblock (with and
