@@ -207,6 +207,57 @@ def nd_leg(args, p, b):
     return out
 
 
+def hybrid_leg(args, p, b):
+    """CPU+GPU hybrid on the headline workload: the subtree partition
+    (mfd_tree_pruning, the reference's tree_pruning) into --hybrid-bins bins,
+    whole bins (lightest first) given to the host up to --hybrid-share of the
+    cost formula's flops, factored by --hybrid-threads host threads while the
+    GPU factors the rest (mfd_set_host_subtrees); FP64, timed like the main
+    leg (the hybrid's factorization is not replayed from a graph)."""
+    import torch
+    import mfdag
+    from mfdag import problems as P
+    s = mfdag.Solver(nb=args.nb, graph=not args.no_graph)
+    s.analyze(p.ccs_col, p.ccs_row, args.max_zero, p.perm)
+    st = s.structure()
+    c, w = st["supernode_size"].astype(np.float64), st["work_size"].astype(np.float64)
+    sub, bins = mfdag.tree_pruning(st["supernode_size"], st["work_size"], st["parent"], 0.5, args.hybrid_bins)
+    parent = st["parent"]
+    # flops of each subtree by the cost formula
+    node_f = np.array([sum(float(t) * t for t in range(int(w[k]) + 1, int(w[k] + c[k]) + 1)) for k in range(len(c))])
+    sub_f = node_f.copy()
+    for k in range(len(c)):  # parents follow their children
+        if parent[k] > 0:
+            sub_f[parent[k] - 1] += sub_f[k]
+    bin_f = np.zeros(bins.max() + 1)
+    np.add.at(bin_f, bins, sub_f[sub - 1])
+    cost = s.cost()
+    roots, share = [], 0.0
+    for k in np.argsort(bin_f):
+        if share + bin_f[k] > args.hybrid_share * cost and roots:
+            break
+        roots += list(sub[bins == k] - 1)
+        share += bin_f[k]
+    s.set_host_subtrees(np.sort(np.array(roots)), threads=args.hybrid_threads)
+    dev = torch.device("cuda:0")
+    d_a = torch.from_numpy(np.ascontiguousarray(p.val)).to(dev)
+    d_b = torch.from_numpy(np.ascontiguousarray(b.T)).to(dev)
+    d_x = torch.empty_like(d_b)
+    torch.cuda.synchronize()
+    wall, f, so = timed_steps(s, d_a, d_b, d_x, mfdag.FP64, args.steps, args.warmup)
+    x = d_x.cpu().numpy().T
+    ms, nfr, hfl = s.host_part()
+    out = {"bins": args.hybrid_bins, "host_subtrees": len(roots), "host_fronts": nfr,
+           "host_flop_share": round(hfl / cost, 5), "host_threads": args.hybrid_threads,
+           "host_ms": round(ms, 3), "host_gflops": round(hfl / (ms * 1e6), 2) if ms > 0 else None,
+           "ms_per_step": round(wall * 1e3, 3), "factor_ms": round(f, 3), "solve_ms": round(so, 3),
+           "backward_error": P.backward_error(p, x, b)}
+    log(f"hybrid: {len(roots)} host subtrees ({nfr} fronts, {hfl / cost:.4f} of the flops, host {ms:.1f} ms); "
+        f"step {wall * 1e3:.2f} ms (factor {f:.2f} ms, solve {so:.2f} ms), eta {out['backward_error']:.2e}")
+    s.close()
+    return out
+
+
 def roofline(p, b, max_zero, nb, precision):
     """One profiled factorization + solve: per-class time (HIP events on the
     library's stream, one pair per launch; the solve sweeps as a whole)
@@ -344,10 +395,16 @@ def main():
     ap.add_argument("--solve-block", type=int, default=-1, metavar="COLS",
                     help="forward solve over 256-column superblocks for fronts with at least COLS pivot columns "
                          "(0: never; -1: the library default)")
+    ap.add_argument("--hybrid-bins", type=int, default=64,
+                    help="bins of the subtree partition in the CPU+GPU hybrid leg (0: skip the leg)")
+    ap.add_argument("--hybrid-share", type=float, default=0.001,
+                    help="most flops (fraction of the cost) the hybrid leg gives to the host")
+    ap.add_argument("--hybrid-threads", type=int, default=8, help="host threads of the hybrid leg")
     ap.add_argument("--front-split", type=int, default=-1, metavar="CMAX",
                     help="medium fronts: those with at most CMAX pivot columns factored by one workgroup each "
                          "(-1: the library default)")
     ap.add_argument("--c5-only", action="store_true", help="run only the low-precision block-stencil leg")
+    ap.add_argument("--hybrid-only", action="store_true", help="run only the CPU+GPU hybrid leg")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default update period instead of the exhaustive tuner's choice")
     ap.add_argument("--roofline-only", action="store_true",
@@ -381,6 +438,9 @@ def main():
         print(json.dumps({"c5": c5_leg(args)}), flush=True)
         return
     p, b = build_problem(args.size, args.seed)
+    if args.hybrid_only:
+        print(json.dumps({"hybrid": hybrid_leg(args, p, b)}), flush=True)
+        return
     if args.roofline_only:
         roof, classes = roofline(p, b, args.max_zero, args.nb, mfdag.FP64)
         print(json.dumps({"roofline": roof, "kernel_classes_fp64": {
@@ -444,6 +504,7 @@ def main():
     log("kernel classes (FP64):", json.dumps({k: {kk: round(v, 4) for kk, v in r.items()} for k, r in classes.items()}))
 
     nd = None if args.no_nd_leg else nd_leg(args, p, b)
+    hybrid = hybrid_leg(args, p, b) if args.hybrid_bins > 0 else None
     batch = batch_leg(args) if args.batch > 0 else None
     c5 = c5_leg(args) if args.c5_size > 0 else None
     cpu = None
@@ -474,12 +535,15 @@ def main():
         "kernel_classes_fp64": {k: {"ms": round(r["ms"], 4), "launches": r["launches"]} for k, r in classes.items()},
         "cpu_baseline": cpu,
         "graph_nd": nd,
+        "hybrid": hybrid,
         "batch": batch,
         "c5": c5,
     }
     # every leg's solution must meet the backward-error bound the metric
     # names; a leg that misses it makes the whole line invalid
     etas = {"fp64": eta64, "fp32_ir": eta32}
+    if hybrid:
+        etas["hybrid.fp64"] = hybrid["backward_error"]
     for leg, d in (("graph_nd", nd), ("batch", batch), ("c5", c5)):
         for form in ("fp64", "fp32_ir", "fp16_ir", "bf16_ir"):
             if d and form in d:

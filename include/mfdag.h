@@ -310,6 +310,30 @@ int mfd_get_tile_bound(mfd_handle* h);
 int mfd_set_solve_block(mfd_handle* h, int min_c);
 int mfd_get_solve_block(mfd_handle* h);
 
+/* CPU+GPU hybrid (the reference's subtree partition, tree_pruning_m.f90:
+   13-73, which its main path does not call, put to work): the fronts of the
+   subtrees rooted at roots[0 .. count) (0-based node ids, e.g. the subtrees
+   mfd_tree_pruning deals to one bin) are assembled and factored by
+   `threads` host threads while the GPU factors the rest; their factors and
+   diagonal-block inverses are copied into device memory before the first
+   GPU launch of the lowest level holding a parent of one of the roots, so
+   the solves run on the GPU unchanged.  count = 0 clears the set.  A root
+   out of range, given twice or inside another root's subtree is
+   MFD_ERR_ARG.  While host subtrees are set, only MFD_FP64 factorizations
+   are accepted (others: MFD_ERR_UNSUPPORTED) and the factorization is
+   never replayed from a graph.  Rebuilds the schedule like
+   mfd_set_update_period (the factorization is lost). */
+int mfd_set_host_subtrees(mfd_handle* h, const int* roots, int count, int threads);
+/* The host part alone, without a GPU: assembles and factors the host
+   subtrees from the caller-order values a; mfd_export_front then returns
+   their fronts.  Returns 0 or the 1-based global column of the first pivot
+   that is not positive. */
+int mfd_factorize_host_subtrees(mfd_handle* h, const double* a);
+/* The host part of the last factorization: wall time in ms, number of
+   fronts and their flops by the reference's cost formula
+   (cost_calculator_m.f90:9-20); each pointer nullable. */
+int mfd_get_host_part(mfd_handle* h, double* ms, int* fronts, double* flops);
+
 /* Static check of the factorization schedule (no counterpart in the
    reference, whose StarPU runtime orders tasks by their data accesses):
    the launches of the current analysis on their streams (main, zeroing and

@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "lib", "libmfdag.so")
-SOURCES = ["mfd_analysis.cpp", "mfd_layout.cpp", "mfd_mmio.cpp", "mfd_ordering.cpp", "mfd_subtree.cpp",
+SOURCES = ["mfd_analysis.cpp", "mfd_layout.cpp", "mfd_mmio.cpp", "mfd_ordering.cpp", "mfd_subtree.cpp", "mfd_host.cpp",
            "mfd_kernels.hip", "mfd_runtime.hip", "mfd_mlp.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 instructions are disabled for all device code.
@@ -28,9 +28,16 @@ FLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=gfx950", "-Wall",
 HIP_FLAGS = ["-mllvm", "-amdgpu-mfma-vgpr-form"]
 
 
+# Host numeric code (the hybrid's subtree factorization) vectorized for
+# AVX2 + FMA, which every x86-64 host of an MI355X has.
+HOST_VECTOR = {"mfd_host.cpp"}
+
+
 def _compile(src):
     obj = os.path.join(OUT, os.path.splitext(src)[0] + ".o")
     lang = ["-x", "hip"] + HIP_FLAGS if src.endswith(".hip") else []
+    if src in HOST_VECTOR:
+        lang += ["-mavx2", "-mfma"]
     cmd = [HIPCC] + FLAGS + lang + ["-c", os.path.join(SRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     noise = "unknown target feature"

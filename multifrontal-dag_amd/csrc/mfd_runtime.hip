@@ -21,10 +21,12 @@
 #include <mutex>
 #include <set>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
 #include "mfd_analysis.h"
+#include "mfd_host.h"
 #include "mfd_kernels.h"
 #include "mfd_layout.h"
 #include "mfdag.h"
@@ -245,6 +247,24 @@ struct mfd_handle {
     mfd_stats st{};
     double ktime[MFD_K_NUM] = {0};
     int kcount[MFD_K_NUM] = {0};
+
+    // CPU+GPU hybrid: the subtrees rooted at host_roots (0-based) are
+    // assembled and factored by host threads (mfd_host.cpp) into hstage,
+    // which is copied into the pool before fseq[host_join] -- the first
+    // launch of the lowest level with a parent of a host subtree root.
+    std::vector<int> host_roots;
+    int host_threads = 1;
+    HostPlan hplan;
+    int host_rc = 0;                   // build_host_plan's result for the current schedule
+    int host_join = -1;
+    double* hstage = nullptr;          // hplan.stage_elems doubles (pinned with a device)
+    std::vector<double> hstage_vec;    // the staging array without a device
+    const double* host_vals = nullptr; // caller-order values for the host part
+    std::vector<double> host_vals_buf; // ... copied from the device for mfd_factorize_device
+    std::thread host_thr;
+    int host_info = 0;
+    bool host_factored = false;        // hstage holds the host fronts of the current values
+    double host_ms = 0;
 };
 
 namespace {
@@ -290,6 +310,10 @@ void drop_solve_graphs(mfd_handle* h)
 
 void free_device(mfd_handle* h)
 {
+    if (h->host_thr.joinable()) h->host_thr.join();
+    if (h->hstage) (void)hipHostFree(h->hstage);
+    h->hstage = nullptr;
+    h->host_factored = false;
     for (auto& g : h->graph)
         if (g) {
             (void)hipGraphExecDestroy(g);
@@ -611,6 +635,13 @@ void build_plan(mfd_handle* h)
     off += inv_blocks * kPanel * kPanel;
     h->pool_elems = off;
     h->z_per_rhs = zoff;
+    // subtrees factored on the host (CPU+GPU hybrid): their fronts get no
+    // zeroing, assembly or factorization launch; a host subtree root adds
+    // into its parent by the extend-add launch, like a small child
+    h->hplan = HostPlan{};
+    h->host_rc = h->host_roots.empty() ? MFD_OK : build_host_plan(S, h->nodes, h->host_roots, h->hplan);
+    if (h->host_rc != MFD_OK) h->hplan = HostPlan{};
+    auto on_host = [&](int s) { return !h->hplan.is_host.empty() && h->hplan.is_host[s]; };
 
     std::vector<int> lev(m, 0);
     for (int s = 0; s < m; ++s)
@@ -641,7 +672,7 @@ void build_plan(mfd_handle* h)
         h->fill_gptr.assign(1, 0);
         for (int g = 0; g < G; ++g) {
             for (int s2 = 0; s2 < m; ++s2)
-                if (group[s2] == g)
+                if (group[s2] == g && !on_host(s2))
                     for (int j0 = 0; j0 < h->nodes[s2].n; j0 += 64) h->fill.push_back(make_int2(s2, j0));
             h->fill_gptr.push_back((int)h->fill.size());
         }
@@ -664,10 +695,14 @@ void build_plan(mfd_handle* h)
     h->fused_pos.assign(m, -1);
     // front classes: small (n <= 64, whole in LDS), medium (one workgroup
     // per front), large (64-column steps over the chip)
-    enum { kClsSmall = 0, kClsMedium = 1, kClsLarge = 2 };
+    enum { kClsSmall = 0, kClsMedium = 1, kClsLarge = 2, kClsHost = 3 };
     std::vector<int> cls(m, kClsLarge);
     for (int s = 0; s < m; ++s) {
         const NodeDev& d = h->nodes[s];
+        if (on_host(s)) {
+            cls[s] = kClsHost;
+            continue;
+        }
         if (d.n <= kSmallFront) {
             cls[s] = kClsSmall;
             continue;
@@ -697,7 +732,7 @@ void build_plan(mfd_handle* h)
             int64_t first = (int64_t)h->ext.size();
             for (int k = 0; k < cnt; ++k) {
                 const auto& chs = S.children[lv[k]];
-                if (chs.size() <= r) continue;
+                if (chs.size() <= r || cls[lv[k]] == kClsHost) continue;
                 int ch = chs[r] - 1, nt = (S.w[ch] + kTile - 1) / kTile;
                 if (cls[ch] == kClsLarge) continue;
                 for (int ti = 0; ti < nt; ++ti)
@@ -877,6 +912,16 @@ void build_plan(mfd_handle* h)
             }
 
     }
+    // the host factors go into the pool before the first launch of the
+    // lowest level holding a parent of a host subtree root (at the end when
+    // no root has a parent)
+    h->host_join = -1;
+    if (!h->hplan.roots.empty()) {
+        int jl = h->num_levels;
+        for (int r : h->hplan.roots)
+            if (S.parent[r] > 0) jl = std::min(jl, lev[S.parent[r] - 1]);
+        h->host_join = jl < h->num_levels ? h->level_launch[jl] : (int)h->fseq.size();
+    }
     build_solve_plan(h);
 }
 
@@ -926,6 +971,9 @@ int setup_device(mfd_handle* h)
     free_device(h);
     const Structure& S = h->S;
     build_plan(h);
+    MFD_TRY(h->host_rc);
+    if (h->hplan.stage_elems > 0)
+        HIP_TRY(hipHostMalloc((void**)&h->hstage, h->hplan.stage_elems * sizeof(double), hipHostMallocDefault));
     MFD_TRY(upload(h, &h->d_nodes, h->nodes));
     MFD_TRY(upload(h, &h->d_lidx, S.lidx));
     MFD_TRY(upload(h, &h->d_ext, h->ext));
@@ -963,22 +1011,28 @@ int setup_device(mfd_handle* h)
         for (int s = 0; s < S.num_node; ++s) {
             if (h->front_group[s] != (int)g) continue;
             const NodeDev& d = h->nodes[s];
+            const bool host = !h->hplan.is_host.empty() && h->hplan.is_host[s];
             for (int j = S.first[s]; j < S.first[s + 1]; ++j) {
                 int64_t q = j - S.first[s];
-                for (int64_t p = S.a_colptr[j - 1]; p < S.a_colptr[j]; ++p, ++e) {
-                    dst[e] = d.front + (S.snidx[p] - 1) + q * d.n;
-                    src[e] = S.ccs_perm[p] - 1;
+                for (int64_t p = S.a_colptr[j - 1]; p < S.a_colptr[j]; ++p) {
                     // global row: a pivot row of this front or one of its update rows
                     const int r = S.snidx[p];
                     const int grow = r <= S.c[s] ? S.first[s] + r - 1 : S.l_rows[S.l_ptr[s] + (r - S.c[s] - 1)];
+                    if (grow == j) diag_src[j - 1] = S.ccs_perm[p] - 1;
+                    if (host) continue;  // assembled on the host
+                    dst[e] = d.front + (S.snidx[p] - 1) + q * d.n;
+                    src[e] = S.ccs_perm[p] - 1;
                     rc[e] = make_int2(grow - 1, j - 1);
-                    if (grow == j) diag_src[j - 1] = src[e];
+                    ++e;
                 }
             }
         }
         h->scat_gptr.push_back(e);
     }
-    if (e != nnz) return MFD_ERR_INTERNAL;
+    if (e != nnz - h->hplan.entries) return MFD_ERR_INTERNAL;
+    dst.resize(e);
+    src.resize(e);
+    rc.resize(e);
     MFD_TRY(upload(h, &h->d_dst, dst));
     MFD_TRY(upload(h, &h->d_src, src));
     MFD_TRY(upload(h, &h->d_scat_rc, rc));
@@ -1057,6 +1111,23 @@ template <typename T> void enqueue_launch(mfd_handle* h, const Launch& l, hipStr
     }
 }
 
+double* stage_of(mfd_handle* h) { return h->hstage ? h->hstage : h->hstage_vec.data(); }
+
+// CPU+GPU hybrid: waits for the host part and copies its fronts and their
+// diagonal-block inverses into the pool (one copy per run of consecutive
+// host fronts and one per run of their inverses).
+hipError_t host_join(mfd_handle* h, hipStream_t st)
+{
+    if (h->host_thr.joinable()) h->host_thr.join();
+    hipError_t err = hipSuccess;
+    for (const HostRun& r : h->hplan.runs) {
+        const hipError_t e = hipMemcpyAsync(h->d_pool64 + r.pool, stage_of(h) + r.stage, r.count * sizeof(double),
+                                            hipMemcpyHostToDevice, st);
+        if (e != hipSuccess && err == hipSuccess) err = e;
+    }
+    return err;
+}
+
 // The factorization schedule: reset the pivot flag, zero the front pool,
 // scatter A, then every level's launches.  No hipMemsetAsync, so the
 // sequence can be captured into a graph.
@@ -1107,9 +1178,15 @@ template <typename T> hipError_t enqueue_factor(mfd_handle* h, bool profile)
     // beside it; a step's launches join the side launch they depend on, and
     // any other launch joins every side launch before it.
     const bool two = !profile && h->side;
+    // the host subtrees' factors join before launch host_join (FP64 only)
+    bool hybrid = sizeof(T) == 8 && !h->hplan.roots.empty();
     int last_side = -1, joined = -1, gnext = split_fill ? 0 : G;
     for (size_t i = 0; i < h->fseq.size(); ++i) {
         const Launch& l = h->fseq[i];
+        if (hybrid && (int)i == h->host_join) {
+            ok(host_join(h, st));
+            hybrid = false;
+        }
         for (; gnext < G && h->level_launch[h->group_level[gnext]] <= (int)i; ++gnext)
             ok(hipStreamWaitEvent(st, h->fill_ev[gnext], 0));
         if (two && l.side) {
@@ -1132,7 +1209,8 @@ template <typename T> hipError_t enqueue_factor(mfd_handle* h, bool profile)
     }
     if (two && last_side > joined) ok(hipStreamWaitEvent(st, h->sev[last_side & 1], 0));
     for (; gnext < G; ++gnext) ok(hipStreamWaitEvent(st, h->fill_ev[gnext], 0));
-    // superblock inverses of the forward solve, after every front is final
+    if (hybrid) ok(host_join(h, st));
+    // superblock inverses of the solves, after every front is final
     for (int dist = 0; dist <= 4; ++dist)
         launch_superinv<T>(pool, sinv_of<T>(h), h->d_nodes, h->d_xinv + h->xinv_ptr[dist],
                            h->xinv_ptr[dist + 1] - h->xinv_ptr[dist], dist == 4, st);
@@ -1159,8 +1237,22 @@ void collect_profile(mfd_handle* h, const std::vector<int>& kinds)
 template <typename T> int run_factor(mfd_handle* h)
 {
     const bool profile = (h->flags & MFD_FLAG_PROFILE) != 0;
-    const bool graph = (h->flags & MFD_FLAG_GRAPH) != 0 && !profile && !h->no_alloc;
+    // the hybrid's host work sits between two parts of the enqueue, so it
+    // is never captured
+    const bool hybrid = sizeof(T) == 8 && !h->hplan.roots.empty();
+    const bool graph = (h->flags & MFD_FLAG_GRAPH) != 0 && !profile && !h->no_alloc && !hybrid;
     const int gi = sizeof(T) == 8 ? 0 : 1 + h->lowp;
+    if (hybrid) {
+        if (!h->host_vals) return MFD_ERR_STATE;
+        if (h->host_thr.joinable()) h->host_thr.join();
+        h->host_factored = false;
+        h->host_info = 0;
+        h->host_thr = std::thread([h]() {
+            const double t0 = now_ms();
+            h->host_info = host_factor(h->hplan, h->S, h->host_vals, stage_of(h), h->host_threads);
+            h->host_ms = now_ms() - t0;
+        });
+    }
     HIP_TRY(hipEventRecord(h->ev[0], h->stream));
     if (graph) {
         if (!h->graph[gi]) {
@@ -1194,6 +1286,11 @@ template <typename T> int run_factor(mfd_handle* h)
     h->st.factor_ms = ms;
     int info = INT_MAX;
     HIP_TRY(hipMemcpy(&info, h->d_info, sizeof(int), hipMemcpyDeviceToHost));
+    if (hybrid) {
+        if (h->host_thr.joinable()) h->host_thr.join();
+        h->host_factored = true;
+        if (h->host_info > 0) info = std::min(info, h->host_info);
+    }
     return info == INT_MAX ? MFD_OK : info;
 }
 
@@ -1202,6 +1299,7 @@ int factorize_common(mfd_handle* h, int precision)
     if (!h->analyzed) return MFD_ERR_STATE;
     if (!h->device_ok) return MFD_ERR_NODEVICE;
     if (precision < MFD_FP64 || precision > MFD_BF16_IR) return MFD_ERR_UNSUPPORTED;
+    if (!h->hplan.roots.empty() && precision != MFD_FP64) return MFD_ERR_UNSUPPORTED;
     h->factored = false;
     h->solved_nrhs = 0;
     int rc = MFD_OK;
@@ -2097,6 +2195,64 @@ int mfd_set_solve_block(mfd_handle* h, int min_c)
 }
 int mfd_get_solve_block(mfd_handle* h) { return h ? h->super_min : MFD_ERR_ARG; }
 
+int mfd_set_host_subtrees(mfd_handle* h, const int* roots, int count, int threads)
+{
+    if (!h || count < 0 || (count > 0 && !roots) || threads < 1) return MFD_ERR_ARG;
+    std::vector<int> r(roots, roots + count);
+    if (h->analyzed) {
+        // every root in range, none twice and none inside another's subtree
+        const int m = h->S.num_node;
+        std::vector<char> mark(m, 0);
+        for (int x : r) {
+            if (x < 0 || x >= m || mark[x]) return MFD_ERR_ARG;
+            mark[x] = 1;
+        }
+        for (int x : r)
+            for (int a = h->S.parent[x] - 1; a >= 0; a = h->S.parent[a] - 1)
+                if (mark[a]) return MFD_ERR_ARG;
+    }
+    h->host_roots = r;
+    h->host_threads = threads;
+    h->host_factored = false;
+    if (!h->analyzed) return MFD_OK;
+    if (!h->device_ok) {
+        build_plan(h);
+        return h->host_rc;
+    }
+    return replan(h);
+}
+
+int mfd_factorize_host_subtrees(mfd_handle* h, const double* a)
+{
+    if (!h || !a) return MFD_ERR_ARG;
+    if (!h->analyzed || h->host_roots.empty()) return MFD_ERR_STATE;
+    if (!h->device_ok) {
+        build_plan(h);
+        MFD_TRY(h->host_rc);
+        h->hstage_vec.assign(h->hplan.stage_elems, 0.0);
+    }
+    if (h->host_thr.joinable()) h->host_thr.join();
+    const double t0 = now_ms();
+    const int info = host_factor(h->hplan, h->S, a, stage_of(h), h->host_threads);
+    h->host_ms = now_ms() - t0;
+    h->host_factored = true;
+    return info;
+}
+
+int mfd_get_host_part(mfd_handle* h, double* ms, int* fronts, double* flops)
+{
+    if (!h) return MFD_ERR_ARG;
+    if (ms) *ms = h->host_ms;
+    if (fronts) *fronts = (int)h->hplan.fronts.size();
+    if (flops) {
+        double f = 0;
+        for (const HostFront& x : h->hplan.fronts)
+            for (int64_t t = x.n; t > x.w; --t) f += (double)t * t;
+        *flops = f;
+    }
+    return MFD_OK;
+}
+
 int mfd_set_front_split(mfd_handle* h, int cmax)
 {
     if (!h || cmax < 0) return MFD_ERR_ARG;
@@ -2148,7 +2304,9 @@ int mfd_factorize(mfd_handle* h, const double* a, int precision)
     double t0 = now_ms();
     const size_t nnz = h->S.snidx.size();
     HIP_TRY(hipMemcpyAsync(h->d_a, a, nnz * 8, hipMemcpyHostToDevice, h->stream));
+    h->host_vals = a;
     int rc = factorize_common(h, precision);
+    h->host_vals = nullptr;
     h->times[MFD_TIME_FACTORIZE] = (now_ms() - t0) / 1e3;
     h->times[MFD_TIME_REAL_FACTORIZE] = h->st.factor_ms / 1e3;
     return rc;
@@ -2162,7 +2320,15 @@ int mfd_factorize_device(mfd_handle* h, const double* d_a, int precision)
     double t0 = now_ms();
     if (d_a != h->d_a)
         HIP_TRY(hipMemcpyAsync(h->d_a, d_a, h->S.snidx.size() * 8, hipMemcpyDeviceToDevice, h->stream));
+    if (!h->hplan.roots.empty()) {  // the host part reads the values on the host
+        h->host_vals_buf.resize(h->S.snidx.size());
+        HIP_TRY(hipMemcpyAsync(h->host_vals_buf.data(), h->d_a, h->S.snidx.size() * 8, hipMemcpyDeviceToHost,
+                               h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        h->host_vals = h->host_vals_buf.data();
+    }
     int rc = factorize_common(h, precision);
+    h->host_vals = nullptr;
     h->times[MFD_TIME_FACTORIZE] = (now_ms() - t0) / 1e3;
     h->times[MFD_TIME_REAL_FACTORIZE] = h->st.factor_ms / 1e3;
     return rc;
@@ -2245,11 +2411,16 @@ int mfd_get_solve_errors(mfd_handle* h, int nrhs, double* eta, double* res2, dou
 int mfd_export_front(mfd_handle* h, int node, double* out)
 {
     if (!h || !out) return MFD_ERR_ARG;
-    if (!h->factored) return MFD_ERR_STATE;
     if (node < 1 || node > h->S.num_node) return MFD_ERR_ARG;
+    // a front of a host subtree, from the host staging array
+    const bool host = h->host_factored && (int)h->hplan.is_host.size() == h->S.num_node && h->hplan.is_host[node - 1];
+    if (!h->factored && !host) return MFD_ERR_STATE;
     const NodeDev& d = h->nodes[node - 1];
     const int64_t cnt = (int64_t)d.n * d.n;
-    if (h->precision == MFD_FP64) {
+    if (host) {
+        for (const HostFront& f : h->hplan.fronts)
+            if (f.node == node - 1) memcpy(out, stage_of(h) + f.front, cnt * sizeof(double));
+    } else if (h->precision == MFD_FP64) {
         HIP_TRY(hipMemcpy(out, h->d_pool64 + d.front, cnt * 8, hipMemcpyDeviceToHost));
     } else {
         std::vector<float> tmp(cnt);

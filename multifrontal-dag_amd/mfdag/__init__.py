@@ -101,6 +101,9 @@ def lib():
             "mfd_set_solve_block": (i, [P, i]),
             "mfd_get_solve_block": (i, [P]),
             "mfd_set_front_kinds": (i, [P, ip]),
+            "mfd_set_host_subtrees": (i, [P, ip, i, i]),
+            "mfd_factorize_host_subtrees": (i, [P, dp]),
+            "mfd_get_host_part": (i, [P, dp, ip, dp]),
             "mfd_get_front_times": (i, [P, dp]),
             "mfd_check_schedule": (i, [P, i, lp, C.c_char_p, i]),
             "mfd_mlp_predict": (i, [i, dp, dp, dp]),
@@ -277,6 +280,27 @@ class Solver:
     @solve_block.setter
     def solve_block(self, min_c):
         self._check(lib().mfd_set_solve_block(self.h, int(min_c)), "mfd_set_solve_block")
+
+    def set_host_subtrees(self, roots, threads=4):
+        """CPU+GPU hybrid: the subtrees rooted at roots (0-based node ids)
+        are factored on the host by `threads` threads (FP64 only)."""
+        r = _i32(roots)
+        self._check(lib().mfd_set_host_subtrees(self.h, _ptr(r, C.c_int) if len(r) else None, len(r), int(threads)),
+                    "mfd_set_host_subtrees")
+
+    def factorize_host_subtrees(self, a):
+        """The host part alone (no GPU needed): returns 0 or the failing column."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        rc = lib().mfd_factorize_host_subtrees(self.h, _ptr(a, C.c_double))
+        if rc < 0:
+            raise MfdError(rc, "mfd_factorize_host_subtrees")
+        return rc
+
+    def host_part(self):
+        """(ms, fronts, flops) of the last host part."""
+        ms, fr, fl = C.c_double(), C.c_int(), C.c_double()
+        self._check(lib().mfd_get_host_part(self.h, C.byref(ms), C.byref(fr), C.byref(fl)), "mfd_get_host_part")
+        return ms.value, fr.value, fl.value
 
     def set_front_split(self, cmax):
         """Fronts of order > 64 with at most cmax pivot columns are factored
