@@ -43,9 +43,14 @@ void launch_upd_diag(T* pool, const NodeDev* nodes, const DiagTask* diags, int n
 // with FP32 accumulation; 0 the front's own precision.
 template <typename T>
 void launch_gemm(T* pool, const GemmTask* tasks, int count, int kind, const int* lidx, hipStream_t st, int lowp = 0);
-// 128 x 128 tiles (m, n <= 128) of modes 0 and 3; lowp as for launch_gemm.
+// 128 x 128 tiles (m, n <= 128) of modes 0 and 3 with their K chunks (16
+// columns) divided evenly over `groups`
+// workgroups (segments [seg_ptr[w], seg_ptr[w + 1]) each), then the nfix
+// tiles whose K was split summed from their partial products in part
+// (16384 entries per slot).
 template <typename T>
-void launch_gemm_wide(T* pool, const GemmTask* tasks, int count, const int* lidx, hipStream_t st, int lowp = 0);
+void launch_gemm_seg(T* pool, const GemmTask* tasks, const SegTask* segs, const int* seg_ptr, int groups,
+                     const FixTask* fixes, int nfix, T* part, const int* lidx, hipStream_t st, int lowp = 0);
 void launch_fwd_init(double* zpool, const double* b, int ldb, int nrhs, const NodeDev* nodes, const int2* tasks,
                      int count, hipStream_t st);
 void launch_fwd_add(double* zpool, int nrhs, const NodeDev* nodes, const int* lidx, const int2* tasks, int count,

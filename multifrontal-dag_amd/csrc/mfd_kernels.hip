@@ -924,49 +924,52 @@ __global__ __launch_bounds__(kThreads) void k_upd_diag(T* pool, const NodeDev* n
                              pool + nd.inv + (int64_t)(t.k0 / kPanel) * kPanel * kPanel);
 }
 
-// C -= A B^T on a 128 x 128 tile (GemmTask with m, n <= 128, mode 0 or 3):
+// C -= A B^T on 128 x 128 tiles (GemmTask with m, n <= 128, mode 0 or 3):
 // the Schur update of fronts with many update rows (and their extend-add,
-// mode 3) and large in-panel updates.  Each of the four waves owns a
-// 64 x 64 quarter as 4 x 4 MFMA 16x16x4 blocks, so every operand read from
-// LDS feeds four MFMAs (half the LDS reads and half the L2 traffic per flop
-// of the 64 x 64 tile).  K is staged through LDS in chunks of 16, double
-// buffered: chunk j+1's global loads are issued before chunk j's MFMAs and
-// stored to the other buffer after them, one barrier per chunk.  A diagonal
-// tile (tri) skips the quarter above the diagonal.  The result goes back one
-// quarter at a time through LDS (coalesced along columns); mode 3 also adds
-// each new entry into the parent front at the rows and columns local_index
-// gives (one launch holds at most one child of each parent).
-template <typename T, int LP = kNative>
-__global__ __launch_bounds__(kThreads, 2) void k_gemm_wide(T* pool, const GemmTask* tasks, const int* lidx)
-{
-    typedef typename Mfma<T>::acc_t acc_t;
-    constexpr int TB = 128, LD = TB + 4;
-    const GemmTask t = tasks[blockIdx.x];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, li = lane & 15;
-    __shared__ union {
+// mode 3).  Each of the four waves owns a 64 x 64 quarter as 4 x 4 MFMA
+// 16x16x4 blocks, so every operand read from LDS feeds four MFMAs (half the
+// LDS reads and half the L2 traffic per flop of the 64 x 64 tile).  K is
+// staged through LDS in chunks of 16, double buffered: chunk j+1's global
+// loads are issued before chunk j's MFMAs and stored to the other buffer
+// after them, one barrier per chunk.  A diagonal tile (tri) skips the
+// quarter above the diagonal.  The result goes back one quarter at a time
+// through LDS (coalesced along columns); mode 3 also adds each new entry
+// into the parent front at the rows and columns local_index gives (one
+// launch holds at most one child of each parent).
+template <typename T> struct WideLds {
+    union {
         struct {
-            T As[2][16][LD];
-            T Bs[2][16][LD];
+            T As[2][16][132];
+            T Bs[2][16][132];
         } k;
         T Cs[64][65];
-    } u;
+    };
+};
+template <typename T> using WideAcc = typename Mfma<T>::acc_t[4][4];
+
+// acc = sum over K columns [k0, k1) of A B^T (a multiple of 16 apart, or k1
+// = K); ends with a barrier.
+template <typename T, int LP>
+__device__ __forceinline__ void wide_product(const T* pool, const GemmTask& t, int k0, int k1, WideAcc<T>& acc,
+                                             WideLds<T>& u)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, li = lane & 15;
     const int wr = (wv & 1) * 64, wc = (wv >> 1) * 64;
     const bool skip = t.tri && wc > wr;  // quarter strictly above the diagonal
     const T* A = pool + t.a;
     const T* B = pool + t.b;
-    acc_t acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+        for (int j = 0; j < 4; ++j) acc[i][j] = typename Mfma<T>::acc_t{0, 0, 0, 0};
     const int lr = tid & 127, lk = tid >> 7;
     T ra[8], rb[8];
-    auto load = [&](int k0) {
+    auto load = [&](int kc) {
 #pragma unroll
         for (int p = 0; p < 8; ++p) {
-            const int kk = k0 + lk + 2 * p;
-            ra[p] = (lr < t.m && kk < t.k) ? A[lr + (int64_t)kk * t.lda] : T(0);
-            rb[p] = (lr < t.n && kk < t.k) ? B[lr + (int64_t)kk * t.ldb] : T(0);
+            const int kk = kc + lk + 2 * p;
+            ra[p] = (lr < t.m && kk < k1) ? A[lr + (int64_t)kk * t.lda] : T(0);
+            rb[p] = (lr < t.n && kk < k1) ? B[lr + (int64_t)kk * t.ldb] : T(0);
         }
     };
     auto store = [&](int buf) {
@@ -976,13 +979,13 @@ __global__ __launch_bounds__(kThreads, 2) void k_gemm_wide(T* pool, const GemmTa
             u.k.Bs[buf][lk + 2 * p][lr] = rb[p];
         }
     };
-    load(0);
+    load(k0);
     store(0);
     __syncthreads();
-    const int nch = (t.k + 15) / 16;
+    const int nch = (k1 - k0 + 15) / 16;
     for (int j = 0; j < nch; ++j) {
         const int buf = j & 1;
-        if (j + 1 < nch) load(16 * (j + 1));
+        if (j + 1 < nch) load(k0 + 16 * (j + 1));
         if (!skip) {
             if constexpr (LP == kNative) {
 #pragma unroll
@@ -1019,6 +1022,17 @@ __global__ __launch_bounds__(kThreads, 2) void k_gemm_wide(T* pool, const GemmTa
         if (j + 1 < nch) store(buf ^ 1);
         __syncthreads();
     }
+}
+
+// C -= acc (and, mode 3, each new entry added into the parent front), or,
+// with part != nullptr, acc stored as partial product `part` (four 64 x 64
+// quarters, column-major); both through LDS one quarter at a time; ends with
+// a barrier.
+template <typename T>
+__device__ __forceinline__ void wide_apply(T* pool, const GemmTask& t, const int* lidx, const WideAcc<T>& acc,
+                                           WideLds<T>& u, T* part)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, li = lane & 15;
     T* C = pool + t.c;
     const bool ea = t.mode == 3;
     for (int w = 0; w < 4; ++w) {
@@ -1034,7 +1048,10 @@ __global__ __launch_bounds__(kThreads, 2) void k_gemm_wide(T* pool, const GemmTa
         }
         __syncthreads();
         const int row = tid & 63, gr = qr + row;
-        if (gr < t.m) {
+        if (part) {
+            T* Q = part + w * 4096 + row;
+            for (int col = tid >> 6; col < 64; col += 4) Q[col * 64] = u.Cs[col][row];
+        } else if (gr < t.m) {
             T* P = ea ? pool + t.p + (lidx[t.li + gr] - 1) : C;
             for (int c0 = tid >> 6; c0 < 64; c0 += 16) {
                 T cv[4], pv[4];
@@ -1059,6 +1076,60 @@ __global__ __launch_bounds__(kThreads, 2) void k_gemm_wide(T* pool, const GemmTa
             }
         }
         __syncthreads();
+    }
+}
+
+// The tiles' K loops divided evenly over the workgroups (every workgroup
+// takes a run of 16-column K chunks, which may start and end inside a
+// tile), so that a launch has no partly filled last round: workgroup w
+// takes segments [seg_ptr[w], seg_ptr[w + 1]).  A segment covering its
+// tile's whole K applies its product at once; the others store their
+// partial products (slot: four 64 x 64 quarters, column-major, staged
+// through LDS) for k_gemm_fix, which adds them in segment order and applies
+// the sum.  Nothing waits on another workgroup.
+template <typename T, int LP = kNative>
+__global__ __launch_bounds__(kThreads, 2) void k_gemm_seg(T* pool, const GemmTask* tasks, const SegTask* segs,
+                                                          const int* seg_ptr, T* part, const int* lidx)
+{
+    __shared__ WideLds<T> u;
+    for (int s = seg_ptr[blockIdx.x]; s < seg_ptr[blockIdx.x + 1]; ++s) {
+        const SegTask g = segs[s];
+        const GemmTask t = tasks[g.task];
+        WideAcc<T> acc;
+        wide_product<T, LP>(pool, t, g.k0, g.k1, acc, u);
+        wide_apply<T>(pool, t, lidx, acc, u, g.slot < 0 ? nullptr : part + (int64_t)g.slot * 16384);
+    }
+}
+
+// Tiles whose K was split: C -= the partial products of slots slot0 ..
+// slot0 + nslot - 1 added in that order (and, mode 3, each new entry added
+// into the parent front), quarter by quarter like wide_apply.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_gemm_fix(T* pool, const GemmTask* tasks, const FixTask* fixes,
+                                                       const T* part, const int* lidx)
+{
+    const FixTask f = fixes[blockIdx.x];
+    const GemmTask t = tasks[f.task];
+    const int tid = threadIdx.x, row = tid & 63;
+    T* C = pool + t.c;
+    const bool ea = t.mode == 3;
+    for (int w = 0; w < 4; ++w) {
+        const int qr = (w & 1) * 64, qc = (w >> 1) * 64, gr = qr + row;
+        if ((t.tri && qc > qr) || qr >= t.m || qc >= t.n || gr >= t.m) continue;
+        const T* Q = part + (int64_t)f.slot0 * 16384 + w * 4096 + row;
+        T* P = ea ? pool + t.p + (lidx[t.li + gr] - 1) : C;
+        for (int col = tid >> 6; col < 64; col += 4) {
+            const int gc = qc + col;
+            if (gc >= t.n || (t.tri && gr < gc)) continue;
+            T sum = Q[col * 64];
+            for (int sl = 1; sl < f.nslot; ++sl) sum += Q[(int64_t)sl * 16384 + col * 64];
+            const T v = C[gr + (int64_t)gc * t.ldc] - sum;
+            C[gr + (int64_t)gc * t.ldc] = v;
+            if (ea) {
+                T* pp = P + (int64_t)(lidx[t.lj + gc] - 1) * t.np;
+                *pp = *pp + v;
+            }
+        }
     }
 }
 
@@ -2196,20 +2267,26 @@ void launch_scale_rows(double* out, const double* in, const double* s, int n, in
                            s, n, nrhs);
 }
 template <typename T>
-void launch_gemm_wide(T* pool, const GemmTask* tasks, int count, const int* lidx, hipStream_t st, int lowp)
+void launch_gemm_seg(T* pool, const GemmTask* tasks, const SegTask* segs, const int* seg_ptr, int groups,
+                     const FixTask* fixes, int nfix, T* part, const int* lidx, hipStream_t st, int lowp)
 {
-    if (count <= 0) return;
+    if (groups <= 0) return;
     if constexpr (sizeof(T) == 4) {
-        if (lowp == kF16) {
-            hipLaunchKernelGGL((k_gemm_wide<T, kF16>), dim3(count), dim3(kThreads), 0, st, pool, tasks, lidx);
-            return;
-        }
-        if (lowp == kBF16) {
-            hipLaunchKernelGGL((k_gemm_wide<T, kBF16>), dim3(count), dim3(kThreads), 0, st, pool, tasks, lidx);
-            return;
-        }
+        if (lowp == kF16)
+            hipLaunchKernelGGL((k_gemm_seg<T, kF16>), dim3(groups), dim3(kThreads), 0, st, pool, tasks, segs, seg_ptr,
+                               part, lidx);
+        else if (lowp == kBF16)
+            hipLaunchKernelGGL((k_gemm_seg<T, kBF16>), dim3(groups), dim3(kThreads), 0, st, pool, tasks, segs, seg_ptr,
+                               part, lidx);
+        else
+            hipLaunchKernelGGL((k_gemm_seg<T, kNative>), dim3(groups), dim3(kThreads), 0, st, pool, tasks, segs,
+                               seg_ptr, part, lidx);
+    } else {
+        hipLaunchKernelGGL((k_gemm_seg<T, kNative>), dim3(groups), dim3(kThreads), 0, st, pool, tasks, segs, seg_ptr,
+                           part, lidx);
     }
-    hipLaunchKernelGGL((k_gemm_wide<T, kNative>), dim3(count), dim3(kThreads), 0, st, pool, tasks, lidx);
+    if (nfix > 0)
+        hipLaunchKernelGGL(k_gemm_fix<T>, dim3(nfix), dim3(kThreads), 0, st, pool, tasks, fixes, part, lidx);
 }
 void launch_fwd_init(double* zpool, const double* b, int ldb, int nrhs, const NodeDev* nodes, const int2* tasks,
                      int count, hipStream_t st)
@@ -2292,13 +2369,14 @@ void launch_scatter_rows(double* out, int ld_out, const double* in, int ld_in, c
     template void launch_diag<T>(T*, const NodeDev*, const DiagTask*, int, int*, hipStream_t);                        \
     template void launch_upd_diag<T>(T*, const NodeDev*, const DiagTask*, int, const GemmTask*, int, int*, hipStream_t); \
     template void launch_gemm<T>(T*, const GemmTask*, int, int, const int*, hipStream_t, int);                         \
-    template void launch_gemm_wide<T>(T*, const GemmTask*, int, const int*, hipStream_t, int);                         \
     template void launch_scatter_a_scaled<T>(T*, const double*, const int64_t*, const int64_t*, const int2*,           \
                                              const double*, int64_t, hipStream_t);                                    \
     template void launch_fwd_step<T>(const T*, double*, int, const NodeDev*, const SolveTask*, int, hipStream_t);       \
     template void launch_superinv<T>(const T*, T*, const NodeDev*, const SuperInvTask*, int, bool, hipStream_t);      \
     template void launch_fwd_super<T>(const T*, const T*, double*, int, const NodeDev*, const SuperTask*, int,         \
                                       hipStream_t);                                                                    \
+    template void launch_gemm_seg<T>(T*, const GemmTask*, const SegTask*, const int*, int, const FixTask*, int, T*,  \
+                                     const int*, hipStream_t, int);                                                    \
     template void launch_bwd_super<T>(const T*, const T*, double*, const double*, double*, int, int, const NodeDev*,    \
                                       const SuperTask*, int, hipStream_t);                                             \
     template void launch_bwd_step<T>(const T*, double*, double*, double*, int, int, const NodeDev*, const SolveTask*,  \

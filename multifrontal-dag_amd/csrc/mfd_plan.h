@@ -112,4 +112,16 @@ struct GemmTask {
     int kd, np, li, lj;
 };
 
+// A run of K of one 128 x 128 tile: GEMM task `task`, K columns [k0, k1);
+// slot < 0: the whole K, applied at once; otherwise its partial product goes
+// to partial-product slot `slot`.
+struct SegTask {
+    int task, k0, k1, slot;
+};
+// A tile whose K was split: its partial products in slots slot0 .. slot0 +
+// nslot - 1, in K order.
+struct FixTask {
+    int task, slot0, nslot, pad;
+};
+
 }  // namespace mfd

@@ -39,7 +39,11 @@ struct Launch {
     int kind;       // MFD_K_*
     int64_t first;  // first task in the kind's task array
     int count;
-    int wide = 0;   // GEMM launches: 1 = 128 x 128 tiles (k_gemm_wide)
+    int wide = 0;   // GEMM launches: 1 = 128 x 128 tiles (k_gemm_seg + k_gemm_fix)
+    // wide launches: K chunks over `groups` workgroups, their segments at
+    // seg_ptr[sfirst .. sfirst + groups], then nfix split tiles from fix[xfirst]
+    int64_t sfirst = 0, xfirst = 0;
+    int groups = 0, nfix = 0;
     // MFD_K_POTRF launches: fused = 1 runs the step's update tiles
     // [first, first + count) of the GEMM tasks beside the diagonal tasks
     // [dfirst, dfirst + dcount) (k_upd_diag); otherwise [first, first +
@@ -78,11 +82,13 @@ constexpr int kMaxPeriod = 8;
 constexpr int kAutoSteps = 48;  // 3072 columns
 
 // Schur updates of fronts with at least wide_min_rows update rows use
-// 128 x 128 tiles (k_gemm_wide), the others 64 x 64 (k_gemm); set per handle
-// by mfd_set_tile_bound, 0 (64 x 64 only) by default: on the 64^3 workload
-// every bound tried (256, 512, 1024) was slower (factor 24.1-24.7 against
-// 21.5 ms), the 128 x 128 launches of its few large fronts holding fewer
-// tiles than the GPU has CUs.
+// 128 x 128 tiles with their K chunks divided evenly over the workgroups
+// (k_gemm_seg, k_gemm_fix), the others 64 x 64 tiles (k_gemm); set per
+// handle by mfd_set_tile_bound, 0 (64 x 64 only) by default: with the
+// extend-add in its epilogue the 128 x 128 tile (two workgroups per CU)
+// hides its scattered parent accesses worse than four 64 x 64 workgroups
+// per CU, and on the 64^3 workload every bound tried (1024, 2048, 3000
+// update rows) was slower (factor 36.3, 35.5, 27.7 ms against 21.8 ms).
 constexpr int kWideTile = 128;
 constexpr int kWideMinRows = 0;
 
@@ -124,6 +130,17 @@ struct mfd_handle {
     int fuse_cmax = 0;
     std::vector<int> front_kind;
     int wide_min_rows = kWideMinRows;  // Schur tiles of 128 x 128 from this many update rows (0: never)
+    // 128 x 128 Schur launches with their K chunks divided evenly over the
+    // workgroups: segments, per-launch workgroup ranges, split tiles, and
+    // the most partial-product slots one launch needs
+    std::vector<SegTask> segs;
+    std::vector<int> seg_ptr;
+    std::vector<FixTask> fixes;
+    int64_t seg_slots = 0;
+    SegTask* d_segs = nullptr;
+    int* d_seg_ptr = nullptr;
+    FixTask* d_fixes = nullptr;
+    double* d_segpart = nullptr;  // seg_slots x 16384 entries (FP32 launches use it as floats)
     int super_min = kSuperMin;         // forward superblocks from this many pivot columns (0: never)
     bool device_ok = false;
     hipStream_t stream = nullptr;
@@ -336,6 +353,10 @@ void free_device(mfd_handle* h)
     h->d_fstep = h->d_bstep = nullptr;
     h->d_xstep = h->d_xbstep = nullptr;
     h->d_xinv = nullptr;
+    h->d_segs = nullptr;
+    h->d_seg_ptr = nullptr;
+    h->d_fixes = nullptr;
+    h->d_segpart = nullptr;
     h->d_sinv64 = nullptr;
     h->d_sinv32 = nullptr;
     h->d_fi = h->d_fa = h->d_bg = nullptr;
@@ -714,7 +735,55 @@ void build_plan(mfd_handle* h)
     h->diag.clear();
     h->gemm.clear();
     h->fseq.clear();
+    h->segs.clear();
+    h->seg_ptr.clear();
+    h->fixes.clear();
+    h->seg_slots = 0;
     int nside = 0;
+    // a wide launch's K chunks (16 columns) divided evenly over at most
+    // kSegGroups workgroups (two per CU), no fewer than kSegMin chunks each
+    constexpr int kSegGroups = 512, kSegMin = 16;
+    auto balance = [&](Launch& l) {
+        std::vector<int64_t> units(l.count);
+        int64_t total = 0;
+        for (int i = 0; i < l.count; ++i) total += units[i] = (h->gemm[l.first + i].k + 15) / 16;
+        const int P = (int)std::max<int64_t>(1, std::min<int64_t>(kSegGroups, total / kSegMin));
+        l.sfirst = (int64_t)h->seg_ptr.size();
+        l.groups = P;
+        const size_t s0 = h->segs.size();
+        int i = 0;
+        int64_t pos = 0;
+        for (int w = 0; w < P; ++w) {
+            h->seg_ptr.push_back((int)h->segs.size());
+            int64_t need = total * (w + 1) / P - total * w / P;
+            while (need > 0 && i < l.count) {
+                const int64_t take = std::min(need, units[i] - pos);
+                const int k = h->gemm[l.first + i].k;
+                h->segs.push_back(SegTask{(int)(l.first + i), (int)(16 * pos), (int)std::min<int64_t>(16 * (pos + take), k), 0});
+                pos += take;
+                need -= take;
+                if (pos == units[i]) ++i, pos = 0;
+            }
+        }
+        h->seg_ptr.push_back((int)h->segs.size());
+        // tiles in more than one segment keep their partial products in
+        // consecutive slots, in K order
+        l.xfirst = (int64_t)h->fixes.size();
+        int slot = 0;
+        for (size_t a = s0; a < h->segs.size();) {
+            size_t b = a + 1;
+            while (b < h->segs.size() && h->segs[b].task == h->segs[a].task) ++b;
+            if (b - a == 1) {
+                h->segs[a].slot = -1;
+            } else {
+                h->fixes.push_back(FixTask{h->segs[a].task, slot, (int)(b - a), 0});
+                for (size_t q = a; q < b; ++q) h->segs[q].slot = slot++;
+            }
+            a = b;
+        }
+        l.nfix = (int)(h->fixes.size() - l.xfirst);
+        h->seg_slots = std::max<int64_t>(h->seg_slots, slot);
+    };
     auto push = [&](int kind, int64_t first, int64_t end, int wide = 0) {
         if (end > first) h->fseq.push_back(Launch{kind, first, (int)(end - first), wide});
     };
@@ -909,6 +978,7 @@ void build_plan(mfd_handle* h)
                         }
                 }
                 push(MFD_K_SCHUR, f4, (int64_t)h->gemm.size(), wide);
+                if (wide && (int64_t)h->gemm.size() > f4) balance(h->fseq.back());
             }
 
     }
@@ -988,6 +1058,10 @@ int setup_device(mfd_handle* h)
         h->fill_ev.push_back(ev);
     }
     MFD_TRY(upload(h, &h->d_gemm, h->gemm));
+    MFD_TRY(upload(h, &h->d_segs, h->segs));
+    MFD_TRY(upload(h, &h->d_seg_ptr, h->seg_ptr));
+    MFD_TRY(upload(h, &h->d_fixes, h->fixes));
+    if (h->seg_slots > 0) MFD_TRY(dalloc(h, &h->d_segpart, h->seg_slots * 16384));
     MFD_TRY(upload(h, &h->d_level_nodes, h->level_nodes));
     MFD_TRY(upload(h, &h->d_child_ptr, h->child_ptr));
     MFD_TRY(upload(h, &h->d_child_list, h->child_list));
@@ -1101,7 +1175,8 @@ template <typename T> void enqueue_launch(mfd_handle* h, const Launch& l, hipStr
     case MFD_K_UPDATE:
     case MFD_K_SCHUR:
         if (l.wide)
-            launch_gemm_wide<T>(pool, h->d_gemm + l.first, l.count, h->d_lidx, st, sizeof(T) == 4 ? h->lowp : 0);
+            launch_gemm_seg<T>(pool, h->d_gemm, h->d_segs, h->d_seg_ptr + l.sfirst, l.groups, h->d_fixes + l.xfirst,
+                               l.nfix, reinterpret_cast<T*>(h->d_segpart), h->d_lidx, st, sizeof(T) == 4 ? h->lowp : 0);
         else
             launch_gemm<T>(pool, h->d_gemm + l.first, l.count,
                            l.kind == MFD_K_TRSM ? 0 : l.kind == MFD_K_UPDATE ? 1 : 2, h->d_lidx, st,
