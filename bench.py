@@ -101,6 +101,8 @@ def batch_leg(args):
     analyze_s = time.perf_counter() - t1
     if args.front_split >= 0:
         s.set_front_split(args.front_split)
+    if args.solve_block >= 0:
+        s.solve_block = args.solve_block
     log(f"batch: {len(systems)} systems, n={bp.n}, fronts={s.num_node}, cost={s.cost():.3e} "
         f"(setup {t1 - t0:.1f}s, analysis {analyze_s:.2f}s)")
     dev = torch.device("cuda:0")
@@ -453,6 +455,8 @@ def main():
         s.tile_bound = args.tile_bound
     if args.solve_block >= 0:
         s.solve_block = args.solve_block
+    if args.front_split >= 0:
+        s.set_front_split(args.front_split)
     analyze_s = time.perf_counter() - t0
     cost = s.cost()
     log(f"analysis {analyze_s:.2f}s: fronts={s.num_node} cost={cost:.3e} nnz(L)={s.nonzero():.3e}")

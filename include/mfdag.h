@@ -305,7 +305,7 @@ int mfd_get_tile_bound(mfd_handle* h);
    explicit inverses of their 256 x 256 diagonal blocks and the products
    coupling neighbouring superblocks (formed at the end of each
    factorization), the others over 64-column blocks; 0: 64 columns only.
-   Default 512.  Rebuilds the schedule like mfd_set_update_period (the
+   Default 256.  Rebuilds the schedule like mfd_set_update_period (the
    factorization is lost). */
 int mfd_set_solve_block(mfd_handle* h, int min_c);
 int mfd_get_solve_block(mfd_handle* h);

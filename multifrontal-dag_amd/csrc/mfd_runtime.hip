@@ -92,11 +92,15 @@ constexpr int kAutoSteps = 48;  // 3072 columns
 constexpr int kWideTile = 128;
 constexpr int kWideMinRows = 0;
 
-// Fronts with at least super_min pivot columns solve forward over
-// 256-column superblocks (k_fwd_super: a quarter of the dependent launches
-// of the 64-column steps), with their superblock inverses formed at the end
-// of the factorization; set per handle by mfd_set_solve_block (0: never).
-constexpr int kSuperMin = 512;
+// Fronts with at least super_min pivot columns solve forward and backward
+// over 256-column superblocks (k_fwd_super, k_bwd_super: a quarter of the
+// dependent launches of the 64-column steps), with their superblock
+// matrices formed at the end of the factorization; set per handle by
+// mfd_set_solve_block (0: never).  256: on the 64^3 workload the bounds 128,
+// 256, 512 and 1024 gave FP64 steps of 25.08, 25.13, 25.17 and 25.39 ms,
+// and on the 512-system batch 128 was slower than 512 (FP64 step 21.26
+// against 20.90 ms: its fronts of one partial superblock gain nothing).
+constexpr int kSuperMin = 256;
 
 double now_ms()
 {

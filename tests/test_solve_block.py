@@ -30,7 +30,7 @@ def log(*a):
 def test_solve_block_checked_without_compute():
     p = P.laplacian_2d(20)
     s = mfdag.Solver(nb=16, host_only=True)
-    assert s.solve_block == 512
+    assert s.solve_block == 256
     with pytest.raises(mfdag.MfdError):
         s.solve_block = -1
     s.analyze(p.ccs_col, p.ccs_row, 8, p.perm)
