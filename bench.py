@@ -103,6 +103,8 @@ def batch_leg(args):
         s.set_front_split(args.front_split)
     if args.solve_block >= 0:
         s.solve_block = args.solve_block
+    if args.level_rule >= 0:
+        s.level_rule = args.level_rule
     log(f"batch: {len(systems)} systems, n={bp.n}, fronts={s.num_node}, cost={s.cost():.3e} "
         f"(setup {t1 - t0:.1f}s, analysis {analyze_s:.2f}s)")
     dev = torch.device("cuda:0")
@@ -397,6 +399,9 @@ def main():
     ap.add_argument("--solve-block", type=int, default=-1, metavar="COLS",
                     help="forward solve over 256-column superblocks for fronts with at least COLS pivot columns "
                          "(0: never; -1: the library default)")
+    ap.add_argument("--level-rule", type=int, default=-1,
+                    help="schedule levels: 1 fronts one below their parent, 0 one above their highest child "
+                         "(-1: the library default)")
     ap.add_argument("--hybrid-bins", type=int, default=64,
                     help="bins of the subtree partition in the CPU+GPU hybrid leg (0: skip the leg)")
     ap.add_argument("--hybrid-share", type=float, default=0.001,
@@ -457,6 +462,8 @@ def main():
         s.solve_block = args.solve_block
     if args.front_split >= 0:
         s.set_front_split(args.front_split)
+    if args.level_rule >= 0:
+        s.level_rule = args.level_rule
     analyze_s = time.perf_counter() - t0
     cost = s.cost()
     log(f"analysis {analyze_s:.2f}s: fronts={s.num_node} cost={cost:.3e} nnz(L)={s.nonzero():.3e}")

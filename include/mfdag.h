@@ -271,6 +271,15 @@ int mfd_mm_read(const char* path, int n, int64_t nnz, int* ccs_col, int* ccs_row
    rebuilt at once and the factorization must be redone (mfd_solve returns
    MFD_ERR_STATE until then); otherwise it applies to the next analysis. */
 int mfd_set_update_period(mfd_handle* h, int period);
+/* Level rule of the schedule (no counterpart in the reference, which walks
+   the fronts in postorder, seq_factorize_m.f90:13-29, or lets StarPU order
+   them, starpu_factorize_m.f90:13-31): the fronts of one level share every
+   launch.  1 (default): each front one level below its parent, roots on the
+   top level, so that siblings' diagonal chains share launches; 0: each front
+   one level above its highest child, leaves on level 0.  Rebuilds the
+   schedule like mfd_set_update_period (the factorization is lost). */
+int mfd_set_level_rule(mfd_handle* h, int rule);
+int mfd_get_level_rule(mfd_handle* h);
 int mfd_get_update_period(mfd_handle* h);
 
 /* Medium fronts (no counterpart in the reference, whose StarPU tasks are

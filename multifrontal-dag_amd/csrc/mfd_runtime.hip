@@ -127,6 +127,7 @@ double now_ms()
 struct mfd_handle {
     int nb = 0, flags = 0;
     int upd_period = kUpdPeriod;
+    int level_rule = 1;  // mfd_set_level_rule
     // Medium fronts (order > 64) factored whole by one workgroup each
     // (k_front_wg): those with at most fuse_cmax pivot columns, unless
     // front_kind (one entry per node, when set) says otherwise: 1 = one
@@ -668,9 +669,22 @@ void build_plan(mfd_handle* h)
     if (h->host_rc != MFD_OK) h->hplan = HostPlan{};
     auto on_host = [&](int s) { return !h->hplan.is_host.empty() && h->hplan.is_host[s]; };
 
+    // Levels.  Rule 1 (the default): a front sits one level below its
+    // parent (roots on the top level), so siblings share a level and their
+    // diagonal chains run in the same launches; rule 0: one level above its
+    // highest child (leaves on level 0).  Either way every child is below
+    // its parent.
     std::vector<int> lev(m, 0);
-    for (int s = 0; s < m; ++s)
-        for (int ch : S.children[s]) lev[s] = std::max(lev[s], lev[ch - 1] + 1);
+    if (h->level_rule == 1) {
+        std::vector<int> depth(m, 0);
+        for (int s = m - 1; s >= 0; --s)  // parents follow their children
+            if (S.parent[s] > 0) depth[s] = depth[S.parent[s] - 1] + 1;
+        const int top = m ? *std::max_element(depth.begin(), depth.end()) : 0;
+        for (int s = 0; s < m; ++s) lev[s] = top - depth[s];
+    } else {
+        for (int s = 0; s < m; ++s)
+            for (int ch : S.children[s]) lev[s] = std::max(lev[s], lev[ch - 1] + 1);
+    }
     h->num_levels = m ? *std::max_element(lev.begin(), lev.end()) + 1 : 0;
     h->level_ptr.assign(h->num_levels + 1, 0);
     for (int s = 0; s < m; ++s) h->level_ptr[lev[s] + 1]++;
@@ -2255,6 +2269,15 @@ int mfd_check_schedule(mfd_handle* h, int variant, int64_t* conflicts, char* rep
     }
     return MFD_OK;
 }
+
+int mfd_set_level_rule(mfd_handle* h, int rule)
+{
+    if (!h || rule < 0 || rule > 1) return MFD_ERR_ARG;
+    if (rule == h->level_rule) return MFD_OK;
+    h->level_rule = rule;
+    return replan(h);
+}
+int mfd_get_level_rule(mfd_handle* h) { return h ? h->level_rule : MFD_ERR_ARG; }
 
 int mfd_set_tile_bound(mfd_handle* h, int min_rows)
 {

@@ -97,6 +97,8 @@ def lib():
             "mfd_get_update_period": (i, [P]),
             "mfd_set_front_split": (i, [P, i]),
             "mfd_set_tile_bound": (i, [P, i]),
+            "mfd_set_level_rule": (i, [P, i]),
+            "mfd_get_level_rule": (i, [P]),
             "mfd_get_tile_bound": (i, [P]),
             "mfd_set_solve_block": (i, [P, i]),
             "mfd_get_solve_block": (i, [P]),
@@ -270,6 +272,16 @@ class Solver:
     @tile_bound.setter
     def tile_bound(self, min_rows):
         self._check(lib().mfd_set_tile_bound(self.h, int(min_rows)), "mfd_set_tile_bound")
+
+    @property
+    def level_rule(self):
+        """1: each front one level below its parent (default); 0: one above
+        its highest child."""
+        return lib().mfd_get_level_rule(self.h)
+
+    @level_rule.setter
+    def level_rule(self, rule):
+        self._check(lib().mfd_set_level_rule(self.h, int(rule)), "mfd_set_level_rule")
 
     @property
     def solve_block(self):

@@ -190,11 +190,21 @@ def diag_slots(elem_bytes):
     return CUS * (1 if elem_bytes == 8 else 2)
 
 
-def level_of(parent):
-    """Level of each node (0-based postorder, parent 1-based, 0 for roots):
-    1 + the highest child level, as the schedule builds its levels."""
-    lev = np.zeros(len(parent), dtype=np.int64)
-    for k in range(len(parent)):
+def level_of(parent, rule=1):
+    """Level of each node (0-based postorder, parent 1-based, 0 for roots),
+    as the schedule builds its levels (mfd_set_level_rule): rule 1, one
+    below the parent with the roots on the top level; rule 0, 1 + the
+    highest child level."""
+    m = len(parent)
+    lev = np.zeros(m, dtype=np.int64)
+    if rule == 1:
+        depth = np.zeros(m, dtype=np.int64)
+        for k in range(m - 1, -1, -1):  # parents follow their children
+            p = int(parent[k]) - 1
+            if p >= 0:
+                depth[k] = depth[p] + 1
+        return (depth.max() if m else 0) - depth
+    for k in range(m):
         p = int(parent[k]) - 1
         if p >= 0:
             lev[p] = max(lev[p], lev[k] + 1)
@@ -301,9 +311,10 @@ def chain_record(ccs_col, ccs_row, max_zero, perm, a, precision, nb=64, batch=No
     kt = s.kernel_times()
     st = s.structure()
     c, w, par = st["supernode_size"], st["work_size"], st["parent"]
+    rule = s.level_rule
     s.close()
     n = c + w
-    lev = level_of(par)
+    lev = level_of(par, rule)
     slots = diag_slots(8 if precision == FP64 else 4)
     rounds, flops = 0, 0.0
     for L in range(lev.max() + 1):
@@ -332,7 +343,8 @@ def chain_constants(records):
     return tau, t0, 1.0 / inv_rate
 
 
-def choose_kinds(structure, model, tau, schur_us, rate, elem_bytes, max_order, slots=2 * CUS, margin=0.1):
+def choose_kinds(structure, model, tau, schur_us, rate, elem_bytes, max_order, slots=2 * CUS, margin=0.1,
+                 level_rule=1):
     """Per node: 1 (medium) or 0 (large), level by level, the k largest
     fronts (by pivots) large for the k that minimises the predicted level
     time: medium part max(longest front, sum / slots) by the model, large
@@ -340,10 +352,11 @@ def choose_kinds(structure, model, tau, schur_us, rate, elem_bytes, max_order, s
     rank) * schur_us + Schur flops / rate.  Fronts of order above max_order
     (outside the model's training data) stay large, and a level keeps all
     its fronts large unless the best split is predicted faster by more than
-    margin (the predictions' error is of that order)."""
+    margin (the predictions' error is of that order).  level_rule: the
+    solver's (mfd_set_level_rule)."""
     c, w, par = structure["supernode_size"], structure["work_size"], structure["parent"]
     n = c + w
-    lev = level_of(par)
+    lev = level_of(par, level_rule)
     kinds = np.zeros(len(c), dtype=np.int32)
     dslots = diag_slots(elem_bytes)
     big = n > 64
