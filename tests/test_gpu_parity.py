@@ -162,6 +162,29 @@ def test_multiple_rhs_and_device_entry_points():
     assert P.backward_error(p, x, b) <= 1e-14
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("rule", [0, 1])
+def test_level_rules_against_oracle(rule):
+    """Both level rules (fronts one level above their highest child, or one
+    below their parent) give the oracle's fronts and the same solution."""
+    p = P.weighted_laplacian_3d(20, seed=3)
+    s = mfdag.Solver(nb=32, graph=True)
+    s.analyze(p.ccs_col, p.ccs_row, 64, p.perm)
+    s.level_rule = rule
+    assert s.level_rule == rule
+    assert s.factorize(p.val) == 0
+    f, st, info = oracle_for(s, p.val)
+    assert info == 0
+    errs = front_errors(s, f, st)
+    xt, b = P.rhs(p, 2, seed=9)
+    x = s.solve(b.copy())
+    eta = P.backward_error(p, x, b)
+    print(f"rule {rule}: worst front error {errs.max():.2e}, eta {eta:.2e}", flush=True)
+    assert errs.max() <= 1e-12
+    assert eta <= 1e-14
+    s.close()
+
+
 @pytest.mark.parametrize("precision", [mfdag.FP64, mfdag.FP32_IR])
 def test_more_right_hand_sides_than_one_group(precision):
     """11 right-hand sides: the solve kernels take them 8 at a time, so the

@@ -5,8 +5,9 @@ entries; a conflict is a pair of tasks touching a common entry, one
 writing, with no order between them (one launch, or two launches no DAG
 path orders).  CPU only: the schedule is built on the host.
 
-The schedules the library builds have no conflict, for every update period
-and front-kind choice.  Two modeled schedules that it does not build show
+The schedules the library builds have no conflict, for both level rules,
+every update period, front-kind choice, 128 x 128 Schur tiles and the
+hybrid's host subtrees.  Two modeled schedules that it does not build show
 what the check finds: the Schur tiles of fronts whose chain ends early
 moved to a fourth stream forked after the launch with the front's last
 diagonal task (variant 1: they read the rows that the front's last TRSM
@@ -39,11 +40,21 @@ CASES = [
 @pytest.mark.parametrize("name,make,max_zero", CASES, ids=[c[0] for c in CASES])
 def test_built_schedules_have_no_conflict(name, make, max_zero):
     s = solver(make(), max_zero)
-    for period in (1, 2, 3, 4, 0):
-        s.update_period = period
-        n, rep = s.check_schedule(0)
-        assert n == 0, (period, rep)
+    for rule in (0, 1):  # levels from the leaves up, and from the roots down
+        s.level_rule = rule
+        for period in (1, 2, 3, 4, 0):
+            s.update_period = period
+            n, rep = s.check_schedule(0)
+            assert n == 0, (rule, period, rep)
     s.update_period = 2
+    s.tile_bound = 1  # every Schur update in 128 x 128 tiles over balanced K segments
+    assert s.check_schedule(0)[0] == 0
+    s.tile_bound = 0
+    st = s.structure()
+    sub, bins = mfdag.tree_pruning(st["supernode_size"], st["work_size"], st["parent"], 0.5, 4)
+    s.set_host_subtrees(np.sort(sub[bins == 0] - 1))  # the hybrid: these fronts get no launch
+    assert s.check_schedule(0)[0] == 0
+    s.set_host_subtrees([])
     s.set_front_split(1 << 30)  # every front of order > 64 by one workgroup
     assert s.check_schedule(0)[0] == 0
     s.set_front_split(0)
