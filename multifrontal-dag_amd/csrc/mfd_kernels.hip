@@ -768,6 +768,23 @@ __device__ __forceinline__ void gemm_task(T* pool, const GemmTask& t, TileLds<T>
     }
 }
 
+// A TRSM tile with syrk > 0, after gemm_task: the diagonal block at cd
+// loses R R^T over its lower syrk x syrk part, R = the tile's rows over the
+// pending panels and the panel just solved (contiguous columns, A then C,
+// which this workgroup has just written).
+template <typename T>
+__device__ __forceinline__ void trsm_syrk(T* pool, const GemmTask& t, TileLds<T>& L, T (*Cs)[kTile + 1])
+{
+    __syncthreads();
+    const T* R = pool + (t.mode == 2 ? t.a : t.c);
+    const int k = t.mode == 2 ? t.k + t.kd : t.k;
+    tile_product<T>(R, t.lda, R, t.lda, false, t.syrk, t.syrk, k, L, Cs);
+    T* D = pool + t.cd;
+    const int row = threadIdx.x & 63;
+    if (row < t.syrk)
+        for (int col = threadIdx.x >> 6; col <= row; col += 4) D[row + (int64_t)col * t.ldc] -= Cs[col][row];
+}
+
 // LP != kNative: the update and Schur tiles (KIND 1, 2) of the FP16_IR /
 // BF16_IR forms multiply 16-bit operands.
 template <typename T, int KIND, int LP = kNative>
@@ -779,7 +796,9 @@ __global__ __launch_bounds__(kThreads) void k_gemm(T* pool, const GemmTask* task
         // in Cs while staging the inverse through L, so the two are separate
         __shared__ TileLds<T> L;
         __shared__ T Cs[kTile][kTile + 1];
-        gemm_task<T>(pool, tasks[blockIdx.x], L, Cs);
+        const GemmTask t = tasks[blockIdx.x];
+        gemm_task<T>(pool, t, L, Cs);
+        if (t.syrk > 0) trsm_syrk<T>(pool, t, L, Cs);
     } else {
         // modes 0 and 1: the staged result Cs is written after the last read
         // of the operand staging L, so the two share LDS (four workgroups per
@@ -811,7 +830,9 @@ __global__ __launch_bounds__(kThreads) void k_upd_diag(T* pool, const NodeDev* n
         T B[2][64][kTile + 1];  // diagonal task: staged operands
     } U;
     if ((int)blockIdx.x >= ndiag) {
-        gemm_task<T>(pool, tiles[blockIdx.x - ndiag], U.L, S.A);
+        const GemmTask g = tiles[blockIdx.x - ndiag];
+        gemm_task<T>(pool, g, U.L, S.A);
+        if (g.syrk > 0) trsm_syrk<T>(pool, g, U.L, S.A);
         return;
     }
     const DiagTask t = diags[blockIdx.x];
@@ -866,7 +887,8 @@ __global__ __launch_bounds__(kThreads) void k_upd_diag(T* pool, const NodeDev* n
     typedef typename Mfma<T>::acc_t acc_t;
     acc_t c[2][2] = {{{0, 0, 0, 0}, {0, 0, 0, 0}}, {{0, 0, 0, 0}, {0, 0, 0, 0}}};
     if (kpp > 0) {
-        // the pending panels (64 columns per round): S.A -= Xd Xd^T,
+        // the pending panels (64 columns per round): S.A -= Xd Xd^T (unless
+        // the previous step's TRSM tile of these rows has applied it),
         // S.X -= Xd Xs^T
         acc_t c2[2][2] = {{{0, 0, 0, 0}, {0, 0, 0, 0}}, {{0, 0, 0, 0}, {0, 0, 0, 0}}};
         for (int q = 0; q < kpp; q += 64) {
@@ -889,10 +911,10 @@ __global__ __launch_bounds__(kThreads) void k_upd_diag(T* pool, const NodeDev* n
                 }
                 __syncthreads();
             }
-            lds_mma<T>(U.B[0], U.B[0], c);
+            if (!t.applied) lds_mma<T>(U.B[0], U.B[0], c);
             lds_mma<T>(U.B[0], U.B[1], c2);
         }
-        stage_acc<T>(c, S.A, true);
+        if (!t.applied) stage_acc<T>(c, S.A, true);
         stage_acc<T>(c2, S.X, true);
     }
     __syncthreads();

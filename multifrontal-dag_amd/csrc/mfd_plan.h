@@ -40,9 +40,12 @@ struct ExtTask {
 // update has applied yet); the factorizing workgroup also forms the
 // panel-s entries of its own rows (their TRSM with block s's inverse) and
 // stores them.
+// applied = 1: the kpp pending columns were already applied to the block
+// itself (by the previous step's TRSM tile of its rows, GemmTask::syrk); the
+// task still applies them to its rows of block s.
 struct DiagTask {
     int node, k0, kb, kprev;
-    int kpp, pad0, pad1, pad2;
+    int kpp, applied, pad1, pad2;
 };
 
 // One block step of the blocked supernodal solves: the pivot block
@@ -104,12 +107,19 @@ struct SuperInvTask {
 // li for the tile's rows, lj for its columns): the Schur update with the
 // extend-add of its result.  Offsets are into the front pool; tri != 0
 // keeps only the lower triangle (row >= col) of C.
+// syrk > 0 (a TRSM tile, modes 1 and 2, whose rows are the next pivot block
+// but one): afterwards the lower syrk x syrk part of the diagonal block at
+// cd (ld ldc) loses R R^T, R the tile's rows over every column from A to the
+// end of C (the pending panels and the panel just solved), so that the
+// diagonal task of the next step finds them applied.
 struct GemmTask {
     int64_t c, a, b, d, p;
     int ldc, lda, ldb;
     int m, n, k;
     int tri, mode;
     int kd, np, li, lj;
+    int syrk, pad;
+    int64_t cd;
 };
 
 // A run of K of one 128 x 128 tile: GEMM task `task`, K columns [k0, k1);

@@ -873,12 +873,20 @@ void build_plan(mfd_handle* h)
             const int pend = t == 0 ? 0 : ((t % period) == 0 ? period : t % period) * kPanel;
             const bool upd = t >= period && t % period == 0;
             int64_t f1 = (int64_t)h->gemm.size();
+            // the TRSM tile of the rows of block t+2 also applies the panels
+            // pending for block t+2 (and panel t) to its diagonal block, which
+            // the diagonal task of step t+1 then skips
+            auto syrk_at = [&](const NodeDev& d) { return k0 + 2 * kPanel < d.c; };
             for (int s : large) {
                 const NodeDev& d = h->nodes[s];
                 if (d.c <= k0) continue;
                 const int kb = std::min(kPanel, d.c - k0), kn = std::min(kPanel, d.c - k0 - kb);
                 for (int r0 = k0 + kb + kn; r0 < d.n; r0 += kTile) {
                     GemmTask g{};
+                    if (r0 == k0 + 2 * kPanel && syrk_at(d)) {
+                        g.syrk = std::min(kPanel, d.c - r0);
+                        g.cd = d.front + r0 + (int64_t)r0 * d.n;
+                    }
                     g.c = d.front + r0 + (int64_t)k0 * d.n;
                     g.ldc = g.lda = d.n;
                     g.m = std::min(kTile, d.n - r0);
@@ -909,8 +917,10 @@ void build_plan(mfd_handle* h)
                     for (int c0 = k0 + kPanel; c0 < d.c; c0 += kTile) {
                         const int nc = std::min(kTile, d.c - c0);
                         for (int r0 = c0; r0 < d.n; r0 += kTile) {
-                            // block t+1 itself belongs to its diagonal task
-                            const int rs = (c0 == k0 + kPanel && r0 == c0) ? c0 + nc : r0;
+                            // block t+1 itself belongs to its diagonal task,
+                            // block t+2 to the TRSM tile of its rows
+                            const bool own = c0 == k0 + kPanel || (c0 == k0 + 2 * kPanel && syrk_at(d));
+                            const int rs = (own && r0 == c0) ? c0 + nc : r0;
                             const int m = std::min(r0 + kTile, d.n) - rs;
                             if (m <= 0) continue;
                             GemmTask g{};
@@ -938,7 +948,10 @@ void build_plan(mfd_handle* h)
             for (int s : large) {
                 const NodeDev& d = h->nodes[s];
                 if (d.c <= k0 + kPanel) continue;
-                h->diag.push_back(DiagTask{s, k0 + kPanel, std::min(kPanel, d.c - k0 - kPanel), kPanel, pend, 0, 0, 0});
+                // from step 1 on, the previous step's TRSM tile of its rows
+                // has applied the pending panels to the block
+                h->diag.push_back(
+                    DiagTask{s, k0 + kPanel, std::min(kPanel, d.c - k0 - kPanel), kPanel, pend, t >= 1 ? 1 : 0, 0, 0});
             }
             const int64_t ng = (fuse ? f4u : f3) - f1, nd = (int64_t)h->diag.size() - f2;
             // many diagonal tasks (many fronts at this level): the GEMM tiles
@@ -1864,6 +1877,11 @@ struct ScheduleCheck {
             locate(g.d, rd, dr, dc);
             rect(rd, dr, dr + 1, 0, 1, node, task, false);
         }
+        if (g.syrk > 0) {  // the diagonal block of the tile's rows
+            int rs, sr, sc;
+            locate(g.cd, rs, sr, sc);
+            rect(rs, sr, sr + g.syrk, sc, sc + g.syrk, node, task, true);
+        }
         if (g.mode == 3) {
             const NodeDev& d = h->nodes[rc];
             parent_rect(rc, r - d.c, r - d.c + g.m, c - d.c, c - d.c + g.n, node, task);
@@ -2618,6 +2636,11 @@ int mfd_get_kernel_work(mfd_handle* h, int k, int s, double* flops, double* byte
     auto tri = [](double x) { return x * (x + 1) / 2; };
     // a GEMM tile of a step launch
     auto step_tile = [&](const GemmTask& g) {
+        if (g.syrk > 0) {  // the diagonal block of its rows: R R^T, K = the columns of A and C
+            const double kk = g.mode == 2 ? g.k + g.kd : g.k;
+            f += (double)g.syrk * (g.syrk + 1) * kk;
+            b += s * ((double)g.syrk * kk + 2.0 * tri(g.syrk));
+        }
         if (g.mode == 1) {  // C = C L^{-T}
             f += (double)g.m * g.n * g.n;
             b += s * (2.0 * g.m * g.n + (double)g.n * g.n);
@@ -2668,7 +2691,8 @@ int mfd_get_kernel_work(mfd_handle* h, int k, int s, double* flops, double* byte
             for (int64_t i = d0; i < d0 + dn; ++i) {
                 const DiagTask& t = h->diag[i];
                 const double kb = t.kb, kp = t.kprev, kpp = t.kpp;
-                f += 2.0 * kb * kb * kb / 3.0 + kb * (kb + 1) * (kp + kpp) + 2.0 * kb * kp * kpp + kb * kp * kp;
+                f += 2.0 * kb * kb * kb / 3.0 + kb * (kb + 1) * (kp + (t.applied ? 0 : kpp)) + 2.0 * kb * kp * kpp +
+                     kb * kp * kp;
                 b += s * (2.0 * tri(t.kb) + kb * kb + 2.0 * kb * kp + kp * kp + kpp * (kb + kp));
             }
             if (!l.fused) continue;
