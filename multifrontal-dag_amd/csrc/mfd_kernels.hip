@@ -712,17 +712,37 @@ __device__ __forceinline__ void stage_acc(const typename Mfma<T>::acc_t (&c)[2][
 // (TRSM, in-panel update, Schur update), so that profiles tell them apart.
 // The result goes through LDS so the write of C is coalesced along columns
 // (written straight from the accumulators it would be 32-byte pieces).
-template <typename T, int LP = kNative>
+template <typename T, int LP = kNative, bool PRE = true>
 __device__ __forceinline__ void gemm_task(T* pool, const GemmTask& t, TileLds<T>& L, T (*Cs)[kTile + 1],
                                           const int* lidx = nullptr)
 {
-    tile_product<T, LP>(pool + t.a, t.lda, pool + t.b, t.ldb, t.mode == 1, t.m, t.n, t.k, L, Cs);
     T* C = pool + t.c;
     const int tid = threadIdx.x, row = tid & 63;
+    // the entries of C this thread reads afterwards (row, column tid / 64 +
+    // 4 j), loaded before the K loop so that their latency hides behind it
+    // (from clamped positions, masked where used); PRE = false (the Schur
+    // tiles, whose K loop is long) loads them after it
+    T cpre[16];
+    if (!PRE && t.mode != 1) {
+        // loaded below, after the product
+    } else if (t.mode != 1) {
+        const int rc = min(row, t.m - 1);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) cpre[j] = C[rc + (int64_t)min((tid >> 6) + 4 * j, t.n - 1) * t.ldc];
+    }
+    tile_product<T, LP>(pool + t.a, t.lda, pool + t.b, t.ldb, t.mode == 1, t.m, t.n, t.k, L, Cs);
+    if (!PRE && t.mode != 1) {
+        const int rc = min(row, t.m - 1);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) cpre[j] = C[rc + (int64_t)min((tid >> 6) + 4 * j, t.n - 1) * t.ldc];
+    }
     if (t.mode == 2) {
         // Cs = C - A B^T, then C = Cs D^T (D = L^{-1}, stored transposed)
-        for (int col = tid >> 6; col < kTile; col += 4)
-            Cs[col][row] = (row < t.m && col < t.n) ? C[row + (int64_t)col * t.ldc] - Cs[col][row] : T(0);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int col = (tid >> 6) + 4 * j;
+            Cs[col][row] = (row < t.m && col < t.n) ? cpre[j] - Cs[col][row] : T(0);
+        }
         __syncthreads();
         typename Mfma<T>::acc_t c[2][2];
         tile_mma_ldsA<T>(Cs, pool + t.d, kPanel, true, t.n, t.kd, L, c);
@@ -742,13 +762,15 @@ __device__ __forceinline__ void gemm_task(T* pool, const GemmTask& t, TileLds<T>
         // compiler knows.
         const bool ea = t.mode == 3;
         T* P = ea ? pool + t.p + (lidx[t.li + row] - 1) : C;
-        for (int c0 = tid >> 6; c0 < t.n; c0 += 16) {
-            T cv[4], pv[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int c0 = (tid >> 6) + 16 * k;
+            if (c0 >= t.n) break;
+            T pv[4];
             int64_t po[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int col = min(c0 + 4 * q, t.n - 1);
-                cv[q] = C[row + (int64_t)col * t.ldc];
                 po[q] = ea ? (int64_t)(lidx[t.lj + col] - 1) * t.np : 0;
             }
 #pragma unroll
@@ -757,7 +779,7 @@ __device__ __forceinline__ void gemm_task(T* pool, const GemmTask& t, TileLds<T>
             for (int q = 0; q < 4; ++q) {
                 const int col = c0 + 4 * q;
                 if (col < t.n && (!t.tri || row >= col)) {
-                    const T v = cv[q] - Cs[col][row];
+                    const T v = cpre[4 * k + q] - Cs[col][row];
                     C[row + (int64_t)col * t.ldc] = v;
                     if (ea) P[po[q]] = pv[q] + v;
                 }
@@ -788,7 +810,7 @@ __device__ __forceinline__ void trsm_syrk(T* pool, const GemmTask& t, TileLds<T>
 // LP != kNative: the update and Schur tiles (KIND 1, 2) of the FP16_IR /
 // BF16_IR forms multiply 16-bit operands.
 template <typename T, int KIND, int LP = kNative>
-__global__ __launch_bounds__(kThreads) void k_gemm(T* pool, const GemmTask* tasks, const int* lidx)
+__global__ __launch_bounds__(kThreads, KIND == 0 ? 3 : 4) void k_gemm(T* pool, const GemmTask* tasks, const int* lidx)
 {
     static_assert(LP == kNative || KIND != 0, "the panel TRSM stays in the front's precision");
     if constexpr (KIND == 0) {
@@ -807,7 +829,7 @@ __global__ __launch_bounds__(kThreads) void k_gemm(T* pool, const GemmTask* task
             TileLds<T> L;
             T Cs[kTile][kTile + 1];
         } u;
-        gemm_task<T, LP>(pool, tasks[blockIdx.x], u.L, u.Cs, lidx);
+        gemm_task<T, LP, KIND != 2>(pool, tasks[blockIdx.x], u.L, u.Cs, lidx);
     }
 }
 
@@ -1192,7 +1214,7 @@ __global__ __launch_bounds__(kThreads) void k_diag(T* pool, const NodeDev* nodes
 template <typename T, int LP>
 __device__ __attribute__((noinline)) void front_tile(T* pool, GemmTask g, TileLds<T>& L, T (*Cs)[kTile + 1])
 {
-    gemm_task<T, LP>(pool, g, L, Cs);
+    gemm_task<T, LP, false>(pool, g, L, Cs);
 }
 template <typename T, int LP>
 __global__ __launch_bounds__(kThreads) void k_front_wg(T* pool, const NodeDev* nodes, const int* tasks, int* info,
