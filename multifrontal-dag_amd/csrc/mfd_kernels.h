@@ -72,6 +72,17 @@ void launch_fwd_super(const T* pool, const T* sinv, double* zpool, int nrhs, con
 template <typename T>
 void launch_bwd_super(const T* pool, const T* sinv, double* zpool, const double* part, double* x, int ldx, int nrhs,
                       const NodeDev* nodes, const SuperTask* tasks, int count, hipStream_t st);
+// One launch of a forward step's superblock tasks (xtasks, workgroups [0,
+// nx)) and its 64-column step tasks (the rest).
+template <typename T>
+void launch_fwd_pair(const T* pool, const T* sinv, double* zpool, int nrhs, const NodeDev* nodes,
+                     const SuperTask* xtasks, int nx, const SolveTask* tasks, int count, hipStream_t st);
+// One launch of a backward step's superblock top tasks (xtasks, workgroups
+// [0, nx)) and its step tasks (the rest).
+template <typename T>
+void launch_bwd_pair(const T* pool, const T* sinv, double* zpool, double* part, double* x, int ldx, int nrhs,
+                     const NodeDev* nodes, const SuperTask* xtasks, int nx, const SolveTask* tasks, int count,
+                     hipStream_t st);
 void launch_bwd_gather(double* zpool, int nrhs, const NodeDev* nodes, const int* lidx, const int2* tasks, int count,
                        hipStream_t st);
 template <typename T>

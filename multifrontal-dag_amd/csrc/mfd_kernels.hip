@@ -1279,6 +1279,37 @@ __global__ __launch_bounds__(kThreads) void k_front_wg(T* pool, const NodeDev* n
 // block, backward two (seq_forward_m.f90:14-81, seq_backward_m.f90:14-131).
 constexpr int kRhsGroup = 8;  // right-hand sides processed together
 
+// LDS of the solve tasks, named so that a launch holding two kinds of task
+// keeps them in one union.
+struct FwdStepLds {
+    union {
+        struct {
+            double LI[64][65];
+            double Lp[64][65];  // Lp[k][i] = L[k0 + i][k0 - 64 + k], block task
+        } b;
+        double red[4][64][kRhsGroup + 1];  // chunk task: partial sums per column quarter
+    } u;
+    double zt[64][kRhsGroup], yt[64][kRhsGroup];
+};
+struct SuperLds {
+    double zs[kSuper][kRhsGroup + 1], ys[kSuper][kRhsGroup + 1];  // the backward task's x_{u+1} in ys
+    double red[16][16][kRhsGroup + 1];
+};
+// block and chunk tasks of the backward step use disjoint layouts (two
+// workgroups per CU)
+union BwdStepLds {
+    struct {
+        double LI[64][65];             // the diagonal block's inverse
+        double Ln[64][65];             // Ln[k][i] = L[k0 + 64 + i][k0 + k]
+        double zt[64][kRhsGroup];
+        double Xn[64][kRhsGroup + 1];  // x of block t+1
+    } b;
+    struct {
+        double Ls[64][65];             // L columns, staged rows
+        double Xs[256][kRhsGroup + 1]; // x of the chunk's rows
+    } c;
+};
+
 // Forward, first launch of a level, task (front, first row): 256 rows of
 // the front's Z, pivot part from b, update part zero.
 __global__ __launch_bounds__(kThreads) void k_fwd_init(double* zpool, const double* b, int ldb, int nrhs,
@@ -1322,19 +1353,13 @@ __global__ __launch_bounds__(kThreads) void k_fwd_add(double* zpool, int nrhs, c
 // No task of a launch depends on another task of the same launch, and every
 // global read of a task is issued before its first wait.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_fwd_step(const T* pool, double* zpool, int nrhs, const NodeDev* nodes,
-                                                       const SolveTask* tasks)
+__device__ __forceinline__ void fwd_step_task(const T* pool, double* zpool, int nrhs, const NodeDev* nodes,
+                                              const SolveTask t, FwdStepLds& S)
 {
-    const SolveTask t = tasks[blockIdx.x];
     const NodeDev nd = nodes[t.node];
-    __shared__ union {
-        struct {
-            double LI[64][65];
-            double Lp[64][65];  // Lp[k][i] = L[k0 + i][k0 - 64 + k], block task
-        } b;
-        double red[4][64][kRhsGroup + 1];  // chunk task: partial sums per column quarter
-    } u;
-    __shared__ double zt[64][kRhsGroup], yt[64][kRhsGroup];
+    auto& u = S.u;
+    auto& zt = S.zt;
+    auto& yt = S.yt;
     double(*LI)[65] = u.b.LI;
     double(*Lp)[65] = u.b.Lp;
     const int n = nd.n, c = nd.c, kb = t.kb, tid = threadIdx.x, lane = tid & 63, ty = tid >> 6;
@@ -1527,6 +1552,13 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(const T* pool, double* zp
         __syncthreads();
     }
 }
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_fwd_step(const T* pool, double* zpool, int nrhs, const NodeDev* nodes,
+                                                       const SolveTask* tasks)
+{
+    __shared__ FwdStepLds S;
+    fwd_step_task<T>(pool, zpool, nrhs, nodes, tasks[blockIdx.x], S);
+}
 
 // ------------------------------------------------------------ superblocks
 // The three 256 x 256 matrices of superblock u, column-major, X at xoff, W
@@ -1651,17 +1683,17 @@ __global__ __launch_bounds__(kThreads) void k_superwv(const T* pool, T* sinv, co
 //     superblock u): Z[rows] -= L[rows, B_{u-1}] y_{u-1}.
 // Every matrix load of a task is issued before its first wait.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_fwd_super(const T* pool, const T* sinv, double* zpool, int nrhs,
-                                                        const NodeDev* nodes, const SuperTask* tasks)
+__device__ __forceinline__ void fwd_super_task(const T* pool, const T* sinv, double* zpool, int nrhs,
+                                               const NodeDev* nodes, const SuperTask t, SuperLds& S)
 {
-    const SuperTask t = tasks[blockIdx.x];
     const NodeDev nd = nodes[t.node];
     const int n = nd.n, c = nd.c, c0 = kSuper * t.u, cp = c0 - kSuper, tid = threadIdx.x;
     const int i = tid & 15, g = tid >> 4;
     double* Z = zpool + nd.z * nrhs;
     double* Y = Z + (int64_t)n * nrhs;
-    __shared__ double zs[kSuper][kRhsGroup + 1], ys[kSuper][kRhsGroup + 1];
-    __shared__ double red[16][16][kRhsGroup + 1];
+    auto& zs = S.zs;
+    auto& ys = S.ys;
+    auto& red = S.red;
     const int pb = t.pb, sb = t.sb;
     const bool top = t.kind == 1;
     T a[16], w[16];
@@ -1724,6 +1756,30 @@ __global__ __launch_bounds__(kThreads) void k_fwd_super(const T* pool, const T* 
         __syncthreads();
     }
 }
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_fwd_super(const T* pool, const T* sinv, double* zpool, int nrhs,
+                                                        const NodeDev* nodes, const SuperTask* tasks)
+{
+    __shared__ SuperLds S;
+    fwd_super_task<T>(pool, sinv, zpool, nrhs, nodes, tasks[blockIdx.x], S);
+}
+
+// A forward step's superblock tasks (workgroups [0, nx)) and 64-column step
+// tasks (the rest) in one launch: they touch different fronts.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_fwd_pair(const T* pool, const T* sinv, double* zpool, int nrhs,
+                                                       const NodeDev* nodes, const SuperTask* xtasks, int nx,
+                                                       const SolveTask* tasks)
+{
+    __shared__ union {
+        SuperLds x;
+        FwdStepLds s;
+    } u;
+    if ((int)blockIdx.x < nx)
+        fwd_super_task<T>(pool, sinv, zpool, nrhs, nodes, xtasks[blockIdx.x], u.x);
+    else
+        fwd_step_task<T>(pool, zpool, nrhs, nodes, tasks[blockIdx.x - nx], u.s);
+}
 
 // Backward sweep over superblocks, top tasks (kind 2), 16 rows of
 // superblock u each, in the launch of backward step u (thread: row
@@ -1735,18 +1791,18 @@ __global__ __launch_bounds__(kThreads) void k_fwd_super(const T* pool, const T* 
 // x_{u+1} from Z (the pivot part of x of these fronts lives in Z, so that
 // no task of a launch reads what another writes); x_u goes to Z and to x.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_bwd_super(const T* pool, const T* sinv, double* zpool, const double* part,
-                                                        double* x, int ldx, int nrhs, const NodeDev* nodes,
-                                                        const SuperTask* tasks)
+__device__ __forceinline__ void bwd_super_task(const T* pool, const T* sinv, double* zpool, const double* part,
+                                               double* x, int ldx, int nrhs, const NodeDev* nodes, const SuperTask t,
+                                               SuperLds& S)
 {
-    const SuperTask t = tasks[blockIdx.x];
     const NodeDev nd = nodes[t.node];
     const int n = nd.n, c = nd.c, c0 = kSuper * t.u, cn = c0 + kSuper, tid = threadIdx.x;
     const int i = tid & 15, g = tid >> 4;
     double* Z = zpool + nd.z * nrhs;
     const double* Y = Z + (int64_t)n * nrhs;
-    __shared__ double zs[kSuper][kRhsGroup + 1], xs[kSuper][kRhsGroup + 1];
-    __shared__ double red[16][16][kRhsGroup + 1];
+    auto& zs = S.zs;
+    auto& xs = S.ys;
+    auto& red = S.red;
     const int pb = t.pb, sb = t.sb;
     const T* X = sinv + t.xoff;
     const T* V = X + 2 * kSuper * kSuper;
@@ -1814,6 +1870,14 @@ __global__ __launch_bounds__(kThreads) void k_bwd_super(const T* pool, const T* 
         __syncthreads();
     }
 }
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_bwd_super(const T* pool, const T* sinv, double* zpool, const double* part,
+                                                        double* x, int ldx, int nrhs, const NodeDev* nodes,
+                                                        const SuperTask* tasks)
+{
+    __shared__ SuperLds S;
+    bwd_super_task<T>(pool, sinv, zpool, part, x, ldx, nrhs, nodes, tasks[blockIdx.x], S);
+}
 
 // Backward, first launch of a level, task (front, first row): 256 rows of
 // the update part of Z take the parent's solution at the parent-front
@@ -1851,26 +1915,11 @@ __global__ __launch_bounds__(kThreads) void k_bwd_gather(double* zpool, int nrhs
 // No task of a launch depends on another task of the same launch, and the
 // partials are summed in a fixed order.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zpool, double* part, double* x, int ldx,
-                                                       int nrhs, const NodeDev* nodes, const SolveTask* tasks)
+__device__ __forceinline__ void bwd_step_task(const T* pool, double* zpool, double* part, double* x, int ldx, int nrhs,
+                                              const NodeDev* nodes, const SolveTask t, BwdStepLds& u)
 {
-    const SolveTask t = tasks[blockIdx.x];
     const NodeDev nd = nodes[t.node];
     const int n = nd.n, c = nd.c, kb = t.kb, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    // block and chunk tasks use disjoint LDS layouts in one union (two
-    // workgroups per CU)
-    __shared__ union {
-        struct {
-            double LI[64][65];             // the diagonal block's inverse
-            double Ln[64][65];             // Ln[k][i] = L[k0 + 64 + i][k0 + k]
-            double zt[64][kRhsGroup];
-            double Xn[64][kRhsGroup + 1];  // x of block t+1
-        } b;
-        struct {
-            double Ls[64][65];             // L columns, staged rows
-            double Xs[256][kRhsGroup + 1]; // x of the chunk's rows
-        } c;
-    } u;
     double* Z = zpool + nd.z * nrhs;
     double* Y = Z + (int64_t)n * nrhs;
     const T* Lb = pool + nd.front + (int64_t)t.k0 * n;
@@ -2059,6 +2108,32 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zp
             x[(int64_t)(nd.first + i) + (int64_t)r * ldx] = Y[i + (int64_t)r * c];
         }
     }
+}
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zpool, double* part, double* x, int ldx,
+                                                       int nrhs, const NodeDev* nodes, const SolveTask* tasks)
+{
+    __shared__ BwdStepLds u;
+    bwd_step_task<T>(pool, zpool, part, x, ldx, nrhs, nodes, tasks[blockIdx.x], u);
+}
+
+// A backward step's superblock top tasks (workgroups [0, nx)) and its step
+// tasks (the rest) in one launch: the top tasks of superblock t read the
+// partial sums of the bank the step tasks do not write and x of superblock
+// t + 1, and write x of superblock t, which no step task of the launch reads.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_bwd_pair(const T* pool, const T* sinv, double* zpool, double* part,
+                                                       double* x, int ldx, int nrhs, const NodeDev* nodes,
+                                                       const SuperTask* xtasks, int nx, const SolveTask* tasks)
+{
+    __shared__ union {
+        SuperLds x;
+        BwdStepLds s;
+    } u;
+    if ((int)blockIdx.x < nx)
+        bwd_super_task<T>(pool, sinv, zpool, part, x, ldx, nrhs, nodes, xtasks[blockIdx.x], u.x);
+    else
+        bwd_step_task<T>(pool, zpool, part, x, ldx, nrhs, nodes, tasks[blockIdx.x - nx], u.s);
 }
 
 // ----------------------------------------------- permutations and residual
@@ -2373,6 +2448,23 @@ void launch_bwd_super(const T* pool, const T* sinv, double* zpool, const double*
         hipLaunchKernelGGL(k_bwd_super<T>, dim3(count), dim3(kThreads), 0, st, pool, sinv, zpool, part, x, ldx, nrhs,
                            nodes, tasks);
 }
+template <typename T>
+void launch_fwd_pair(const T* pool, const T* sinv, double* zpool, int nrhs, const NodeDev* nodes,
+                     const SuperTask* xtasks, int nx, const SolveTask* tasks, int count, hipStream_t st)
+{
+    if (nx + count > 0)
+        hipLaunchKernelGGL(k_fwd_pair<T>, dim3(nx + count), dim3(kThreads), 0, st, pool, sinv, zpool, nrhs, nodes, xtasks,
+                           nx, tasks);
+}
+template <typename T>
+void launch_bwd_pair(const T* pool, const T* sinv, double* zpool, double* part, double* x, int ldx, int nrhs,
+                     const NodeDev* nodes, const SuperTask* xtasks, int nx, const SolveTask* tasks, int count,
+                     hipStream_t st)
+{
+    if (nx + count > 0)
+        hipLaunchKernelGGL(k_bwd_pair<T>, dim3(nx + count), dim3(kThreads), 0, st, pool, sinv, zpool, part, x, ldx, nrhs,
+                           nodes, xtasks, nx, tasks);
+}
 void launch_bwd_gather(double* zpool, int nrhs, const NodeDev* nodes, const int* lidx, const int2* tasks, int count,
                        hipStream_t st)
 {
@@ -2423,6 +2515,10 @@ void launch_scatter_rows(double* out, int ld_out, const double* in, int ld_in, c
                                      const int*, hipStream_t, int);                                                    \
     template void launch_bwd_super<T>(const T*, const T*, double*, const double*, double*, int, int, const NodeDev*,    \
                                       const SuperTask*, int, hipStream_t);                                             \
+    template void launch_fwd_pair<T>(const T*, const T*, double*, int, const NodeDev*, const SuperTask*, int,          \
+                                     const SolveTask*, int, hipStream_t);                                              \
+    template void launch_bwd_pair<T>(const T*, const T*, double*, double*, double*, int, int, const NodeDev*,          \
+                                     const SuperTask*, int, const SolveTask*, int, hipStream_t);                       \
     template void launch_bwd_step<T>(const T*, double*, double*, double*, int, int, const NodeDev*, const SolveTask*,  \
                                      int, hipStream_t);
 MFD_INSTANTIATE(double)

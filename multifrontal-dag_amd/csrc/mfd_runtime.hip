@@ -55,6 +55,9 @@ struct Launch {
     // side_id) beside the next step's launch; wait_id >= 0: this launch
     // first waits for side launch wait_id.  Serial when profiling.
     int side = 0, side_id = -1, wait_id = -1;
+    // solve launches: the step they belong to (a step's kFS and kFX
+    // launches, and its kBS and kBX launches, run as one launch)
+    int step = -1;
 };
 
 // A step's update tiles join the step's launch (k_upd_diag, one workgroup
@@ -418,8 +421,11 @@ void build_solve_plan(mfd_handle* h)
     h->xbstep.clear();
     h->xinv.clear();
     h->sinv_elems = 0;
-    auto push = [](std::vector<Launch>& seq, int kind, int64_t first, int64_t end) {
-        if (end > first) seq.push_back(Launch{kind, first, (int)(end - first)});
+    auto push = [](std::vector<Launch>& seq, int kind, int64_t first, int64_t end, int step = -1) {
+        if (end > first) {
+            seq.push_back(Launch{kind, first, (int)(end - first)});
+            seq.back().step = step;
+        }
     };
     auto super = [&](const NodeDev& d) { return h->super_min > 0 && d.c >= h->super_min; };
     // steps of a level when superblock fronts step 256 columns at a time
@@ -528,8 +534,8 @@ void build_solve_plan(mfd_handle* h)
                         h->fstep.push_back(SolveTask{s, k0, kb, r0, 0, 0, 0, 0});
                 }
             }
-            push(h->fwd_seq, kFS, f0, (int64_t)h->fstep.size());
-            push(h->fwd_seq, kFX, x0, (int64_t)h->xstep.size());
+            push(h->fwd_seq, kFS, f0, (int64_t)h->fstep.size(), t);
+            push(h->fwd_seq, kFX, x0, (int64_t)h->xstep.size(), t);
         }
     }
     // chunks of the partial sums of block j of front s: rows [min(64 (j + 2), c), n)
@@ -627,8 +633,8 @@ void build_solve_plan(mfd_handle* h)
                                 SolveTask{s, k0, kb, rc.first + 256 * q, (int)((j % 2) * bank + idx), 0, split ? cq : -1, 0});
                 }
             }
-            push(h->bwd_seq, kBS, f0, (int64_t)h->bstep.size());
-            push(h->bwd_seq, kBX, x0, (int64_t)h->xbstep.size());
+            push(h->bwd_seq, kBS, f0, (int64_t)h->bstep.size(), t);
+            push(h->bwd_seq, kBX, x0, (int64_t)h->xbstep.size(), t);
         }
     }
     h->part_slots = 2 * bank;
@@ -1448,7 +1454,21 @@ template <typename T> void enqueue_solve(mfd_handle* h, const double* bp, double
     if (profile)
         for (auto& e : pe) (void)hipEventCreate(&e);
     if (profile) (void)hipEventRecord(pe[0], st);
-    for (const Launch& l : h->fwd_seq) {
+    // a step's kFS and kFX launches (kBS and kBX) touch different fronts (the
+    // kBX tasks read the partial-sum bank the kBS tasks do not write): one
+    // launch of both
+    auto paired = [](const std::vector<Launch>& seq, size_t i, int a, int b) {
+        return seq[i].kind == a && i + 1 < seq.size() && seq[i + 1].kind == b && seq[i].step == seq[i + 1].step;
+    };
+    int nfwd = 0, nbwd = 0;  // launches issued
+    for (size_t i = 0; i < h->fwd_seq.size(); ++i, ++nfwd) {
+        const Launch& l = h->fwd_seq[i];
+        if (paired(h->fwd_seq, i, kFS, kFX)) {
+            const Launch& x = h->fwd_seq[++i];
+            launch_fwd_pair<T>(pool, sinv_of<T>(h), h->d_z, nrhs, h->d_nodes, h->d_xstep + x.first, x.count,
+                               h->d_fstep + l.first, l.count, st);
+            continue;
+        }
         switch (l.kind) {
         case kFI: launch_fwd_init(h->d_z, bp, n, nrhs, h->d_nodes, h->d_fi + l.first, l.count, st); break;
         case kFA: launch_fwd_add(h->d_z, nrhs, h->d_nodes, h->d_lidx, h->d_fa + l.first, l.count, st); break;
@@ -1460,7 +1480,14 @@ template <typename T> void enqueue_solve(mfd_handle* h, const double* bp, double
         }
     }
     if (profile) (void)hipEventRecord(pe[1], st);
-    for (const Launch& l : h->bwd_seq) {
+    for (size_t i = 0; i < h->bwd_seq.size(); ++i, ++nbwd) {
+        const Launch& l = h->bwd_seq[i];
+        if (paired(h->bwd_seq, i, kBS, kBX)) {
+            const Launch& x = h->bwd_seq[++i];
+            launch_bwd_pair<T>(pool, sinv_of<T>(h), h->d_z, h->d_part, xp, n, nrhs, h->d_nodes, h->d_xbstep + x.first,
+                               x.count, h->d_bstep + l.first, l.count, st);
+            continue;
+        }
         switch (l.kind) {
         case kBG: launch_bwd_gather(h->d_z, nrhs, h->d_nodes, h->d_lidx, h->d_bg + l.first, l.count, st); break;
         case kBS:
@@ -1480,9 +1507,9 @@ template <typename T> void enqueue_solve(mfd_handle* h, const double* bp, double
         (void)hipEventElapsedTime(&a, pe[0], pe[1]);
         (void)hipEventElapsedTime(&b, pe[1], pe[2]);
         h->ktime[MFD_K_SOLVE_FWD] += a;
-        h->kcount[MFD_K_SOLVE_FWD] += (int)h->fwd_seq.size();
+        h->kcount[MFD_K_SOLVE_FWD] += nfwd;
         h->ktime[MFD_K_SOLVE_BWD] += b;
-        h->kcount[MFD_K_SOLVE_BWD] += (int)h->bwd_seq.size();
+        h->kcount[MFD_K_SOLVE_BWD] += nbwd;
         for (auto e : pe) (void)hipEventDestroy(e);
     }
 }
