@@ -391,10 +391,11 @@ int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 // ------------------------------------------------------------- schedule
 enum SolveKind { kFI = 100, kFA, kFS, kBG, kBS, kFX, kBX };
 
-// Solve schedule.  Forward, per level: a gather launch, then launches
-// t = 0 .. steps: the block task of block t (which first applies block t - 1
-// to block t) and the row chunks applying block t - 1 to the rows after
-// block t.  Fronts with at least super_min pivot columns step over
+// Solve schedule.  Forward: one launch setting every front's Z from b,
+// then per level: one launch per child rank adding the children's update
+// parts, then launches t = 0 .. steps: the block task of block t (which
+// first applies block t - 1 to block t) and the row chunks applying block
+// t - 1 to the rows after block t.  Fronts with at least super_min pivot columns step over
 // superblocks of 256 columns instead (kind kFX, launched beside step t's
 // kFS launch): the superblock task of superblock t and the row chunks
 // applying superblock t - 1 to the rows after it.
@@ -471,15 +472,18 @@ void build_solve_plan(mfd_handle* h)
         }
     }
     h->xinv_ptr[5] = (int)h->xinv.size();
+    // every front's Z (pivot part from b, update part zero) in one launch:
+    // the fronts' Z regions are disjoint
+    for (int L = 0; L < h->num_levels; ++L)
+        for (int k = h->level_ptr[L]; k < h->level_ptr[L + 1]; ++k) {
+            const int s = h->level_nodes[k];
+            for (int r0 = 0; r0 < h->nodes[s].n; r0 += 256) h->fi.push_back(make_int2(s, r0));
+        }
+    push(h->fwd_seq, kFI, 0, (int64_t)h->fi.size());
     for (int L = 0; L < h->num_levels; ++L) {
         const int b0 = h->level_ptr[L], b1 = h->level_ptr[L + 1];
         {
-            int64_t f = (int64_t)h->fi.size();
-            for (int k = b0; k < b1; ++k) {
-                const int s = h->level_nodes[k];
-                for (int r0 = 0; r0 < h->nodes[s].n; r0 += 256) h->fi.push_back(make_int2(s, r0));
-            }
-            push(h->fwd_seq, kFI, f, (int64_t)h->fi.size());
+            int64_t f;
             // children's update parts, one launch per child rank
             size_t maxch = 0;
             for (int k = b0; k < b1; ++k) {
