@@ -58,6 +58,15 @@ void launch_fwd_add(double* zpool, int nrhs, const NodeDev* nodes, const int* li
 template <typename T>
 void launch_fwd_step(const T* pool, double* zpool, int nrhs, const NodeDev* nodes, const SolveTask* tasks, int count,
                      hipStream_t st);
+// The chunk tasks of a forward step alone (row0 >= 0), several workgroups
+// per CU.
+template <typename T>
+void launch_fwd_chunk(const T* pool, double* zpool, int nrhs, const NodeDev* nodes, const SolveTask* tasks, int count,
+                      hipStream_t st);
+// The chunk tasks of a backward step alone (row0 >= 0).
+template <typename T>
+void launch_bwd_chunk(const T* pool, const double* zpool, double* part, int nrhs, const NodeDev* nodes,
+                      const SolveTask* tasks, int count, hipStream_t st);
 // 64 x 64 blocks of the superblock matrices: X by block distance (wv =
 // false), then the W and V blocks (wv = true, MFMA tile products)
 template <typename T>

@@ -1297,6 +1297,10 @@ struct SuperLds {
 };
 // block and chunk tasks of the backward step use disjoint layouts (two
 // workgroups per CU)
+struct BwdChunkLds {
+    double Ls[64][65];              // L columns, staged rows
+    double Xs[256][kRhsGroup + 1];  // x of the chunk's rows
+};
 union BwdStepLds {
     struct {
         double LI[64][65];             // the diagonal block's inverse
@@ -1304,10 +1308,7 @@ union BwdStepLds {
         double zt[64][kRhsGroup];
         double Xn[64][kRhsGroup + 1];  // x of block t+1
     } b;
-    struct {
-        double Ls[64][65];             // L columns, staged rows
-        double Xs[256][kRhsGroup + 1]; // x of the chunk's rows
-    } c;
+    BwdChunkLds c;
 };
 
 // Forward, first launch of a level, task (front, first row): 256 rows of
@@ -1352,6 +1353,67 @@ __global__ __launch_bounds__(kThreads) void k_fwd_add(double* zpool, int nrhs, c
 //     threads per row, 16 columns each, summed through LDS).
 // No task of a launch depends on another task of the same launch, and every
 // global read of a task is issued before its first wait.
+// Forward chunk task (row0 >= 0, pad0 != 1): block B = k0/64 applied to the
+// 64 rows from row0 (all after the block the step solves), Z[rows] -=
+// L[rows, B] y_B (yt: staging of y_B; red: the four column quarters' partial
+// sums).
+template <typename T>
+__device__ __forceinline__ void fwd_chunk_task(const T* pool, double* zpool, int nrhs, const NodeDev* nodes,
+                                               const SolveTask t, double (&yt)[64][kRhsGroup],
+                                               double (&red)[4][64][kRhsGroup + 1])
+{
+    const NodeDev nd = nodes[t.node];
+    const int n = nd.n, c = nd.c, kb = t.kb, tid = threadIdx.x, lane = tid & 63, ty = tid >> 6;
+    double* Z = zpool + nd.z * nrhs;
+    const double* Y = Z + (int64_t)n * nrhs;
+    const T* Lb = pool + nd.front + (int64_t)t.k0 * n;
+    // 64 rows; thread (row lane, quarter ty) takes columns 16 ty .. 16 ty + 15
+    const int row = t.row0 + lane;
+    const bool live = row < n;
+    // Unconditional loads from clamped (in-bounds) positions, kept in the
+    // raw element type (a conditional or converted load would wait for
+    // the previous one) and not masked: past column kb they multiply the
+    // zeros of yt, and rows past n are not stored, so the z and y loads
+    // below go out without waiting for these.
+    const int rowc = min(row, n - 1);
+    T l[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) l[j] = ld_off(Lb, (unsigned)(rowc + min(16 * ty + j, kb - 1) * n));
+    for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
+        const int nr = min(kRhsGroup, nrhs - r0);
+        // this thread stores right-hand sides r0 + ty and r0 + ty + 4
+        const double z0 = (live && ty < nr) ? Z[row + (int64_t)(r0 + ty) * n] : 0.0;
+        const double z1 = (live && ty + 4 < nr) ? Z[row + (int64_t)(r0 + ty + 4) * n] : 0.0;
+        for (int e = tid; e < 64 * kRhsGroup; e += kThreads) {
+            const int k = e & 63, r = e >> 6;
+            yt[k][r] = (k < kb && r < nr) ? Y[t.k0 + k + (int64_t)(r0 + r) * c] : 0.0;
+        }
+        __syncthreads();
+        double acc[kRhsGroup];
+#pragma unroll
+        for (int r = 0; r < kRhsGroup; ++r) acc[r] = 0.0;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const double lk = (double)l[j];
+#pragma unroll
+            for (int r = 0; r < kRhsGroup; ++r) acc[r] += lk * yt[16 * ty + j][r];
+        }
+#pragma unroll
+        for (int r = 0; r < kRhsGroup; ++r) red[ty][lane][r] = acc[r];
+        __syncthreads();
+        if (live) {
+            if (ty < nr)
+                Z[row + (int64_t)(r0 + ty) * n] =
+                    z0 - ((red[0][lane][ty] + red[1][lane][ty]) + (red[2][lane][ty] + red[3][lane][ty]));
+            if (ty + 4 < nr)
+                Z[row + (int64_t)(r0 + ty + 4) * n] =
+                    z1 - ((red[0][lane][ty + 4] + red[1][lane][ty + 4]) +
+                          (red[2][lane][ty + 4] + red[3][lane][ty + 4]));
+        }
+        __syncthreads();
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ void fwd_step_task(const T* pool, double* zpool, int nrhs, const NodeDev* nodes,
                                               const SolveTask t, FwdStepLds& S)
@@ -1439,51 +1501,7 @@ __device__ __forceinline__ void fwd_step_task(const T* pool, double* zpool, int 
         return;
     }
     if (t.row0 >= 0) {
-        // 64 rows; thread (row lane, quarter ty) takes columns 16 ty .. 16 ty + 15
-        const int row = t.row0 + lane;
-        const bool live = row < n;
-        // Unconditional loads from clamped (in-bounds) positions, kept in the
-        // raw element type (a conditional or converted load would wait for
-        // the previous one) and not masked: past column kb they multiply the
-        // zeros of yt, and rows past n are not stored, so the z and y loads
-        // below go out without waiting for these.
-        const int rowc = min(row, n - 1);
-        T l[16];
-#pragma unroll
-        for (int j = 0; j < 16; ++j) l[j] = ld_off(Lb, (unsigned)(rowc + min(16 * ty + j, kb - 1) * n));
-        for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
-            const int nr = min(kRhsGroup, nrhs - r0);
-            // this thread stores right-hand sides r0 + ty and r0 + ty + 4
-            const double z0 = (live && ty < nr) ? Z[row + (int64_t)(r0 + ty) * n] : 0.0;
-            const double z1 = (live && ty + 4 < nr) ? Z[row + (int64_t)(r0 + ty + 4) * n] : 0.0;
-            for (int e = tid; e < 64 * kRhsGroup; e += kThreads) {
-                const int k = e & 63, r = e >> 6;
-                yt[k][r] = (k < kb && r < nr) ? Y[t.k0 + k + (int64_t)(r0 + r) * c] : 0.0;
-            }
-            __syncthreads();
-            double acc[kRhsGroup];
-#pragma unroll
-            for (int r = 0; r < kRhsGroup; ++r) acc[r] = 0.0;
-#pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                const double lk = (double)l[j];
-#pragma unroll
-                for (int r = 0; r < kRhsGroup; ++r) acc[r] += lk * yt[16 * ty + j][r];
-            }
-#pragma unroll
-            for (int r = 0; r < kRhsGroup; ++r) u.red[ty][lane][r] = acc[r];
-            __syncthreads();
-            if (live) {
-                if (ty < nr)
-                    Z[row + (int64_t)(r0 + ty) * n] =
-                        z0 - ((u.red[0][lane][ty] + u.red[1][lane][ty]) + (u.red[2][lane][ty] + u.red[3][lane][ty]));
-                if (ty + 4 < nr)
-                    Z[row + (int64_t)(r0 + ty + 4) * n] =
-                        z1 - ((u.red[0][lane][ty + 4] + u.red[1][lane][ty + 4]) +
-                              (u.red[2][lane][ty + 4] + u.red[3][lane][ty + 4]));
-            }
-            __syncthreads();
-        }
+        fwd_chunk_task<T>(pool, zpool, nrhs, nodes, t, yt, u.red);
         return;
     }
     // block task
@@ -1558,6 +1576,17 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(const T* pool, double* zp
 {
     __shared__ FwdStepLds S;
     fwd_step_task<T>(pool, zpool, nrhs, nodes, tasks[blockIdx.x], S);
+}
+
+// A forward step's chunk tasks alone (steps with many of them): without the
+// block task's LDS and registers several workgroups share a CU.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_fwd_chunk(const T* pool, double* zpool, int nrhs, const NodeDev* nodes,
+                                                        const SolveTask* tasks)
+{
+    __shared__ double yt[64][kRhsGroup];
+    __shared__ double red[4][64][kRhsGroup + 1];
+    fwd_chunk_task<T>(pool, zpool, nrhs, nodes, tasks[blockIdx.x], yt, red);
 }
 
 // ------------------------------------------------------------ superblocks
@@ -1914,14 +1943,17 @@ __global__ __launch_bounds__(kThreads) void k_bwd_gather(double* zpool, int nrhs
 //     after block t, so its x is final).
 // No task of a launch depends on another task of the same launch, and the
 // partials are summed in a fixed order.
+// Backward chunk task (row0 >= 0): the partial sums of block k0/64 (all 64
+// columns, pad0 < 0, or columns 16 pad0 .. 16 pad0 + 15) over the 256 rows
+// from row0, into slot `slot` of part.
 template <typename T>
-__device__ __forceinline__ void bwd_step_task(const T* pool, double* zpool, double* part, double* x, int ldx, int nrhs,
-                                              const NodeDev* nodes, const SolveTask t, BwdStepLds& u)
+__device__ __forceinline__ void bwd_chunk_task(const T* pool, const double* zpool, double* part, int nrhs,
+                                               const NodeDev* nodes, const SolveTask t, BwdChunkLds& S)
 {
     const NodeDev nd = nodes[t.node];
     const int n = nd.n, c = nd.c, kb = t.kb, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    double* Z = zpool + nd.z * nrhs;
-    double* Y = Z + (int64_t)n * nrhs;
+    const double* Z = zpool + nd.z * nrhs;
+    const double* Y = Z + (int64_t)n * nrhs;
     const T* Lb = pool + nd.front + (int64_t)t.k0 * n;
     if (t.row0 >= 0 && t.pad0 < 0) {
         // all 64 columns in one task: wave w takes columns 16 w .. 16 w + 15,
@@ -1942,7 +1974,7 @@ __device__ __forceinline__ void bwd_step_task(const T* pool, double* zpool, doub
             const int nr = min(kRhsGroup, nrhs - r0);
 #pragma unroll
             for (int r = 0; r < kRhsGroup; ++r)
-                u.c.Xs[tid][r] = (live && r < nr) ? ((row < c && !nd.xz) ? Y[row + (int64_t)(r0 + r) * c]
+                S.Xs[tid][r] = (live && r < nr) ? ((row < c && !nd.xz) ? Y[row + (int64_t)(r0 + r) * c]
                                                                           : Z[row + (int64_t)(r0 + r) * n])
                                               : 0.0;
             acc_t acc = {0, 0, 0, 0};
@@ -1951,14 +1983,14 @@ __device__ __forceinline__ void bwd_step_task(const T* pool, double* zpool, doub
                 __syncthreads();
                 if (wv == sb) {
 #pragma unroll
-                    for (int k = 0; k < 64; ++k) u.c.Ls[k][lane] = l[k];
+                    for (int k = 0; k < 64; ++k) S.Ls[k][lane] = l[k];
                 }
                 __syncthreads();
 #pragma unroll
                 for (int q = 0; q < 16; ++q) {
                     const int rr = 4 * q + (lane >> 4);
-                    const double a = u.c.Ls[16 * wv + rc][rr];
-                    const double b = rc < nr ? u.c.Xs[64 * sb + rr][rc] : 0.0;
+                    const double a = S.Ls[16 * wv + rc][rr];
+                    const double b = rc < nr ? S.Xs[64 * sb + rr][rc] : 0.0;
                     acc = Mfma<double>::mma(a, b, acc);
                 }
             }
@@ -1987,14 +2019,14 @@ __device__ __forceinline__ void bwd_step_task(const T* pool, double* zpool, doub
         T l[16];
 #pragma unroll
         for (int j = 0; j < 16; ++j) l[j] = ld_off(Lb, (unsigned)(rowc + min(col0 + j, kb - 1) * n));
-        double(*Lc)[kThreads + 4] = reinterpret_cast<double(*)[kThreads + 4]>(&u.c.Ls[0][0]);   // 16 x 260
-        double(*red)[16][17] = reinterpret_cast<double(*)[16][17]>(&u.c.Ls[0][0]);              // 4 x 16 x 17
+        double(*Lc)[kThreads + 4] = reinterpret_cast<double(*)[kThreads + 4]>(&S.Ls[0][0]);   // 16 x 260
+        double(*red)[16][17] = reinterpret_cast<double(*)[16][17]>(&S.Ls[0][0]);              // 4 x 16 x 17
         const int rc = lane & 15;
         for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
             const int nr = min(kRhsGroup, nrhs - r0);
 #pragma unroll
             for (int r = 0; r < kRhsGroup; ++r)
-                u.c.Xs[tid][r] = (live && r < nr) ? ((row < c && !nd.xz) ? Y[row + (int64_t)(r0 + r) * c]
+                S.Xs[tid][r] = (live && r < nr) ? ((row < c && !nd.xz) ? Y[row + (int64_t)(r0 + r) * c]
                                                                           : Z[row + (int64_t)(r0 + r) * n])
                                               : 0.0;
 #pragma unroll
@@ -2005,7 +2037,7 @@ __device__ __forceinline__ void bwd_step_task(const T* pool, double* zpool, doub
             for (int q = 0; q < 16; ++q) {
                 const int rr = 64 * wv + 4 * q + (lane >> 4);
                 const double a = Lc[rc][rr];
-                const double b = rc < nr ? u.c.Xs[rr][rc] : 0.0;
+                const double b = rc < nr ? S.Xs[rr][rc] : 0.0;
                 acc = Mfma<double>::mma(a, b, acc);
             }
             __syncthreads();
@@ -2019,6 +2051,21 @@ __device__ __forceinline__ void bwd_step_task(const T* pool, double* zpool, doub
             }
             __syncthreads();
         }
+        return;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void bwd_step_task(const T* pool, double* zpool, double* part, double* x, int ldx, int nrhs,
+                                              const NodeDev* nodes, const SolveTask t, BwdStepLds& u)
+{
+    const NodeDev nd = nodes[t.node];
+    const int n = nd.n, c = nd.c, kb = t.kb, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    double* Z = zpool + nd.z * nrhs;
+    double* Y = Z + (int64_t)n * nrhs;
+    const T* Lb = pool + nd.front + (int64_t)t.k0 * n;
+    if (t.row0 >= 0) {
+        bwd_chunk_task<T>(pool, zpool, part, nrhs, nodes, t, u.c);
         return;
     }
     // block task: every global read first
@@ -2115,6 +2162,16 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(const T* pool, double* zp
 {
     __shared__ BwdStepLds u;
     bwd_step_task<T>(pool, zpool, part, x, ldx, nrhs, nodes, tasks[blockIdx.x], u);
+}
+
+// A backward step's chunk tasks alone (steps with many of them): without the
+// block task's LDS and registers more workgroups share a CU.
+template <typename T>
+__global__ __launch_bounds__(kThreads, 3) void k_bwd_chunk(const T* pool, const double* zpool, double* part, int nrhs,
+                                                        const NodeDev* nodes, const SolveTask* tasks)
+{
+    __shared__ BwdChunkLds S;
+    bwd_chunk_task<T>(pool, zpool, part, nrhs, nodes, tasks[blockIdx.x], S);
 }
 
 // A backward step's superblock top tasks (workgroups [0, nx)) and its step
@@ -2424,6 +2481,20 @@ void launch_fwd_step(const T* pool, double* zpool, int nrhs, const NodeDev* node
     if (count > 0) hipLaunchKernelGGL(k_fwd_step<T>, dim3(count), dim3(kThreads), 0, st, pool, zpool, nrhs, nodes, tasks);
 }
 template <typename T>
+void launch_fwd_chunk(const T* pool, double* zpool, int nrhs, const NodeDev* nodes, const SolveTask* tasks, int count,
+                      hipStream_t st)
+{
+    if (count > 0)
+        hipLaunchKernelGGL(k_fwd_chunk<T>, dim3(count), dim3(kThreads), 0, st, pool, zpool, nrhs, nodes, tasks);
+}
+template <typename T>
+void launch_bwd_chunk(const T* pool, const double* zpool, double* part, int nrhs, const NodeDev* nodes,
+                      const SolveTask* tasks, int count, hipStream_t st)
+{
+    if (count > 0)
+        hipLaunchKernelGGL(k_bwd_chunk<T>, dim3(count), dim3(kThreads), 0, st, pool, zpool, part, nrhs, nodes, tasks);
+}
+template <typename T>
 void launch_superinv(const T* pool, T* sinv, const NodeDev* nodes, const SuperInvTask* tasks, int count, bool wv,
                      hipStream_t st)
 {
@@ -2508,6 +2579,9 @@ void launch_scatter_rows(double* out, int ld_out, const double* in, int ld_in, c
     template void launch_scatter_a_scaled<T>(T*, const double*, const int64_t*, const int64_t*, const int2*,           \
                                              const double*, int64_t, hipStream_t);                                    \
     template void launch_fwd_step<T>(const T*, double*, int, const NodeDev*, const SolveTask*, int, hipStream_t);       \
+    template void launch_fwd_chunk<T>(const T*, double*, int, const NodeDev*, const SolveTask*, int, hipStream_t);      \
+    template void launch_bwd_chunk<T>(const T*, const double*, double*, int, const NodeDev*, const SolveTask*, int,    \
+                                      hipStream_t);                                                                    \
     template void launch_superinv<T>(const T*, T*, const NodeDev*, const SuperInvTask*, int, bool, hipStream_t);      \
     template void launch_fwd_super<T>(const T*, const T*, double*, int, const NodeDev*, const SuperTask*, int,         \
                                       hipStream_t);                                                                    \

@@ -389,7 +389,12 @@ void free_device(mfd_handle* h)
 int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 // ------------------------------------------------------------- schedule
-enum SolveKind { kFI = 100, kFA, kFS, kBG, kBS, kFX, kBX };
+enum SolveKind { kFI = 100, kFA, kFS, kBG, kBS, kFX, kBX, kFC, kBC };
+
+// A solve step with at least this many chunk tasks launches them apart from
+// its block tasks (k_fwd_chunk, k_bwd_chunk: more workgroups per CU than the
+// block tasks' two).
+constexpr int kChunkLaunch = 1024;
 
 // Solve schedule.  Forward: one launch setting every front's Z from b,
 // then per level: one launch per child rank adding the children's update
@@ -502,8 +507,10 @@ void build_solve_plan(mfd_handle* h)
             }
         }
         const int steps = fsteps_of(L);
+        std::vector<SolveTask> chunks;
         for (int t = 0; t <= steps; ++t) {
             int64_t f0 = (int64_t)h->fstep.size(), x0 = (int64_t)h->xstep.size();
+            chunks.clear();
             for (int k = b0; k < b1; ++k) {
                 const int s = h->level_nodes[k];
                 const NodeDev& d = h->nodes[s];
@@ -535,11 +542,20 @@ void build_solve_plan(mfd_handle* h)
                     // block t - 1 applied to the rows after block t
                     const int k0 = 64 * (t - 1), kb = std::min(64, d.c - k0);
                     for (int r0 = std::min(64 * (t + 1), d.c); r0 < d.n; r0 += 64)
-                        h->fstep.push_back(SolveTask{s, k0, kb, r0, 0, 0, 0, 0});
+                        chunks.push_back(SolveTask{s, k0, kb, r0, 0, 0, 0, 0});
                 }
             }
+            // the step's chunk tasks after its block tasks, in the same
+            // launch, or in a launch of their own when there are many
+            const bool apart = (int)chunks.size() >= kChunkLaunch;
+            if (!apart) h->fstep.insert(h->fstep.end(), chunks.begin(), chunks.end());
             push(h->fwd_seq, kFS, f0, (int64_t)h->fstep.size(), t);
             push(h->fwd_seq, kFX, x0, (int64_t)h->xstep.size(), t);
+            if (apart) {
+                const int64_t c0 = (int64_t)h->fstep.size();
+                h->fstep.insert(h->fstep.end(), chunks.begin(), chunks.end());
+                push(h->fwd_seq, kFC, c0, (int64_t)h->fstep.size(), t);
+            }
         }
     }
     // chunks of the partial sums of block j of front s: rows [min(64 (j + 2), c), n)
@@ -572,6 +588,7 @@ void build_solve_plan(mfd_handle* h)
         }
     }
     std::vector<int> qfirst(h->nodes.size(), 0), qcount(h->nodes.size(), 0);
+    std::vector<SolveTask> bchunks;  // a backward step's chunk tasks
     for (int L = h->num_levels - 1; L >= 0; --L) {
         const int b0 = h->level_ptr[L], b1 = h->level_ptr[L + 1];
         {
@@ -585,6 +602,7 @@ void build_solve_plan(mfd_handle* h)
         }
         for (int t = fsteps_of(L); t >= 0; --t) {
             int64_t f0 = (int64_t)h->bstep.size(), x0 = (int64_t)h->xbstep.size();
+            bchunks.clear();
             int idx = 0;
             // few chunks (latency-bound steps): four 16-column tasks per
             // chunk; many: one task per chunk
@@ -615,7 +633,7 @@ void build_solve_plan(mfd_handle* h)
                             const int k0 = kSuper * w + 64 * j, kb = std::min(64, d.c - k0);
                             for (int q = 0; q < rc.second; ++q, ++idx)
                                 for (int cq = 0; cq < (split ? 4 : 1); ++cq)
-                                    h->bstep.push_back(SolveTask{s, k0, kb, rc.first + 256 * q,
+                                    bchunks.push_back(SolveTask{s, k0, kb, rc.first + 256 * q,
                                                                  (int)((w % 2) * bank + idx), 0, split ? cq : -1, 0});
                         }
                     }
@@ -633,12 +651,19 @@ void build_solve_plan(mfd_handle* h)
                     qcount[s] = rc.second;
                     for (int q = 0; q < rc.second; ++q, ++idx)
                         for (int cq = 0; cq < (split ? 4 : 1); ++cq)
-                            h->bstep.push_back(
+                            bchunks.push_back(
                                 SolveTask{s, k0, kb, rc.first + 256 * q, (int)((j % 2) * bank + idx), 0, split ? cq : -1, 0});
                 }
             }
+            const bool apart = (int)bchunks.size() >= kChunkLaunch;
+            if (!apart) h->bstep.insert(h->bstep.end(), bchunks.begin(), bchunks.end());
             push(h->bwd_seq, kBS, f0, (int64_t)h->bstep.size(), t);
             push(h->bwd_seq, kBX, x0, (int64_t)h->xbstep.size(), t);
+            if (apart) {
+                const int64_t c0 = (int64_t)h->bstep.size();
+                h->bstep.insert(h->bstep.end(), bchunks.begin(), bchunks.end());
+                push(h->bwd_seq, kBC, c0, (int64_t)h->bstep.size(), t);
+            }
         }
     }
     h->part_slots = 2 * bank;
@@ -1477,6 +1502,7 @@ template <typename T> void enqueue_solve(mfd_handle* h, const double* bp, double
         case kFI: launch_fwd_init(h->d_z, bp, n, nrhs, h->d_nodes, h->d_fi + l.first, l.count, st); break;
         case kFA: launch_fwd_add(h->d_z, nrhs, h->d_nodes, h->d_lidx, h->d_fa + l.first, l.count, st); break;
         case kFS: launch_fwd_step<T>(pool, h->d_z, nrhs, h->d_nodes, h->d_fstep + l.first, l.count, st); break;
+        case kFC: launch_fwd_chunk<T>(pool, h->d_z, nrhs, h->d_nodes, h->d_fstep + l.first, l.count, st); break;
         case kFX:
             launch_fwd_super<T>(pool, sinv_of<T>(h), h->d_z, nrhs, h->d_nodes, h->d_xstep + l.first, l.count, st);
             break;
@@ -1496,6 +1522,9 @@ template <typename T> void enqueue_solve(mfd_handle* h, const double* bp, double
         case kBG: launch_bwd_gather(h->d_z, nrhs, h->d_nodes, h->d_lidx, h->d_bg + l.first, l.count, st); break;
         case kBS:
             launch_bwd_step<T>(pool, h->d_z, h->d_part, xp, n, nrhs, h->d_nodes, h->d_bstep + l.first, l.count, st);
+            break;
+        case kBC:
+            launch_bwd_chunk<T>(pool, h->d_z, h->d_part, nrhs, h->d_nodes, h->d_bstep + l.first, l.count, st);
             break;
         case kBX:
             launch_bwd_super<T>(pool, sinv_of<T>(h), h->d_z, h->d_part, xp, n, nrhs, h->d_nodes, h->d_xbstep + l.first,
