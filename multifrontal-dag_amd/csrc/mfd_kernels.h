@@ -100,12 +100,16 @@ void launch_bwd_step(const T* pool, double* zpool, double* part, double* x, int 
 void launch_set_int(int* p, int v, hipStream_t st);
 void launch_residual(double* r, const double* b, const double* x, int n, int nrhs, const int* ptr, const int* col,
                      const int64_t* src, const double* a, hipStream_t st);
-void launch_absmax(unsigned long long* out, const double* x, int64_t n, int ld, int nrhs, hipStream_t st);
-// Sums of squares of the columns of x, as sumsq_parts(n) (<= kSumParts)
-// partials per column at part + k * sumsq_parts(n).
+// Partials per column of the sums of squares in launch_err_norms:
+// sumsq_parts(n) <= kSumParts.
 constexpr int kSumParts = 256;
 int sumsq_parts(int64_t n);
-void launch_sumsq(double* part, const double* x, int64_t n, int ld, int nrhs, hipStream_t st);
+// The maxima of |r|, |x|, |b| and (d != nullptr) |d| per column into out
+// (at 0, nrhs, 2 nrhs, 3 nrhs; zeroed by the caller) and the sums of squares
+// of r and x as sumsq_parts(n) partials per column (part, then part +
+// sumsq_parts(n) nrhs), in one pass.
+void launch_err_norms(unsigned long long* out, double* part, const double* r, const double* x, const double* b,
+                      const double* d, int64_t n, int nrhs, hipStream_t st);
 void launch_norm_inf(unsigned long long* out, int n, const int* ptr, const int64_t* src, const double* a,
                      hipStream_t st);
 void launch_axpy(double* x, const double* d, int64_t n, hipStream_t st);

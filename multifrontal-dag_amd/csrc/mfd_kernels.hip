@@ -2246,36 +2246,51 @@ __device__ __forceinline__ void block_max_to(unsigned long long* out, double m)
     }
 }
 
-// out[k] = max(out[k], max_i |x[i + k*ld]|) as IEEE bits (order-preserving
-// for non-negative doubles, so the atomic max is exact and deterministic).
-__global__ __launch_bounds__(kThreads) void k_absmax(unsigned long long* out, const double* x, int64_t n, int ld,
-                                                     int nrhs)
+// The norms a backward error needs, in one pass, per column k (blockIdx.y)
+// of the n x nrhs vectors r, x, b and (if not null) d, ld n: the maxima of
+// |r|, |x|, |b|, |d| into out[k], out[nrhs + k], out[2 nrhs + k], out[3 nrhs
+// + k] (IEEE bits, one atomic per workgroup: order-preserving for
+// non-negative doubles, so exact), and the sums of squares of r and x as
+// per-workgroup partials part[k parts + w], part[(nrhs + k) parts + w] (parts
+// = gridDim.x), which the caller adds in order, so the sums do not depend on
+// the timing of the workgroups.
+__global__ __launch_bounds__(kThreads) void k_err_norms(unsigned long long* out, double* part, const double* r,
+                                                        const double* x, const double* b, const double* d, int64_t n,
+                                                        int nrhs)
 {
+    __shared__ double red[6][kThreads / 64];
     const int k = blockIdx.y;
-    double m = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
-        m = fmax(m, fabs(x[i + (int64_t)k * ld]));
-    block_max_to(out + k, m);
-}
-
-// part[k * gridDim.x + b] = sum over block b's share of x[:, k] of x^2.
-// The caller adds the gridDim.x partials of each column in order, so the sum
-// does not depend on the timing of the blocks.
-__global__ __launch_bounds__(kThreads) void k_sumsq(double* part, const double* x, int64_t n, int ld)
-{
-    __shared__ double wsum[kThreads / 64];
-    const int k = blockIdx.y;
-    double s = 0.0;
+    double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // max |r|, |x|, |b|, |d|; sum r^2, x^2
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-        const double v = x[i + (int64_t)k * ld];
-        s += v * v;
+        const int64_t e = i + (int64_t)k * n;
+        const double rv = r[e], xv = x[e], bv = b[e], dv = d ? d[e] : 0.0;
+        v[0] = fmax(v[0], fabs(rv));
+        v[1] = fmax(v[1], fabs(xv));
+        v[2] = fmax(v[2], fabs(bv));
+        v[3] = fmax(v[3], fabs(dv));
+        v[4] += rv * rv;
+        v[5] += xv * xv;
     }
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const double o = __shfl_xor(v[q], off);
+            v[q] = q < 4 ? fmax(v[q], o) : v[q] + o;
+        }
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) red[q][threadIdx.x >> 6] = v[q];
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int w = 1; w < kThreads / 64; ++w) s += wsum[w];
-        part[(int64_t)k * gridDim.x + blockIdx.x] = s;
+        for (int w = 1; w < kThreads / 64; ++w)
+#pragma unroll
+            for (int q = 0; q < 6; ++q) v[q] = q < 4 ? fmax(v[q], red[q][w]) : v[q] + red[q][w];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (q < 3 || d) atomicMax(out + q * nrhs + k, (unsigned long long)__double_as_longlong(v[q]));
+        part[(int64_t)k * gridDim.x + blockIdx.x] = v[4];
+        part[(int64_t)(nrhs + k) * gridDim.x + blockIdx.x] = v[5];
     }
 }
 
@@ -2310,19 +2325,18 @@ void launch_residual(double* r, const double* b, const double* x, int n, int nrh
         hipLaunchKernelGGL(k_residual, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, st, r, b, x, n, nrhs, ptr,
                            col, src, a);
 }
-void launch_absmax(unsigned long long* out, const double* x, int64_t n, int ld, int nrhs, hipStream_t st)
+// at least 8 entries per thread: few workgroups per column, so few atomics
+// on each maximum and a short read-back of the partial sums
+int sumsq_parts(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(n / (8 * kThreads), kSumParts)); }
+void launch_err_norms(unsigned long long* out, double* part, const double* r, const double* x, const double* b,
+                      const double* d, int64_t n, int nrhs, hipStream_t st)
 {
-    int g = (int)std::min<int64_t>((n + kThreads - 1) / kThreads, 256);
-    if (n > 0) hipLaunchKernelGGL(k_absmax, dim3(std::max(g, 1), nrhs), dim3(kThreads), 0, st, out, x, n, ld, nrhs);
-}
-int sumsq_parts(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + kThreads - 1) / kThreads, kSumParts)); }
-void launch_sumsq(double* part, const double* x, int64_t n, int ld, int nrhs, hipStream_t st)
-{
-    if (n > 0) hipLaunchKernelGGL(k_sumsq, dim3(sumsq_parts(n), nrhs), dim3(kThreads), 0, st, part, x, n, ld);
+    if (n > 0)
+        hipLaunchKernelGGL(k_err_norms, dim3(sumsq_parts(n), nrhs), dim3(kThreads), 0, st, out, part, r, x, b, d, n, nrhs);
 }
 void launch_norm_inf(unsigned long long* out, int n, const int* ptr, const int64_t* src, const double* a, hipStream_t st)
 {
-    const int g = std::min((n + kThreads - 1) / kThreads, 256);
+    const int g = std::min((n + kThreads - 1) / kThreads, 4096);
     if (n > 0) hipLaunchKernelGGL(k_norm_inf, dim3(g), dim3(kThreads), 0, st, out, n, ptr, src, a);
 }
 void launch_axpy(double* x, const double* d, int64_t n, hipStream_t st)

@@ -246,6 +246,7 @@ struct mfd_handle {
     int64_t csr_nnz = 0;
     double *d_r = nullptr, *d_d = nullptr;
     unsigned long long* d_norms = nullptr;
+    double anorm = -1;  // ||A||_inf of the factored values (< 0: not formed yet)
     // mfd_solve_device_noalloc: device address of the fixed scratch array on
     // this handle's GPU (nullptr if it could not be resolved), and, during
     // such a call, no_alloc = true (no allocation, no graph capture; the
@@ -1442,6 +1443,7 @@ int factorize_common(mfd_handle* h, int precision)
     if (precision < MFD_FP64 || precision > MFD_BF16_IR) return MFD_ERR_UNSUPPORTED;
     if (!h->hplan.roots.empty() && precision != MFD_FP64) return MFD_ERR_UNSUPPORTED;
     h->factored = false;
+    h->anorm = -1;  // the values may change
     h->solved_nrhs = 0;
     int rc = MFD_OK;
     if (precision == MFD_FP16_IR || precision == MFD_BF16_IR) {
@@ -1591,13 +1593,10 @@ int backward_error(mfd_handle* h, const double* bp, const double* xp, double* r,
     double* sq = reinterpret_cast<double*>(h->d_norms + nm);
     launch_residual(r, bp, xp, n, nrhs, h->d_csr_ptr, h->d_csr_col, h->d_csr_src, h->d_a, st);
     launch_zero_u64(h->d_norms, nm, st);
-    launch_absmax(h->d_norms, r, n, n, nrhs, st);
-    launch_absmax(h->d_norms + nrhs, xp, n, n, nrhs, st);
-    launch_absmax(h->d_norms + 2 * nrhs, bp, n, n, nrhs, st);
-    if (d) launch_absmax(h->d_norms + 3 * nrhs, d, n, n, nrhs, st);
-    launch_norm_inf(h->d_norms + 4 * nrhs, n, h->d_csr_ptr, h->d_csr_src, h->d_a, st);
-    launch_sumsq(sq, r, n, n, nrhs, st);
-    launch_sumsq(sq + (int64_t)parts * nrhs, xp, n, n, nrhs, st);
+    launch_err_norms(h->d_norms, sq, r, xp, bp, d, n, nrhs, st);
+    // ||A||_inf once per factorization (the values do not change in between)
+    const bool anorm_known = h->anorm >= 0;
+    if (!anorm_known) launch_norm_inf(h->d_norms + 4 * nrhs, n, h->d_csr_ptr, h->d_csr_src, h->d_a, st);
     std::vector<unsigned long long> v(nm + 2 * parts * nrhs);
     HIP_TRY(hipMemcpyAsync(v.data(), h->d_norms, v.size() * 8, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -1615,7 +1614,9 @@ int backward_error(mfd_handle* h, const double* bp, const double* xp, double* r,
     h->err_eta.assign(nrhs, 0.0);
     h->err_res2.assign(nrhs, 0.0);
     if (d) h->err_ferr.assign(nrhs, 0.0);
-    double an = val(4 * nrhs), worst = 0, worst2 = 0;
+    if (!anorm_known) h->anorm = val(4 * nrhs);
+    const double an = h->anorm;
+    double worst = 0, worst2 = 0;
     for (int k = 0; k < nrhs; ++k) {
         const double e = ratio(val(k), an * val(nrhs + k) + val(2 * nrhs + k));
         if (!(e <= worst)) worst = e;  // NaN propagates
