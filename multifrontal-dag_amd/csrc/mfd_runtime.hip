@@ -1671,7 +1671,7 @@ int refine(mfd_handle* h, int nrhs, double* eta, double* res2, int* iters)
         if (!(*eta > kRefineTarget) || it >= kMaxRefine || !(*eta < 0.5 * prev)) break;
         prev = *eta;
         MFD_TRY(correction_solve(h, h->d_r, h->d_d, nrhs));
-        for (int k = 0; k < nrhs; ++k) launch_axpy(h->d_xp + k * n, h->d_d + k * n, n, st);
+        launch_axpy(h->d_xp, h->d_d, n * nrhs, st);  // both n x nrhs with ld n
         ++it;
     }
     *iters = it;
