@@ -224,8 +224,9 @@ int mfd_get_kernel_time(mfd_handle* h, int k, double* total_ms, int* launches);
 /* Algorithmic work of kernel class k over one factorization (k < MFD_K_SOLVE_FWD)
    or one single-RHS solve (solve classes) in the current schedule: flops
    (2 per multiply-add of the entries the kernel must produce) and the HBM
-   bytes the class must move at minimum (each operand read once, each result
-   written once) for an element size of elem_bytes. */
+   bytes the class must move at minimum (within each launch every operand
+   read once, however many tiles read it, and each result written once) for
+   an element size of elem_bytes. */
 int mfd_get_kernel_work(mfd_handle* h, int k, int elem_bytes, double* flops, double* bytes);
 
 /* Fill-reducing ordering in place of METIS_NodeND (hppllt.f90:47-51,

@@ -2695,23 +2695,35 @@ int mfd_get_kernel_work(mfd_handle* h, int k, int s, double* flops, double* byte
     if (!h->analyzed) return MFD_ERR_STATE;
     double f = 0, b = 0;
     auto tri = [](double x) { return x * (x + 1) / 2; };
+    // operands read by more than one tile of a launch (the row blocks of a
+    // panel, a diagonal block's inverse) count once per launch
+    std::set<int64_t> seen;
+    auto once = [&](int64_t at, double elems) {
+        if (seen.insert(at).second) b += s * elems;
+    };
     // a GEMM tile of a step launch
     auto step_tile = [&](const GemmTask& g) {
         if (g.syrk > 0) {  // the diagonal block of its rows: R R^T, K = the columns of A and C
             const double kk = g.mode == 2 ? g.k + g.kd : g.k;
             f += (double)g.syrk * (g.syrk + 1) * kk;
-            b += s * ((double)g.syrk * kk + 2.0 * tri(g.syrk));
+            b += s * 2.0 * tri(g.syrk);  // R: the tile's own rows, counted below
         }
         if (g.mode == 1) {  // C = C L^{-T}
             f += (double)g.m * g.n * g.n;
-            b += s * (2.0 * g.m * g.n + (double)g.n * g.n);
+            b += s * 2.0 * g.m * g.n;
+            once(g.b, (double)g.n * g.n);
         } else if (g.mode == 2) {  // C = (C - A B^T) L^{-T}
             f += 2.0 * g.m * g.n * g.k + (double)g.m * g.n * g.n;
-            b += s * ((double)g.m * g.k + (double)g.n * g.k + 2.0 * g.m * g.n + (double)g.n * g.n);
+            b += s * 2.0 * g.m * g.n;
+            once(g.a, (double)g.m * g.k);
+            once(g.b, (double)g.n * g.k);
+            once(g.d, (double)g.n * g.n);
         } else {
             double e = g.tri ? (double)g.n * g.m - tri(g.n - 1) : (double)g.m * g.n;
             f += 2.0 * g.k * e;
-            b += s * ((double)g.m * g.k + (double)g.n * g.k + 2.0 * e);
+            b += s * 2.0 * e;
+            once(g.a, (double)g.m * g.k);
+            once(g.b, (double)g.n * g.k);
         }
     };
     switch (k) {
@@ -2757,12 +2769,14 @@ int mfd_get_kernel_work(mfd_handle* h, int k, int s, double* flops, double* byte
                 b += s * (2.0 * tri(t.kb) + kb * kb + 2.0 * kb * kp + kp * kp + kpp * (kb + kp));
             }
             if (!l.fused) continue;
+            seen.clear();
             for (int64_t i = l.first; i < l.first + l.count; ++i) step_tile(h->gemm[i]);
         }
         break;
     case MFD_K_TRSM:  // a step's GEMM tiles launched apart from its diagonal tasks
         for (const Launch& l : h->fseq) {
             if (l.kind != k) continue;
+            seen.clear();
             for (int64_t i = l.first; i < l.first + l.count; ++i) step_tile(h->gemm[i]);
         }
         break;
@@ -2770,11 +2784,14 @@ int mfd_get_kernel_work(mfd_handle* h, int k, int s, double* flops, double* byte
     case MFD_K_SCHUR:
         for (const Launch& l : h->fseq) {
             if (l.kind != k) continue;
+            seen.clear();
             for (int64_t i = l.first; i < l.first + l.count; ++i) {
                 const GemmTask& g = h->gemm[i];
                 double e = g.tri ? (double)g.n * g.m - tri(g.n - 1) : (double)g.m * g.n;
                 f += 2.0 * g.k * e;
-                b += s * ((double)g.m * g.k + (double)g.n * g.k + 2.0 * e);
+                b += s * 2.0 * e;  // C read and written
+                once(g.a, (double)g.m * g.k);  // the panel's row blocks, once each
+                once(g.b, (double)g.n * g.k);
                 if (g.mode == 3) {  // the extend-add: e adds into the parent, plus the index maps
                     f += e;
                     b += 2.0 * s * e + 4.0 * (g.m + g.n);
