@@ -99,7 +99,9 @@ void launch_bwd_step(const T* pool, double* zpool, double* part, double* x, int 
                      const SolveTask* tasks, int count, hipStream_t st);
 void launch_set_int(int* p, int v, hipStream_t st);
 void launch_residual(double* r, const double* b, const double* x, int n, int nrhs, const int* ptr, const int* col,
-                     const int64_t* src, const double* a, hipStream_t st);
+                     const double* val, hipStream_t st);
+// val[e] = a[src[e]] for e < nnz (the matrix values in CSR order).
+void launch_gather_vals(double* val, const double* a, const int64_t* src, int64_t nnz, hipStream_t st);
 // Partials per column of the sums of squares in launch_err_norms:
 // sumsq_parts(n) <= kSumParts.
 constexpr int kSumParts = 256;
@@ -110,8 +112,7 @@ int sumsq_parts(int64_t n);
 // sumsq_parts(n) nrhs), in one pass.
 void launch_err_norms(unsigned long long* out, double* part, const double* r, const double* x, const double* b,
                       const double* d, int64_t n, int nrhs, hipStream_t st);
-void launch_norm_inf(unsigned long long* out, int n, const int* ptr, const int64_t* src, const double* a,
-                     hipStream_t st);
+void launch_norm_inf(unsigned long long* out, int n, const int* ptr, const double* val, hipStream_t st);
 void launch_axpy(double* x, const double* d, int64_t n, hipStream_t st);
 void launch_zero_u64(unsigned long long* p, int n, hipStream_t st);
 void launch_gather_rows(double* out, int ld_out, const double* in, int ld_in, const int* perm, int n, int nrhs,

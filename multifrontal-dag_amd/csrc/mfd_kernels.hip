@@ -2214,22 +2214,35 @@ __global__ __launch_bounds__(kThreads) void k_scatter_rows(double* out, int ld_o
 }
 
 // ------------------------------------------------------ FP64 refinement
-// r = b - A x for the full symmetric matrix in CSR (postordered numbering);
-// entry e's value is a[src[e]] (values stay in the caller's order).  One
-// wave per row, so the row sum is reduced in a fixed order.
-// One thread per row (rows of a sparse matrix are short).
+// r = b - A x over the full symmetric matrix in CSR order (val: the values
+// in that order, gathered once per factorization), four right-hand sides per
+// pass over a row; each sum runs over the row's entries in order.
 __global__ __launch_bounds__(kThreads) void k_residual(double* r, const double* b, const double* x, int n, int nrhs,
-                                                       const int* ptr, const int* col, const int64_t* src,
-                                                       const double* a)
+                                                       const int* ptr, const int* col, const double* val)
 {
     const int row = blockIdx.x * kThreads + threadIdx.x;
     if (row >= n) return;
     const int e0 = ptr[row], e1 = ptr[row + 1];
-    for (int k = 0; k < nrhs; ++k) {
-        double acc = 0.0;
-        for (int e = e0; e < e1; ++e) acc += a[src[e]] * x[col[e] + (int64_t)k * n];
-        r[row + (int64_t)k * n] = b[row + (int64_t)k * n] - acc;
+    for (int k0 = 0; k0 < nrhs; k0 += 4) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int e = e0; e < e1; ++e) {
+            const double v = val[e];
+            const double* xc = x + col[e] + (int64_t)k0 * n;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (k0 + q < nrhs) acc[q] += v * xc[(int64_t)q * n];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (k0 + q < nrhs) r[row + (int64_t)(k0 + q) * n] = b[row + (int64_t)(k0 + q) * n] - acc[q];
     }
+}
+
+// val[e] = a[src[e]]: the caller's values in CSR order
+__global__ __launch_bounds__(kThreads) void k_gather_vals(double* val, const double* a, const int64_t* src, int64_t nnz)
+{
+    const int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (e < nnz) val[e] = a[src[e]];
 }
 
 // Maximum over the workgroup, then one atomic per workgroup (as IEEE bits:
@@ -2294,14 +2307,14 @@ __global__ __launch_bounds__(kThreads) void k_err_norms(unsigned long long* out,
     }
 }
 
-// out[0] = max over rows of sum_e |a[src[e]]|  (the infinity norm of A)
+// out[0] = max over rows of sum_e |val[e]|  (the infinity norm of A)
 __global__ __launch_bounds__(kThreads) void k_norm_inf(unsigned long long* out, int n, const int* ptr,
-                                                       const int64_t* src, const double* a)
+                                                       const double* val)
 {
     double m = 0.0;
     for (int row = blockIdx.x * kThreads + threadIdx.x; row < n; row += gridDim.x * kThreads) {
         double s = 0.0;
-        for (int e = ptr[row]; e < ptr[row + 1]; ++e) s += fabs(a[src[e]]);
+        for (int e = ptr[row]; e < ptr[row + 1]; ++e) s += fabs(val[e]);
         m = fmax(m, s);
     }
     block_max_to(out, m);
@@ -2319,11 +2332,17 @@ __global__ void k_zero_u64(unsigned long long* p, int n)
 }
 
 void launch_residual(double* r, const double* b, const double* x, int n, int nrhs, const int* ptr, const int* col,
-                     const int64_t* src, const double* a, hipStream_t st)
+                     const double* val, hipStream_t st)
 {
     if (n > 0)
         hipLaunchKernelGGL(k_residual, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, st, r, b, x, n, nrhs, ptr,
-                           col, src, a);
+                           col, val);
+}
+void launch_gather_vals(double* val, const double* a, const int64_t* src, int64_t nnz, hipStream_t st)
+{
+    if (nnz > 0)
+        hipLaunchKernelGGL(k_gather_vals, dim3((unsigned)((nnz + kThreads - 1) / kThreads)), dim3(kThreads), 0, st, val, a,
+                           src, nnz);
 }
 // at least 8 entries per thread: few workgroups per column, so few atomics
 // on each maximum and a short read-back of the partial sums
@@ -2334,10 +2353,10 @@ void launch_err_norms(unsigned long long* out, double* part, const double* r, co
     if (n > 0)
         hipLaunchKernelGGL(k_err_norms, dim3(sumsq_parts(n), nrhs), dim3(kThreads), 0, st, out, part, r, x, b, d, n, nrhs);
 }
-void launch_norm_inf(unsigned long long* out, int n, const int* ptr, const int64_t* src, const double* a, hipStream_t st)
+void launch_norm_inf(unsigned long long* out, int n, const int* ptr, const double* val, hipStream_t st)
 {
     const int g = std::min((n + kThreads - 1) / kThreads, 4096);
-    if (n > 0) hipLaunchKernelGGL(k_norm_inf, dim3(g), dim3(kThreads), 0, st, out, n, ptr, src, a);
+    if (n > 0) hipLaunchKernelGGL(k_norm_inf, dim3(g), dim3(kThreads), 0, st, out, n, ptr, val);
 }
 void launch_axpy(double* x, const double* d, int64_t n, hipStream_t st)
 {

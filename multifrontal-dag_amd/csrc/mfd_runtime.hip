@@ -243,6 +243,7 @@ struct mfd_handle {
     // refinement
     int *d_csr_ptr = nullptr, *d_csr_col = nullptr;
     int64_t* d_csr_src = nullptr;  // index into the caller-order values of each CSR entry
+    double* d_csr_val = nullptr;   // the values in CSR order (gathered with ||A||_inf)
     int64_t csr_nnz = 0;
     double *d_r = nullptr, *d_d = nullptr;
     unsigned long long* d_norms = nullptr;
@@ -383,6 +384,7 @@ void free_device(mfd_handle* h)
     h->d_z = h->d_bin = h->d_bp = h->d_xp = nullptr;
     h->d_csr_ptr = h->d_csr_col = nullptr;
     h->d_csr_src = nullptr;
+    h->d_csr_val = nullptr;
     h->d_r = h->d_d = nullptr;
     h->d_norms = nullptr;
 }
@@ -1193,6 +1195,7 @@ int setup_device(mfd_handle* h)
     MFD_TRY(upload(h, &h->d_csr_ptr, cptr));
     MFD_TRY(upload(h, &h->d_csr_col, ccol));
     MFD_TRY(upload(h, &h->d_csr_src, csrc));
+    MFD_TRY(dalloc(h, &h->d_csr_val, h->csr_nnz));
     return MFD_OK;
 }
 
@@ -1591,12 +1594,14 @@ int backward_error(mfd_handle* h, const double* bp, const double* xp, double* r,
     hipStream_t st = h->stream;
     const int nm = 4 * nrhs + 1, parts = sumsq_parts(n);
     double* sq = reinterpret_cast<double*>(h->d_norms + nm);
-    launch_residual(r, bp, xp, n, nrhs, h->d_csr_ptr, h->d_csr_col, h->d_csr_src, h->d_a, st);
+    // the values in CSR order and ||A||_inf once per factorization (the
+    // values do not change in between)
+    const bool anorm_known = h->anorm >= 0;
+    if (!anorm_known) launch_gather_vals(h->d_csr_val, h->d_a, h->d_csr_src, h->csr_nnz, st);
+    launch_residual(r, bp, xp, n, nrhs, h->d_csr_ptr, h->d_csr_col, h->d_csr_val, st);
     launch_zero_u64(h->d_norms, nm, st);
     launch_err_norms(h->d_norms, sq, r, xp, bp, d, n, nrhs, st);
-    // ||A||_inf once per factorization (the values do not change in between)
-    const bool anorm_known = h->anorm >= 0;
-    if (!anorm_known) launch_norm_inf(h->d_norms + 4 * nrhs, n, h->d_csr_ptr, h->d_csr_src, h->d_a, st);
+    if (!anorm_known) launch_norm_inf(h->d_norms + 4 * nrhs, n, h->d_csr_ptr, h->d_csr_val, st);
     std::vector<unsigned long long> v(nm + 2 * parts * nrhs);
     HIP_TRY(hipMemcpyAsync(v.data(), h->d_norms, v.size() * 8, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
