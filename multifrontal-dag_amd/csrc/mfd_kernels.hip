@@ -526,54 +526,69 @@ template <typename T> struct TileLds {
 //
 // LP = kF16 / kBF16 (FP32 operands only) rounds each operand to 16 bits as
 // it leaves LDS and multiplies on the 16-bit MFMA with FP32 accumulation.
-template <typename T, int LP = kNative>
+// TB: 0 = B stored like A (B[col + kk * ldb]), 1 = B transposed in memory,
+// -1 = as the runtime flag tb says (the panel TRSM kernel holds both).
+template <typename T, int LP = kNative, int TB = -1>
 __device__ __forceinline__ void tile_mma(const T* A, int lda, const T* B, int ldb, bool tb, int m, int n, int k,
-                                         TileLds<T>& L, typename Mfma<T>::acc_t (&c)[2][2])
+                                         TileLds<T>& L, typename Mfma<T>::acc_t (&c)[2][2], int tid = threadIdx.x)
 {
     static_assert(LP == kNative || sizeof(T) == 4, "16-bit operands come from FP32 fronts");
     typedef typename Mfma<T>::acc_t acc_t;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const bool trb = TB < 0 ? tb : TB == 1;
+    const int lane = tid & 63, wv = tid >> 6;
     const int wr = (wv & 1) * 32, wc = (wv >> 1) * 32;
     const int li = tid & 63, lp = tid >> 6;
     acc_t c00 = {0, 0, 0, 0}, c01 = {0, 0, 0, 0}, c10 = {0, 0, 0, 0}, c11 = {0, 0, 0, 0};
     T ra[4], rb[4];
     // transposed-B chunk: element e = tid + 256 p is (col e / 16, kk e % 16)
     const int tcol = tid >> 4, tkk = tid & 15;
+    // Every load from a clamped in-bounds position (m, n, k >= 1), its value
+    // kept raw and masked only as it is stored to LDS: no branch around the
+    // loads, which all go out together.
+    const int lic = min(li, m - 1), lib = min(li, n - 1);
     auto load = [&](int k0) {
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
-            int kk = k0 + lp + 4 * p;
-            ra[p] = (li < m && kk < k) ? A[li + (int64_t)kk * lda] : T(0);
-            if (tb) {
-                int col = tcol + 16 * p, kt = k0 + tkk;
-                rb[p] = (col < n && kt < k) ? B[(int64_t)col * ldb + kt] : T(0);
-            } else {
-                rb[p] = (li < n && kk < k) ? B[li + (int64_t)kk * ldb] : T(0);
-            }
+            const int kc = min(k0 + lp + 4 * p, k - 1);
+            ra[p] = A[lic + (int64_t)kc * lda];
+            if (trb)
+                rb[p] = B[(int64_t)min(tcol + 16 * p, n - 1) * ldb + min(k0 + tkk, k - 1)];
+            else
+                rb[p] = B[lib + (int64_t)kc * ldb];
         }
     };
     load(0);
     for (int k0 = 0; k0 < k; k0 += 16) {
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
-            L.As[lp + 4 * p][li] = ra[p];
-            if (tb)
-                L.Bs[tkk][tcol + 16 * p] = rb[p];
-            else
-                L.Bs[lp + 4 * p][li] = rb[p];
+            const int kk = k0 + lp + 4 * p;
+            L.As[lp + 4 * p][li] = (li < m && kk < k) ? ra[p] : T(0);
+            if (trb) {
+                const int col = tcol + 16 * p;
+                L.Bs[tkk][col] = (col < n && k0 + tkk < k) ? rb[p] : T(0);
+            } else {
+                L.Bs[lp + 4 * p][li] = (li < n && kk < k) ? rb[p] : T(0);
+            }
         }
         __syncthreads();
         if (k0 + 16 < k) load(k0 + 16);
         if constexpr (LP == kNative) {
+            // the chunk's operands read from LDS together, then its MFMAs
+            T a0[4], a1[4], b0[4], b1[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int kr = q * 4 + (lane >> 4);
-                const T a0 = L.As[kr][wr + (lane & 15)], a1 = L.As[kr][wr + 16 + (lane & 15)];
-                const T b0 = L.Bs[kr][wc + (lane & 15)], b1 = L.Bs[kr][wc + 16 + (lane & 15)];
-                c00 = Mfma<T>::mma(a0, b0, c00);
-                c01 = Mfma<T>::mma(a0, b1, c01);
-                c10 = Mfma<T>::mma(a1, b0, c10);
-                c11 = Mfma<T>::mma(a1, b1, c11);
+                a0[q] = L.As[kr][wr + (lane & 15)];
+                a1[q] = L.As[kr][wr + 16 + (lane & 15)];
+                b0[q] = L.Bs[kr][wc + (lane & 15)];
+                b1[q] = L.Bs[kr][wc + 16 + (lane & 15)];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                c00 = Mfma<T>::mma(a0[q], b0[q], c00);
+                c01 = Mfma<T>::mma(a0[q], b1[q], c01);
+                c10 = Mfma<T>::mma(a1[q], b0[q], c10);
+                c11 = Mfma<T>::mma(a1[q], b1[q], c11);
             }
         } else {
             typedef Mfma16<LP> M16;
@@ -599,12 +614,12 @@ __device__ __forceinline__ void tile_mma(const T* A, int lda, const T* B, int ld
     c[1][1] = c11;
 }
 // The same product staged as Cs[col][row]; ends with a barrier.
-template <typename T, int LP = kNative>
+template <typename T, int LP = kNative, int TB = -1>
 __device__ __forceinline__ void tile_product(const T* A, int lda, const T* B, int ldb, bool tb, int m, int n, int k,
                                              TileLds<T>& L, T (*Cs)[kTile + 1])
 {
     typename Mfma<T>::acc_t c[2][2];
-    tile_mma<T, LP>(A, lda, B, ldb, tb, m, n, k, L, c);
+    tile_mma<T, LP, TB>(A, lda, B, ldb, tb, m, n, k, L, c);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int wr = (wv & 1) * 32, wc = (wv >> 1) * 32, q0 = wc + (lane & 15);
 #pragma unroll
@@ -712,7 +727,7 @@ __device__ __forceinline__ void stage_acc(const typename Mfma<T>::acc_t (&c)[2][
 // (TRSM, in-panel update, Schur update), so that profiles tell them apart.
 // The result goes through LDS so the write of C is coalesced along columns
 // (written straight from the accumulators it would be 32-byte pieces).
-template <typename T, int LP = kNative, bool PRE = true>
+template <typename T, int LP = kNative, bool PRE = true, int TB = -1>
 __device__ __forceinline__ void gemm_task(T* pool, const GemmTask& t, TileLds<T>& L, T (*Cs)[kTile + 1],
                                           const int* lidx = nullptr)
 {
@@ -730,7 +745,7 @@ __device__ __forceinline__ void gemm_task(T* pool, const GemmTask& t, TileLds<T>
 #pragma unroll
         for (int j = 0; j < 16; ++j) cpre[j] = C[rc + (int64_t)min((tid >> 6) + 4 * j, t.n - 1) * t.ldc];
     }
-    tile_product<T, LP>(pool + t.a, t.lda, pool + t.b, t.ldb, t.mode == 1, t.m, t.n, t.k, L, Cs);
+    tile_product<T, LP, TB>(pool + t.a, t.lda, pool + t.b, t.ldb, t.mode == 1, t.m, t.n, t.k, L, Cs);
     if (!PRE && t.mode != 1) {
         const int rc = min(row, t.m - 1);
 #pragma unroll
@@ -800,7 +815,7 @@ __device__ __forceinline__ void trsm_syrk(T* pool, const GemmTask& t, TileLds<T>
     __syncthreads();
     const T* R = pool + (t.mode == 2 ? t.a : t.c);
     const int k = t.mode == 2 ? t.k + t.kd : t.k;
-    tile_product<T>(R, t.lda, R, t.lda, false, t.syrk, t.syrk, k, L, Cs);
+    tile_product<T, kNative, 0>(R, t.lda, R, t.lda, false, t.syrk, t.syrk, k, L, Cs);
     T* D = pool + t.cd;
     const int row = threadIdx.x & 63;
     if (row < t.syrk)
@@ -822,14 +837,14 @@ __global__ __launch_bounds__(kThreads, KIND == 0 ? 3 : 4) void k_gemm(T* pool, c
         gemm_task<T>(pool, t, L, Cs);
         if (t.syrk > 0) trsm_syrk<T>(pool, t, L, Cs);
     } else {
-        // modes 0 and 1: the staged result Cs is written after the last read
-        // of the operand staging L, so the two share LDS (four workgroups per
-        // CU instead of three)
+        // modes 0 and 3 (B never transposed): the staged result Cs is written
+        // after the last read of the operand staging L, so the two share LDS
+        // (four workgroups per CU instead of three)
         __shared__ union {
             TileLds<T> L;
             T Cs[kTile][kTile + 1];
         } u;
-        gemm_task<T, LP, KIND != 2>(pool, tasks[blockIdx.x], u.L, u.Cs, lidx);
+        gemm_task<T, LP, KIND != 2, 0>(pool, tasks[blockIdx.x], u.L, u.Cs, lidx);
     }
 }
 
@@ -1688,12 +1703,12 @@ __global__ __launch_bounds__(kThreads) void k_superwv(const T* pool, T* sinv, co
     if (t.j >= 8) {
         const int jv = t.j - 8, rv = c0 + kSuper + 64 * i;
         const int m = max(0, min(64, nd.c - rv));
-        tile_product<T>(F + rv + (int64_t)(c0 + 64 * jv) * n, n, X + kSuper * 64 * jv + 64 * jv, kSuper, true, m, 64,
+        tile_product<T, kNative, 1>(F + rv + (int64_t)(c0 + 64 * jv) * n, n, X + kSuper * 64 * jv + 64 * jv, kSuper, true, m, 64,
                         kSuper - 64 * jv, u.L, u.Cs);
         dst = X + 2 * kSuper * kSuper + 64 * i + kSuper * 64 * jv;
     } else {
         const int jw = t.j - 4, cp = c0 - kSuper;
-        tile_product<T>(X + 64 * i, kSuper, F + c0 + (int64_t)(cp + 64 * jw) * n, n, true, 64, 64, min(64 * (i + 1), sb),
+        tile_product<T, kNative, 1>(X + 64 * i, kSuper, F + c0 + (int64_t)(cp + 64 * jw) * n, n, true, 64, 64, min(64 * (i + 1), sb),
                         u.L, u.Cs);
         dst = X + kSuper * kSuper + 64 * i + kSuper * 64 * jw;
     }
