@@ -573,7 +573,8 @@ __device__ __forceinline__ void tile_mma(const T* A, int lda, const T* B, int ld
         __syncthreads();
         if (k0 + 16 < k) load(k0 + 16);
         if constexpr (LP == kNative) {
-            // the chunk's operands read from LDS together, then its MFMAs
+            // the operands of the chunk's four k-steps, then its 16 MFMAs (the
+            // compiler interleaves the reads with the MFMAs)
             T a0[4], a1[4], b0[4], b1[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
