@@ -648,14 +648,24 @@ __device__ __forceinline__ void tile_mma_ldsA(const T (*Am)[kTile + 1], const T*
     const int li = tid & 63, lp = tid >> 6, tcol = tid >> 4, tkk = tid & 15;
     acc_t c00 = {0, 0, 0, 0}, c01 = {0, 0, 0, 0}, c10 = {0, 0, 0, 0}, c11 = {0, 0, 0, 0};
     for (int k0 = 0; k0 < k; k0 += 16) {
+        // loads from clamped in-bounds positions, all issued before the
+        // masked stores (no branch around each load)
+        T rb[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            if (tb)
+                rb[p] = B[(int64_t)min(tcol + 16 * p, n - 1) * ldb + min(k0 + tkk, k - 1)];
+            else
+                rb[p] = B[min(li, n - 1) + (int64_t)min(k0 + lp + 4 * p, k - 1) * ldb];
+        }
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
             if (tb) {
                 const int col = tcol + 16 * p, kt = k0 + tkk;
-                L.Bs[tkk][col] = (col < n && kt < k) ? B[(int64_t)col * ldb + kt] : T(0);
+                L.Bs[tkk][col] = (col < n && kt < k) ? rb[p] : T(0);
             } else {
                 const int kk = k0 + lp + 4 * p;
-                L.Bs[lp + 4 * p][li] = (li < n && kk < k) ? B[li + (int64_t)kk * ldb] : T(0);
+                L.Bs[lp + 4 * p][li] = (li < n && kk < k) ? rb[p] : T(0);
             }
         }
         __syncthreads();
