@@ -365,9 +365,16 @@ template <typename T>
 __device__ __forceinline__ void factor_dense64(Front64<T>& S, T* g, int64_t ld, int n, int c, int* info, int col0,
                                                T* inv)
 {
-    for (int e = threadIdx.x; e < 64 * 64; e += kThreads) {
-        const int row = e & 63, col = e >> 6;
-        S.A[col][row] = (row < n && col <= row) ? g[row + (int64_t)col * ld] : T(0);
+    // every load from a clamped in-bounds position, all issued before the
+    // masked stores (no branch around each load)
+    T v[16];
+    const int row = threadIdx.x & 63, rc = min(row, n - 1);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = ld_off(g, (unsigned)(rc + min((int)(threadIdx.x >> 6) + 4 * j, n - 1) * ld));
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int col = (threadIdx.x >> 6) + 4 * j;
+        S.A[col][row] = (row < n && col <= row) ? v[j] : T(0);
         S.X[col][row] = T(0);
     }
     __syncthreads();

@@ -184,9 +184,17 @@ SPLITS = (0, 96, 128, 192, 256, 384, 1 << 30)
 CUS = 256
 
 
-def diag_slots(elem_bytes):
+APART_DIAG = 512  # the schedule's kApartDiag
+
+
+def diag_slots(elem_bytes, ntasks=0):
     """Diagonal tasks of one step launch that run at once: one workgroup
-    per CU at the FP64 step kernel's LDS, two at the FP32 one's."""
+    per CU at the FP64 step kernel's LDS, two at the FP32 one's; a step
+    with at least APART_DIAG diagonal tasks runs them in a launch of their
+    own (k_diag: two per CU in FP64, four in FP32, at its registers and
+    LDS)."""
+    if ntasks >= APART_DIAG:
+        return CUS * (2 if elem_bytes == 8 else 4)
     return CUS * (1 if elem_bytes == 8 else 2)
 
 
@@ -315,7 +323,7 @@ def chain_record(ccs_col, ccs_row, max_zero, perm, a, precision, nb=64, batch=No
     s.close()
     n = c + w
     lev = level_of(par, rule)
-    slots = diag_slots(8 if precision == FP64 else 4)
+    eb = 8 if precision == FP64 else 4
     rounds, flops = 0, 0.0
     for L in range(lev.max() + 1):
         big = (lev == L) & (n > 64)
@@ -323,7 +331,8 @@ def chain_record(ccs_col, ccs_row, max_zero, perm, a, precision, nb=64, batch=No
             continue
         blocks = (c[big] + 63) // 64
         for t in range(-1, int(blocks.max()) - 1):
-            rounds += max(1, -(-int((blocks > t + 1).sum()) // slots))
+            nt = int((blocks > t + 1).sum())
+            rounds += max(1, -(-nt // diag_slots(eb, nt)))
         flops += float((w[big].astype(np.float64) ** 2 * c[big]).sum())
     return dict(rounds=rounds, step_us=(kt["potrf"][0] + kt["trsm"][0]) * 1e3, schur_launches=kt["schur"][1],
                 schur_flops=flops, schur_us=kt["schur"][0] * 1e3)
@@ -358,7 +367,6 @@ def choose_kinds(structure, model, tau, schur_us, rate, elem_bytes, max_order, s
     n = c + w
     lev = level_of(par, level_rule)
     kinds = np.zeros(len(c), dtype=np.int32)
-    dslots = diag_slots(elem_bytes)
     big = n > 64
     pred = np.zeros(len(c))
     if big.any():
@@ -386,7 +394,8 @@ def choose_kinds(structure, model, tau, schur_us, rate, elem_bytes, max_order, s
                 b = blocks[:k]
                 rounds = 0
                 for t in range(-1, int(b.max()) - 1):
-                    rounds += max(1, -(-int((b > t + 1).sum()) // dslots))
+                    nt = int((b > t + 1).sum())
+                    rounds += max(1, -(-nt // diag_slots(elem_bytes, nt)))
                 t_chain = rounds * tau + schur_us + flops[k] / rate
             if k == m:
                 t_large = t_med + t_chain
