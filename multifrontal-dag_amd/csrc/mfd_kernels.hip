@@ -1414,12 +1414,22 @@ __device__ __forceinline__ void fwd_chunk_task(const T* pool, double* zpool, int
     for (int j = 0; j < 16; ++j) l[j] = ld_off(Lb, (unsigned)(rowc + min(16 * ty + j, kb - 1) * n));
     for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
         const int nr = min(kRhsGroup, nrhs - r0);
-        // this thread stores right-hand sides r0 + ty and r0 + ty + 4
-        const double z0 = (live && ty < nr) ? Z[row + (int64_t)(r0 + ty) * n] : 0.0;
-        const double z1 = (live && ty + 4 < nr) ? Z[row + (int64_t)(r0 + ty + 4) * n] : 0.0;
-        for (int e = tid; e < 64 * kRhsGroup; e += kThreads) {
-            const int k = e & 63, r = e >> 6;
-            yt[k][r] = (k < kb && r < nr) ? Y[t.k0 + k + (int64_t)(r0 + r) * c] : 0.0;
+        // this thread stores right-hand sides r0 + ty and r0 + ty + 4; the
+        // right-hand side of every load below is uniform over the wave (no
+        // load for the group's missing ones), its row clamped in bounds
+        const double z0 = ty < nr ? Z[rowc + (int64_t)(r0 + ty) * n] : 0.0;
+        const double z1 = ty + 4 < nr ? Z[rowc + (int64_t)(r0 + ty + 4) * n] : 0.0;
+        static_assert(64 * kRhsGroup == 2 * kThreads, "two staged entries per thread");
+        double yv[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int e = tid + kThreads * h, k = e & 63, r = e >> 6;
+            yv[h] = r < nr ? Y[t.k0 + min(k, kb - 1) + (int64_t)(r0 + r) * c] : 0.0;
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int e = tid + kThreads * h, k = e & 63, r = e >> 6;
+            yt[k][r] = (k < kb && r < nr) ? yv[h] : 0.0;
         }
         __syncthreads();
         double acc[kRhsGroup];
@@ -1483,10 +1493,16 @@ __device__ __forceinline__ void fwd_step_task(const T* pool, double* zpool, int 
                 const int e = tid + kThreads * p;
                 LI[e & 63][e >> 6] = (double)li[p];
             }
+            double zv[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int e = tid + kThreads * h, k = e & 63, r = e >> 6;  // r: uniform over the wave
+                zv[h] = r < nr ? Z[min(k, kb - 1) + (int64_t)(r0 + r) * n] : 0.0;
+            }
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int e = tid + kThreads * h, k = e & 63, r = e >> 6;
-                zt[k][r] = (k < kb && r < nr) ? Z[k + (int64_t)(r0 + r) * n] : 0.0;
+                zt[k][r] = (k < kb && r < nr) ? zv[h] : 0.0;
             }
             __syncthreads();
             for (int r = ty; r < kRhsGroup; r += 4) {
@@ -1505,8 +1521,9 @@ __device__ __forceinline__ void fwd_step_task(const T* pool, double* zpool, int 
                 T l[16];
 #pragma unroll
                 for (int j = 0; j < 16; ++j) l[j] = ld_off(Lb, (unsigned)(rowc + min(16 * ty + j, kb - 1) * n));
-                const double z0 = (live && ty < nr) ? Z[row + (int64_t)(r0 + ty) * n] : 0.0;
-                const double z1 = (live && ty + 4 < nr) ? Z[row + (int64_t)(r0 + ty + 4) * n] : 0.0;
+                // rows clamped in bounds (used only where in range); ty: the wave
+                const double z0 = ty < nr ? Z[rowc + (int64_t)(r0 + ty) * n] : 0.0;
+                const double z1 = ty + 4 < nr ? Z[rowc + (int64_t)(r0 + ty + 4) * n] : 0.0;
                 double acc[kRhsGroup];
 #pragma unroll
                 for (int r = 0; r < kRhsGroup; ++r) acc[r] = 0.0;
@@ -1553,13 +1570,16 @@ __device__ __forceinline__ void fwd_step_task(const T* pool, double* zpool, int 
     }
     // the first right-hand-side group's z and y, in flight with the above
     double zv[2], yv[2];
+    // from clamped in-bounds positions (y of block t-1 from entry 0 when
+    // there is none), masked as they are staged
     auto load_zy = [&](int r0) {
         const int nr = min(kRhsGroup, nrhs - r0);
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int e = tid + kThreads * h, k = e & 63, r = e >> 6;
-            zv[h] = (k < kb && r < nr) ? Z[t.k0 + k + (int64_t)(r0 + r) * n] : 0.0;
-            yv[h] = (k < kp && r < nr) ? Y[t.k0 - 64 + k + (int64_t)(r0 + r) * c] : 0.0;
+            const int e = tid + kThreads * h, k = e & 63, r = e >> 6;  // r: uniform over the wave
+            const int64_t rc = r0 + r;
+            zv[h] = r < nr ? Z[t.k0 + min(k, kb - 1) + rc * n] : 0.0;
+            yv[h] = (r < nr && kp > 0) ? Y[t.k0 - 64 + min(k, kp - 1) + rc * c] : 0.0;
         }
     };
     load_zy(0);
@@ -1582,9 +1602,9 @@ __device__ __forceinline__ void fwd_step_task(const T* pool, double* zpool, int 
         if (r0 > 0) load_zy(r0);
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int e = tid + kThreads * h;
-            zt[e & 63][e >> 6] = zv[h];
-            yt[e & 63][e >> 6] = yv[h];
+            const int e = tid + kThreads * h, k = e & 63, r = e >> 6;
+            zt[k][r] = (k < kb && r < nr) ? zv[h] : 0.0;
+            yt[k][r] = (k < kp && r < nr) ? yv[h] : 0.0;
         }
         __syncthreads();
         for (int r = ty; r < nr; r += 4) {
@@ -1614,7 +1634,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(const T* pool, double* zp
 // A forward step's chunk tasks alone (steps with many of them): without the
 // block task's LDS and registers several workgroups share a CU.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_fwd_chunk(const T* pool, double* zpool, int nrhs, const NodeDev* nodes,
+__global__ __launch_bounds__(kThreads, 6) void k_fwd_chunk(const T* pool, double* zpool, int nrhs, const NodeDev* nodes,
                                                         const SolveTask* tasks)
 {
     __shared__ double yt[64][kRhsGroup];
@@ -1762,11 +1782,12 @@ __device__ __forceinline__ void fwd_super_task(const T* pool, const T* sinv, dou
     if (top) {
         const T* X = sinv + t.xoff;
         const int r = t.row0 + i;
-        const bool lower = 16 * g <= t.row0 + 15;  // X is lower triangular
+        // all loads inside the superblock's X and W (W of the first
+        // superblock is unused storage), masked where used
 #pragma unroll
-        for (int q = 0; q < 16; ++q) a[q] = lower ? X[r + kSuper * (16 * g + q)] : T(0);
+        for (int q = 0; q < 16; ++q) a[q] = X[r + kSuper * (16 * g + q)];
 #pragma unroll
-        for (int q = 0; q < 16; ++q) w[q] = pb > 0 ? X[kSuper * kSuper + r + kSuper * (16 * g + q)] : T(0);
+        for (int q = 0; q < 16; ++q) w[q] = X[kSuper * kSuper + r + kSuper * (16 * g + q)];
     } else {
         const T* Lr = pool + nd.front + min(t.row0 + i, n - 1) + (int64_t)cp * n;
 #pragma unroll
@@ -1774,12 +1795,23 @@ __device__ __forceinline__ void fwd_super_task(const T* pool, const T* sinv, dou
 #pragma unroll
         for (int q = 0; q < 16; ++q) a[q] = T(0);
     }
+    const bool lower = 16 * g <= t.row0 + 15;  // X is lower triangular
     for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
         const int nr = min(kRhsGroup, nrhs - r0);
-        for (int e = tid; e < kSuper * kRhsGroup; e += kThreads) {
-            const int m = e & (kSuper - 1), rr = e / kSuper;
-            ys[m][rr] = (m < pb && rr < nr) ? Y[cp + m + (int64_t)(r0 + rr) * c] : 0.0;
-            if (top) zs[m][rr] = (m < sb && rr < nr) ? Z[c0 + m + (int64_t)(r0 + rr) * n] : 0.0;
+        // y of superblock u-1 and z of superblock u from clamped positions
+        // (entry 0 where there is none), all loads before the masked stores
+        static_assert(kSuper == kThreads, "one superblock entry per thread");
+        double yv[kRhsGroup], zv[kRhsGroup];
+#pragma unroll
+        for (int rr = 0; rr < kRhsGroup; ++rr) {
+            // rr < nr, pb > 0 and top are uniform
+            yv[rr] = (rr < nr && pb > 0) ? Y[cp + min(tid, pb - 1) + (int64_t)(r0 + rr) * c] : 0.0;
+            zv[rr] = (rr < nr && top) ? Z[c0 + min(tid, sb - 1) + (int64_t)(r0 + rr) * n] : 0.0;
+        }
+#pragma unroll
+        for (int rr = 0; rr < kRhsGroup; ++rr) {
+            ys[tid][rr] = (tid < pb && rr < nr) ? yv[rr] : 0.0;
+            if (top) zs[tid][rr] = (tid < sb && rr < nr) ? zv[rr] : 0.0;
         }
         __syncthreads();
         double acc[kRhsGroup];
@@ -1788,14 +1820,14 @@ __device__ __forceinline__ void fwd_super_task(const T* pool, const T* sinv, dou
         if (top) {
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
-                const double av = (double)a[q];
+                const double av = lower ? (double)a[q] : 0.0;
 #pragma unroll
                 for (int rr = 0; rr < kRhsGroup; ++rr) acc[rr] += av * zs[16 * g + q][rr];
             }
         }
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
-            const double wv = 16 * g + q < pb ? (double)w[q] : 0.0;
+            const double wv = 16 * g + q < pb ? (double)w[q] : 0.0;  // pb = 0: w unused
 #pragma unroll
             for (int rr = 0; rr < kRhsGroup; ++rr) acc[rr] -= wv * ys[16 * g + q][rr];
         }
@@ -1873,10 +1905,13 @@ __device__ __forceinline__ void bwd_super_task(const T* pool, const T* sinv, dou
 #pragma unroll
     for (int q = 0; q < 16; ++q) a[q] = X[(16 * g + q) + kSuper * col];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = pb > 0 ? V[(16 * g + q) + kSuper * col] : T(0);
+    for (int q = 0; q < 16; ++q) v[q] = V[(16 * g + q) + kSuper * col];  // unused storage when pb = 0
     static_assert(kSuper == kThreads, "one superblock entry per thread");
     const int m = tid;  // the entry of y_u, q_u and x_{u+1} this thread stages
-    const double* pp = part + (int64_t)(t.slot + t.nchunk * (m >> 6)) * 64 * nrhs + (m & 63);
+    // every load below from a clamped in-bounds position (x_{u+1} from entry
+    // 0 when there is none), masked as it is staged
+    const int mc = min(m, sb - 1);
+    const double* pp = part + (int64_t)(t.slot + t.nchunk * (mc >> 6)) * 64 * nrhs + (mc & 63);
     const int64_t cs = (int64_t)64 * nrhs;  // stride between chunks
     for (int r0 = 0; r0 < nrhs; r0 += kRhsGroup) {
         const int nr = min(kRhsGroup, nrhs - r0);
@@ -1885,26 +1920,27 @@ __device__ __forceinline__ void bwd_super_task(const T* pool, const T* sinv, dou
         double zv[kRhsGroup], xv[kRhsGroup];
 #pragma unroll
         for (int rr = 0; rr < kRhsGroup; ++rr) {
-            zv[rr] = (rr < nr && m < sb) ? Y[c0 + m + (int64_t)(r0 + rr) * c] : 0.0;
-            xv[rr] = (rr < nr && m < pb) ? Z[cn + m + (int64_t)(r0 + rr) * n] : 0.0;
+            // rr < nr and pb > 0 are uniform
+            zv[rr] = rr < nr ? Y[c0 + mc + (int64_t)(r0 + rr) * c] : 0.0;
+            xv[rr] = (rr < nr && pb > 0) ? Z[cn + min(m, pb - 1) + (int64_t)(r0 + rr) * n] : 0.0;
         }
-        if (m < sb)
-            for (int ch = 0; ch < t.nchunk; ch += 2) {
-                const bool two = ch + 1 < t.nchunk;
-                double p0[kRhsGroup], p1[kRhsGroup];
+        for (int ch = 0; ch < t.nchunk; ch += 2) {
+            const int ch2 = min(ch + 1, t.nchunk - 1);  // the last chunk twice when odd
+            double p0[kRhsGroup], p1[kRhsGroup];
 #pragma unroll
-                for (int rr = 0; rr < kRhsGroup; ++rr) {
-                    const double* q = pp + (int64_t)(r0 + rr) * 64 + ch * cs;
-                    p0[rr] = rr < nr ? q[0] : 0.0;
-                    p1[rr] = (rr < nr && two) ? q[cs] : 0.0;
-                }
-#pragma unroll
-                for (int rr = 0; rr < kRhsGroup; ++rr) zv[rr] = (zv[rr] - p0[rr]) - p1[rr];
+            for (int rr = 0; rr < kRhsGroup; ++rr) {
+                const double* q = pp + (int64_t)(r0 + rr) * 64;
+                p0[rr] = rr < nr ? q[ch * cs] : 0.0;
+                p1[rr] = rr < nr ? q[ch2 * cs] : 0.0;
             }
+            const bool two = ch + 1 < t.nchunk;
+#pragma unroll
+            for (int rr = 0; rr < kRhsGroup; ++rr) zv[rr] = (zv[rr] - p0[rr]) - (two ? p1[rr] : 0.0);
+        }
 #pragma unroll
         for (int rr = 0; rr < kRhsGroup; ++rr) {
-            zs[m][rr] = zv[rr];
-            xs[m][rr] = xv[rr];
+            zs[m][rr] = (rr < nr && m < sb) ? zv[rr] : 0.0;
+            xs[m][rr] = (rr < nr && m < pb) ? xv[rr] : 0.0;
         }
         __syncthreads();
         double acc[kRhsGroup];
@@ -1912,7 +1948,7 @@ __device__ __forceinline__ void bwd_super_task(const T* pool, const T* sinv, dou
         for (int rr = 0; rr < kRhsGroup; ++rr) acc[rr] = 0.0;
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
-            const double av = (double)a[q], vv = (double)v[q];
+            const double av = (double)a[q], vv = pb > 0 ? (double)v[q] : 0.0;
 #pragma unroll
             for (int rr = 0; rr < kRhsGroup; ++rr) acc[rr] += av * zs[16 * g + q][rr] - vv * xs[16 * g + q][rr];
         }
@@ -2117,20 +2153,23 @@ __device__ __forceinline__ void bwd_step_task(const T* pool, double* zpool, doub
     // right-hand sides (two entries of each per thread); the first group's
     // loads go out with the factor loads above
     double zv[2], xv[2];
+    // (from clamped in-bounds positions, x of block t+1 from entry 0 when
+    // there is none; masked as they are staged)
     auto load_zx = [&](int r0) {
         const int nr = min(kRhsGroup, nrhs - r0);
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int e = tid + kThreads * h, k = e & 63, r = e >> 6;
+            const int e = tid + kThreads * h, k = e & 63, r = e >> 6, kc = min(k, kb - 1);  // r: uniform over the wave
+            const int64_t rc = r0 + r;
             double z = 0.0;
-            if (k < kb && r < nr) {
-                z = Y[t.k0 + k + (int64_t)(r0 + r) * c];
-                const double* pp = part + (int64_t)t.slot * 64 * nrhs + k + (int64_t)(r0 + r) * 64;
+            if (r < nr) {
+                z = Y[t.k0 + kc + rc * c];
+                const double* pp = part + (int64_t)t.slot * 64 * nrhs + kc + rc * 64;
 #pragma unroll 4
                 for (int ch = 0; ch < t.nchunk; ++ch) z -= pp[(int64_t)ch * 64 * nrhs];
             }
             zv[h] = z;
-            xv[h] = (k < kn && r < nr) ? Y[t.k0 + 64 + k + (int64_t)(r0 + r) * c] : 0.0;
+            xv[h] = (r < nr && kn > 0) ? Y[t.k0 + 64 + min(k, kn - 1) + rc * c] : 0.0;
         }
     };
     load_zx(0);
@@ -2152,9 +2191,9 @@ __device__ __forceinline__ void bwd_step_task(const T* pool, double* zpool, doub
         // zt, and x of block t+1 staged in u.b.Xn[0..kn)
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int e = tid + kThreads * h;
-            u.b.zt[e & 63][e >> 6] = zv[h];
-            u.b.Xn[e & 63][e >> 6] = xv[h];
+            const int e = tid + kThreads * h, k = e & 63, r = e >> 6;
+            u.b.zt[k][r] = (k < kb && r < nr) ? zv[h] : 0.0;
+            u.b.Xn[k][r] = (k < kn && r < nr) ? xv[h] : 0.0;
         }
         __syncthreads();
         // z_k -= sum_i L[k0 + 64 + i][k0 + k] x_{t+1}[i]; Ln, LI, Xn and zt
