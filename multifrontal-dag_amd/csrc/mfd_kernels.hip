@@ -763,6 +763,16 @@ __device__ __forceinline__ void gemm_task(T* pool, const GemmTask& t, TileLds<T>
 #pragma unroll
         for (int j = 0; j < 16; ++j) cpre[j] = C[rc + (int64_t)min((tid >> 6) + 4 * j, t.n - 1) * t.ldc];
     }
+    // mode 3 in the instances that load C after the K loop (the Schur
+    // tiles): the parent positions of this thread's row and columns
+    // (local_index), loaded before it so that the parent entries can be read
+    // as soon as it ends (the others read them in the epilogue)
+    int lrow = 1, lcol[16];
+    if (!PRE && t.mode == 3) {
+        lrow = lidx[t.li + min(row, t.m - 1)];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) lcol[j] = lidx[t.lj + min((tid >> 6) + 4 * j, t.n - 1)];
+    }
     tile_product<T, LP, TB>(pool + t.a, t.lda, pool + t.b, t.ldb, t.mode == 1, t.m, t.n, t.k, L, Cs);
     if (!PRE && t.mode != 1) {
         const int rc = min(row, t.m - 1);
@@ -794,7 +804,7 @@ __device__ __forceinline__ void gemm_task(T* pool, const GemmTask& t, TileLds<T>
         // issued before the stores, which may alias them as far as the
         // compiler knows.
         const bool ea = t.mode == 3;
-        T* P = ea ? pool + t.p + (lidx[t.li + row] - 1) : C;
+        T* P = ea ? pool + t.p + ((PRE ? lidx[t.li + row] : lrow) - 1) : C;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int c0 = (tid >> 6) + 16 * k;
@@ -802,10 +812,8 @@ __device__ __forceinline__ void gemm_task(T* pool, const GemmTask& t, TileLds<T>
             T pv[4];
             int64_t po[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int col = min(c0 + 4 * q, t.n - 1);
-                po[q] = ea ? (int64_t)(lidx[t.lj + col] - 1) * t.np : 0;
-            }
+            for (int q = 0; q < 4; ++q)
+                po[q] = ea ? (int64_t)((PRE ? lidx[t.lj + min(c0 + 4 * q, t.n - 1)] : lcol[4 * k + q]) - 1) * t.np : 0;
 #pragma unroll
             for (int q = 0; q < 4; ++q) pv[q] = ea ? P[po[q]] : T(0);
 #pragma unroll
